@@ -1,0 +1,300 @@
+/*
+ * oaph.h -- C ABI of the MI355X (gfx950) implementation of oap's complex
+ * matrix procedures and implicitly restarted Arnoldi eigensolver.
+ *
+ * Every entry point is extern "C", takes plain pointers and sizes, and
+ * returns an oaph_status (0 = success).  Errors never abort the process.
+ * Each declaration cites the reference interface it replaces
+ * (paths relative to the marcinmatula/oap source tree).
+ *
+ * Threading: an oaph_api owns one HIP stream; calls on one oaph_api must not
+ * overlap.  Procedures are enqueued on that stream; calls that return a host
+ * scalar (magnitude, sum, is_upper_triangular, get_value) synchronise it.
+ */
+#ifndef OAPH_H
+#define OAPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* status                                                              */
+/* ------------------------------------------------------------------ */
+typedef int oaph_status;
+#define OAPH_OK 0
+#define OAPH_ERR_ARG 1           /* NULL or inconsistent argument */
+#define OAPH_ERR_SHAPE 2         /* matrix dimensions do not match */
+#define OAPH_ERR_PLANE 3         /* required re/im plane missing */
+#define OAPH_ERR_HIP 4           /* HIP runtime error */
+#define OAPH_ERR_NOMEM 5         /* device allocation failed */
+#define OAPH_ERR_UNSUPPORTED 6   /* option or size not supported */
+#define OAPH_ERR_NOT_CONVERGED 7 /* solver stopped before all pairs converged */
+#define OAPH_ERR_STATE 8         /* call out of order (e.g. no callback set) */
+
+/* Human-readable text for a status; replaces the reference's abort() on
+ * driver errors (oapCuda/KernelExecutor.h:34-41). */
+const char* oaph_status_string(oaph_status s);
+/* Text of the last HIP error seen by the library (thread-local). */
+const char* oaph_last_error(void);
+
+/* ------------------------------------------------------------------ */
+/* data model                                                          */
+/* ------------------------------------------------------------------ */
+/* math::Matrix (oapMatrix/Matrix.h:31-44): element (c, r) at c + r*columns
+ * of each plane; either plane may be NULL.  For device matrices the struct
+ * lives in host memory and the planes in HBM (the reference keeps the
+ * struct itself on the device, oapMatrixCuda/CudaUtils.cpp:175-248). */
+typedef struct oaph_matrix {
+  uint32_t columns;
+  uint32_t rows;
+  uint32_t realColumns;
+  uint32_t realRows;
+  double* reValues;
+  double* imValues;
+} oaph_matrix;
+
+/* math::MatrixInfo (oapMatrix/MatrixInfo.h:16-67) */
+typedef struct oaph_matrix_info {
+  int32_t isRe;
+  int32_t isIm;
+  uint32_t columns;
+  uint32_t rows;
+} oaph_matrix_info;
+
+/* ------------------------------------------------------------------ */
+/* context                                                             */
+/* ------------------------------------------------------------------ */
+/* oap::cuda::Context::create (oapCuda/KernelExecutor.cpp:147-187): selects
+ * the device.  Optional; the first call of any entry point initialises
+ * device 0. */
+oaph_status oaph_init(int device);
+oaph_status oaph_device_synchronize(void);
+/* number of visible devices (0 when no GPU) */
+int oaph_device_count(void);
+
+/* ------------------------------------------------------------------ */
+/* device matrices (oapMatrixCuda/oapCudaMatrixUtils.h:38-74,         */
+/* oapMatrixCuda/CudaUtils.cpp:175-359)                               */
+/* ------------------------------------------------------------------ */
+/* oap::cuda::NewDeviceMatrix(isRe, isIm, columns, rows) */
+oaph_status oaph_matrix_new(oaph_matrix** out, int isRe, int isIm, uint32_t columns,
+                            uint32_t rows);
+/* oap::cuda::DeleteDeviceMatrix; NULL is accepted */
+oaph_status oaph_matrix_free(oaph_matrix* m);
+/* oap::cuda::GetMatrixInfo */
+oaph_status oaph_matrix_info_get(const oaph_matrix* m, oaph_matrix_info* info);
+/* oap::cuda::CopyHostMatrixToDeviceMatrix with plain host arrays (NULL skips a plane) */
+oaph_status oaph_matrix_copy_h2d(oaph_matrix* dst, const double* re, const double* im);
+/* oap::cuda::CopyDeviceMatrixToHostMatrix */
+oaph_status oaph_matrix_copy_d2h(double* re, double* im, const oaph_matrix* src);
+/* oap::cuda::CopyDeviceMatrixToDeviceMatrix (same element count) */
+oaph_status oaph_matrix_copy_d2d(oaph_matrix* dst, const oaph_matrix* src);
+/* CudaUtils::GetReValue/GetImValue, SetReValue/SetImValue (index = c + r*columns) */
+oaph_status oaph_matrix_get_value(const oaph_matrix* m, uint32_t index, double* re, double* im);
+oaph_status oaph_matrix_set_value(oaph_matrix* m, uint32_t index, double re, double im);
+/* CudaUtils::SetZeroRow (oapMatrixCuda/CudaUtils.cpp:332-359) */
+oaph_status oaph_matrix_set_zero_row(oaph_matrix* m, uint32_t row, int re, int im);
+
+/* ------------------------------------------------------------------ */
+/* procedures API: oap::CuProceduresApi (oapMatrixCuda/CuProceduresApi.h:45-309) */
+/* ------------------------------------------------------------------ */
+typedef struct oaph_api oaph_api;
+
+/* CuProceduresApi::CuProceduresApi / ~CuProceduresApi (CuProceduresApi.cpp:64-89) */
+oaph_status oaph_api_create(oaph_api** out);
+oaph_status oaph_api_destroy(oaph_api* api);
+/* waits for all work enqueued on the api's stream */
+oaph_status oaph_api_synchronize(oaph_api* api);
+/* the api's hipStream_t, as void* */
+void* oaph_api_stream(oaph_api* api);
+
+/* dotProduct(out, A, B): C = A*B complex (CuProceduresApi.cpp:91-105) */
+oaph_status oaph_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                             const oaph_matrix* b);
+/* dotProduct(out, A, B, dims[3][2]) (CuProceduresApi.cpp:221-225) */
+oaph_status oaph_dot_product_dims(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                  const oaph_matrix* b, const uint32_t dims[3][2]);
+/* dotProductShared (CuProceduresApi.cpp:107-117): same result as dotProduct */
+oaph_status oaph_dot_product_shared(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                    const oaph_matrix* b);
+/* transpose / conjugateTranspose (CuProceduresApi.cpp:268-307) */
+oaph_status oaph_transpose(oaph_api* api, oaph_matrix* out, const oaph_matrix* a);
+oaph_status oaph_conjugate_transpose(oaph_api* api, oaph_matrix* out, const oaph_matrix* a);
+/* tensorProduct(out, A, B, dims[3][2]) (CuProceduresApi.cpp:153-156); re plane only */
+oaph_status oaph_tensor_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                const oaph_matrix* b, const uint32_t dims[3][2]);
+/* hadamardProduct / hadamardProductVec (CuProceduresApi.cpp:158-192); per plane */
+oaph_status oaph_hadamard_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                  const oaph_matrix* b);
+oaph_status oaph_hadamard_product_vec(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                      const oaph_matrix* b);
+/* add / substract (CuProceduresApi.cpp:309-325) */
+oaph_status oaph_add(oaph_api* api, oaph_matrix* out, const oaph_matrix* a, const oaph_matrix* b);
+oaph_status oaph_substract(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                           const oaph_matrix* b);
+/* multiplyReConstant (re plane only) / multiplyConstant (per plane)
+ * (CuProceduresApi.cpp:492-507) */
+oaph_status oaph_multiply_re_constant(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                      double re);
+oaph_status oaph_multiply_constant(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                   double re, double im);
+/* setDiagonal / setIdentity / setZeroMatrix (CuProceduresApi.cpp:422-437) */
+oaph_status oaph_set_diagonal(oaph_api* api, oaph_matrix* m, double re, double im);
+oaph_status oaph_set_identity(oaph_api* api, oaph_matrix* m);
+oaph_status oaph_set_zero_matrix(oaph_api* api, oaph_matrix* m);
+/* setVector / getVector (CuProceduresApi.cpp:327-340) */
+oaph_status oaph_set_vector(oaph_api* api, oaph_matrix* V, uint32_t column, const oaph_matrix* v,
+                            uint32_t length);
+oaph_status oaph_get_vector(oaph_api* api, oaph_matrix* v, uint32_t length, const oaph_matrix* V,
+                            uint32_t column);
+/* magnitude(floatt&, m) = sqrt(sum re^2 + im^2) (CuProceduresApi.cpp:342-345) */
+oaph_status oaph_magnitude(oaph_api* api, double* out, const oaph_matrix* m);
+/* sum(re&, im&, m) (CuProceduresApi.cpp:358-388) */
+oaph_status oaph_sum(oaph_api* api, double* re, double* im, const oaph_matrix* m);
+/* QRGR(Q, R, H, aux0..aux3) (CuProceduresApi.cpp:439-455); the aux matrices
+ * are accepted for signature parity and may be NULL (not used). */
+oaph_status oaph_qrgr(oaph_api* api, oaph_matrix* Q, oaph_matrix* R, const oaph_matrix* H,
+                      oaph_matrix* aux0, oaph_matrix* aux1, oaph_matrix* aux2,
+                      oaph_matrix* aux3);
+/* isUpperTriangular (CuProceduresApi.cpp:462-470) */
+oaph_status oaph_is_upper_triangular(oaph_api* api, const oaph_matrix* m, int* result);
+/* calcTriangularH(H, Q, R, aux1..aux6) (CuProceduresApi.cpp:472-480); aux may be NULL */
+oaph_status oaph_calc_triangular_h(oaph_api* api, oaph_matrix* H, oaph_matrix* Q, oaph_matrix* R,
+                                   oaph_matrix* aux1, oaph_matrix* aux2, oaph_matrix* aux3,
+                                   oaph_matrix* aux4, oaph_matrix* aux5, oaph_matrix* aux6);
+
+/* utils::mapper::SetThreadsBlocks (oapUtils/ThreadsMapper.cpp:36-74): the
+ * reference's launch-geometry map, kept bit-exact for callers that size
+ * their own grids with it. */
+oaph_status oaph_set_threads_blocks(uint32_t blocks[2], uint32_t threads[2], uint32_t w,
+                                    uint32_t h, uint32_t threads_limit);
+
+/* ------------------------------------------------------------------ */
+/* Arnoldi: CuHArnoldi / CuHArnoldiCallback                            */
+/* (ArnoldiPackage/ArnoldiProcedures.h:33-241, ArnoldiProceduresImpl.h:45-74) */
+/* ------------------------------------------------------------------ */
+typedef struct oaph_arnoldi oaph_arnoldi;
+
+/* oap::VecMultiplicationType (ArnoldiPackage/oapGenericArnoldiApi.h:38-42) */
+#define OAPH_TYPE_EIGENVECTOR 0
+#define OAPH_TYPE_WV 1
+/* oap::QRType (oapGenericArnoldiApi.h:31-36) */
+#define OAPH_QRTYPE_NONE (-1)
+#define OAPH_QRTYPE_QRGR 0
+#define OAPH_QRTYPE_QRHT 1
+/* oap::InitVVectorType (oapGenericArnoldiApi.h:44-48) */
+#define OAPH_INIT_RANDOM 0
+#define OAPH_INIT_FIRST_VALUE_IS_ONE 1
+/* ArnUtils sort functions (ArnoldiPackage/ArnoldiUtils.h:60-72) */
+#define OAPH_SORT_LARGEST_VALUES 0
+#define OAPH_SORT_LARGEST_RE 1
+#define OAPH_SORT_LARGEST_IM 2
+#define OAPH_SORT_SMALLEST_VALUES 3
+#define OAPH_SORT_SMALLEST_RE 4
+#define OAPH_SORT_SMALLEST_IM 5
+/* ArnUtils::CheckType (ArnoldiUtils.h:74-78) */
+#define OAPH_CHECK_INTERNAL 0
+#define OAPH_CHECK_FIRST_STOP 1
+#define OAPH_CHECK_COUNTER 2
+/* ArnUtils::TriangularHProcedureType (ArnoldiUtils.h:80-84) */
+#define OAPH_CALC_IN_DEVICE 0
+#define OAPH_CALC_IN_HOST 1
+#define OAPH_CALC_IN_DEVICE_STEP 2
+/* ArnUtils::Type (ArnoldiUtils.h:86) */
+#define OAPH_UNDEFINED 0
+#define OAPH_DEVICE 1
+#define OAPH_HOST 2
+/* matvec precision (extension, no reference counterpart) */
+#define OAPH_PREC_FP64 0
+#define OAPH_PREC_MIXED_FP32 1 /* FP32 matvec bulk, FP64 polish */
+
+/* CuHArnoldiCallbackBase::MultiplyFunc (ArnoldiProceduresImpl.h:51):
+ * w = A v.  w and v are 1-column device matrices owned by the solver; the
+ * callback may use any oaph_* procedure on `api` (the solver's stream) and
+ * must leave the result enqueued on that stream. */
+typedef void (*oaph_multiply_fn)(oaph_matrix* w, oaph_matrix* v, oaph_api* api, void* user,
+                                 int type);
+/* CuHArnoldiCallbackBase::CheckFunc (ArnoldiProceduresImpl.h:53) */
+typedef int (*oaph_check_fn)(double re, double im, oaph_matrix* vector, uint32_t index,
+                             uint32_t max, void* user);
+
+/* CuHArnoldi::CuHArnoldi / ~CuHArnoldi (ArnoldiProcedures.cpp:35-66) */
+oaph_status oaph_arnoldi_create(oaph_arnoldi** out);
+oaph_status oaph_arnoldi_destroy(oaph_arnoldi* a);
+/* the procedures api the solver runs on (CuHArnoldi::m_cuApi, ArnoldiProcedures.h:81) */
+oaph_api* oaph_arnoldi_api(oaph_arnoldi* a);
+/* setters (ArnoldiProcedures.h:41-62, ArnoldiProcedures.cpp:68-126) */
+oaph_status oaph_arnoldi_set_rho(oaph_arnoldi* a, double rho);
+oaph_status oaph_arnoldi_set_blimit(oaph_arnoldi* a, double blimit);
+oaph_status oaph_arnoldi_set_qr_type(oaph_arnoldi* a, int qrtype);
+oaph_status oaph_arnoldi_set_vec_init_type(oaph_arnoldi* a, int init_type);
+oaph_status oaph_arnoldi_set_sort_type(oaph_arnoldi* a, int sort_type);
+oaph_status oaph_arnoldi_set_check_type(oaph_arnoldi* a, int check_type);
+oaph_status oaph_arnoldi_set_check_counts(oaph_arnoldi* a, uint32_t count);
+/* residual tolerance: a pair is converged when ||A x - l x|| <= tol * norm_a
+ * (the reference stores this value but never reads it, ArnoldiProcedures.cpp:102-105) */
+oaph_status oaph_arnoldi_set_check_tolerance(oaph_arnoldi* a, double tolerance);
+oaph_status oaph_arnoldi_set_outputs_eigenvalues(oaph_arnoldi* a, double* re, double* im);
+oaph_status oaph_arnoldi_set_outputs_eigenvectors(oaph_arnoldi* a, oaph_matrix** vectors);
+oaph_status oaph_arnoldi_set_output_type(oaph_arnoldi* a, int type);
+oaph_status oaph_arnoldi_set_calc_triangular_h_type(oaph_arnoldi* a, int type);
+/* CuHArnoldiCallbackBase::setCallback / setCheckCallback (ArnoldiProceduresImpl.cpp:42-52) */
+oaph_status oaph_arnoldi_set_callback(oaph_arnoldi* a, oaph_multiply_fn fn, void* user);
+oaph_status oaph_arnoldi_set_check_callback(oaph_arnoldi* a, oaph_check_fn fn, void* user);
+/* CuHArnoldiDefault::setMatrix (ArnoldiProceduresImpl.h:32): a dense device
+ * operator used instead of the callback, with the solver's own matvec kernel */
+oaph_status oaph_arnoldi_set_matrix(oaph_arnoldi* a, const oaph_matrix* A);
+/* CuHArnoldiCallback::execute(hdim, wanted, info, type) (ArnoldiProceduresImpl.cpp:54-57,
+ * ArnoldiProcedures.cpp:129-140).  Returns OAPH_ERR_NOT_CONVERGED when the
+ * restart cap is hit; outputs then hold the best available pairs. */
+oaph_status oaph_arnoldi_execute(oaph_arnoldi* a, uint32_t hdim, uint32_t wanted,
+                                 const oaph_matrix_info* info, int type);
+/* CuHArnoldi::testOutcome(index) (ArnoldiProcedures.cpp:642-649, 521-542):
+ * ||A x - lambda x||_2 with the full complex lambda */
+oaph_status oaph_arnoldi_test_outcome(oaph_arnoldi* a, uint32_t index, double* out);
+
+/* extensions (no reference counterpart) */
+oaph_status oaph_arnoldi_set_seed(oaph_arnoldi* a, uint64_t seed);
+oaph_status oaph_arnoldi_set_norm_a(oaph_arnoldi* a, double norm_a);
+oaph_status oaph_arnoldi_set_max_restarts(oaph_arnoldi* a, uint32_t max_restarts);
+oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* a, int precision);
+/* start vector for the next execute (n complex values, NULL im = real) */
+oaph_status oaph_arnoldi_set_start_vector(oaph_arnoldi* a, const double* re, const double* im,
+                                          uint32_t n);
+typedef struct oaph_arnoldi_stats {
+  uint32_t restarts;
+  uint32_t matvecs;
+  uint32_t matvecs_low; /* matvecs done in reduced precision */
+  uint32_t nconv;
+  double seconds;       /* wall time of execute */
+  double max_resid;     /* max explicit ||A x - l x|| over the wanted pairs */
+} oaph_arnoldi_stats;
+oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
+
+/* Average duration (HIP events on the solver's stream) of the solver's own
+ * operator application for the dense operator set with
+ * oaph_arnoldi_set_matrix: FP64 (low = 0) or the FP32 copy (low = 1). */
+oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* a, int low, uint32_t iters, double* avg_ms);
+
+/* ------------------------------------------------------------------ */
+/* measurement support (no reference counterpart)                      */
+/* ------------------------------------------------------------------ */
+/* Seeded synthetic square operator written in place (counter-based
+ * SplitMix64 + Box-Muller):
+ *   kind 0: A = diag(1/(1+i)) + p1 G / sqrt(n), real (im plane zeroed if present)
+ *   kind 1: A = diag(exp(-k/p2) e^{2 pi i u_k}) + p1 (G_re + i G_im) / sqrt(n) */
+oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1, double p2);
+/* Average duration of `iters` back-to-back dotProduct calls (HIP events on
+ * the api's stream), after one warm-up call. */
+oaph_status oaph_time_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                  const oaph_matrix* b, uint32_t iters, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* OAPH_H */

@@ -1,0 +1,495 @@
+"""Python mirror of oap's procedures API over the C ABI.
+
+`CuProceduresApi` keeps the reference's method names and argument order
+(oapMatrixCuda/CuProceduresApi.h:45-309); matrices are `DeviceMatrix`
+handles (math::Matrix with planes in HBM).  Scalars come back as return
+values instead of `floatt&` out-parameters.  Every call goes through
+liboaph.so; a non-zero status raises OapError.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from ._lib import DIMS, DP, MP, U32, OapError, check, load, oaph_matrix, oaph_matrix_info
+
+
+def _dptr(a: np.ndarray | None):
+    if a is None:
+        return None
+    return a.ctypes.data_as(DP)
+
+
+class DeviceMatrix:
+    """A device matrix: math::Matrix with its re/im planes in HBM.
+
+    Mirrors oap::cuda::NewDeviceMatrix / DeleteDeviceMatrix
+    (oapMatrixCuda/oapCudaMatrixUtils.h:38-74).
+    """
+
+    def __init__(self, columns: int, rows: int, is_re: bool = True, is_im: bool = True):
+        lib = load()
+        self._p = MP()
+        check(lib.oaph_matrix_new(C.byref(self._p), int(is_re), int(is_im), columns, rows),
+              "oaph_matrix_new")
+
+    # ---- construction helpers
+    @classmethod
+    def from_numpy(cls, re: np.ndarray | None, im: np.ndarray | None = None) -> "DeviceMatrix":
+        """re/im are (rows, columns) arrays; either may be None (absent plane)."""
+        ref = re if re is not None else im
+        if ref is None:
+            raise ValueError("need at least one plane")
+        ref = np.asarray(ref)
+        if ref.ndim == 1:
+            ref = ref.reshape(-1, 1)
+        rows, cols = ref.shape
+        m = cls(cols, rows, re is not None, im is not None)
+        m.upload(re, im)
+        return m
+
+    @classmethod
+    def zeros_like_info(cls, info) -> "DeviceMatrix":
+        return cls(info.columns, info.rows, bool(info.isRe), bool(info.isIm))
+
+    def upload(self, re: np.ndarray | None, im: np.ndarray | None = None) -> None:
+        r = None if re is None else np.ascontiguousarray(np.asarray(re, dtype=np.float64).reshape(-1))
+        i = None if im is None else np.ascontiguousarray(np.asarray(im, dtype=np.float64).reshape(-1))
+        for a in (r, i):
+            if a is not None and a.size != self.columns * self.rows:
+                raise ValueError("plane size does not match the matrix")
+        check(load().oaph_matrix_copy_h2d(self._p, _dptr(r), _dptr(i)), "oaph_matrix_copy_h2d")
+
+    def download(self) -> tuple[np.ndarray | None, np.ndarray | None]:
+        n = self.columns * self.rows
+        re = np.empty(n) if self.is_re else None
+        im = np.empty(n) if self.is_im else None
+        check(load().oaph_matrix_copy_d2h(_dptr(re), _dptr(im), self._p), "oaph_matrix_copy_d2h")
+        shape = (self.rows, self.columns)
+        return (None if re is None else re.reshape(shape), None if im is None else im.reshape(shape))
+
+    def to_complex(self) -> np.ndarray:
+        re, im = self.download()
+        z = np.zeros((self.rows, self.columns), dtype=np.complex128)
+        if re is not None:
+            z.real = re
+        if im is not None:
+            z.imag = im
+        return z
+
+    # ---- info
+    @property
+    def ptr(self):
+        return self._p
+
+    @property
+    def columns(self) -> int:
+        return self._p.contents.columns
+
+    @property
+    def rows(self) -> int:
+        return self._p.contents.rows
+
+    @property
+    def is_re(self) -> bool:
+        return bool(self._p.contents.reValues)
+
+    @property
+    def is_im(self) -> bool:
+        return bool(self._p.contents.imValues)
+
+    def info(self) -> oaph_matrix_info:
+        inf = oaph_matrix_info()
+        check(load().oaph_matrix_info_get(self._p, C.byref(inf)), "oaph_matrix_info_get")
+        return inf
+
+    def get_value(self, index: int) -> complex:
+        re, im = C.c_double(), C.c_double()
+        check(load().oaph_matrix_get_value(self._p, index, C.byref(re), C.byref(im)),
+              "oaph_matrix_get_value")
+        return complex(re.value, im.value)
+
+    def set_value(self, index: int, re: float, im: float = 0.0) -> None:
+        check(load().oaph_matrix_set_value(self._p, index, re, im), "oaph_matrix_set_value")
+
+    def free(self) -> None:
+        if getattr(self, "_p", None):
+            load().oaph_matrix_free(self._p)
+            self._p = MP()
+
+    def __del__(self):  # pragma: no cover - interpreter teardown order varies
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dims(d) -> DIMS:
+    arr = DIMS()
+    for i in range(3):
+        for j in range(2):
+            arr[i][j] = int(d[i][j])
+    return arr
+
+
+def _m(x):
+    return None if x is None else x.ptr
+
+
+class CuProceduresApi:
+    """oap::CuProceduresApi (oapMatrixCuda/CuProceduresApi.h:45-309) over liboaph.so."""
+
+    def __init__(self, handle: int | None = None):
+        self._lib = load()
+        self._owned = handle is None
+        if handle is None:
+            h = C.c_void_p()
+            check(self._lib.oaph_api_create(C.byref(h)), "oaph_api_create")
+            self._h = h
+        else:
+            self._h = C.c_void_p(handle)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def synchronize(self) -> None:
+        check(self._lib.oaph_api_synchronize(self._h), "oaph_api_synchronize")
+
+    def destroy(self) -> None:
+        if self._owned and self._h:
+            self._lib.oaph_api_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    # ---- products
+    def dotProduct(self, out, a, b, dims=None):
+        if dims is None:
+            check(self._lib.oaph_dot_product(self._h, out.ptr, a.ptr, b.ptr), "dotProduct")
+        else:
+            check(self._lib.oaph_dot_product_dims(self._h, out.ptr, a.ptr, b.ptr, _dims(dims)),
+                  "dotProduct(dims)")
+
+    def dotProductShared(self, out, a, b):
+        check(self._lib.oaph_dot_product_shared(self._h, out.ptr, a.ptr, b.ptr), "dotProductShared")
+
+    def tensorProduct(self, out, a, b, dims=None):
+        if dims is None:
+            dims = [[out.columns, out.rows], [a.columns, a.rows], [b.columns, b.rows]]
+        check(self._lib.oaph_tensor_product(self._h, out.ptr, a.ptr, b.ptr, _dims(dims)),
+              "tensorProduct")
+
+    def hadamardProduct(self, out, a, b):
+        check(self._lib.oaph_hadamard_product(self._h, out.ptr, a.ptr, b.ptr), "hadamardProduct")
+
+    def hadamardProductVec(self, out, a, b):
+        check(self._lib.oaph_hadamard_product_vec(self._h, out.ptr, a.ptr, b.ptr),
+              "hadamardProductVec")
+
+    # ---- element-wise
+    def transpose(self, out, a):
+        check(self._lib.oaph_transpose(self._h, out.ptr, a.ptr), "transpose")
+
+    def conjugateTranspose(self, out, a):
+        check(self._lib.oaph_conjugate_transpose(self._h, out.ptr, a.ptr), "conjugateTranspose")
+
+    def add(self, out, a, b):
+        check(self._lib.oaph_add(self._h, out.ptr, a.ptr, b.ptr), "add")
+
+    def substract(self, out, a, b):
+        check(self._lib.oaph_substract(self._h, out.ptr, a.ptr, b.ptr), "substract")
+
+    def multiplyReConstant(self, out, a, re: float):
+        check(self._lib.oaph_multiply_re_constant(self._h, out.ptr, a.ptr, re),
+              "multiplyReConstant")
+
+    def multiplyConstant(self, out, a, re: float, im: float):
+        check(self._lib.oaph_multiply_constant(self._h, out.ptr, a.ptr, re, im), "multiplyConstant")
+
+    def setDiagonal(self, m, re: float, im: float):
+        check(self._lib.oaph_set_diagonal(self._h, m.ptr, re, im), "setDiagonal")
+
+    def setIdentity(self, m):
+        check(self._lib.oaph_set_identity(self._h, m.ptr), "setIdentity")
+
+    def setZeroMatrix(self, m):
+        check(self._lib.oaph_set_zero_matrix(self._h, m.ptr), "setZeroMatrix")
+
+    def setVector(self, V, column: int, v, length: int):
+        check(self._lib.oaph_set_vector(self._h, V.ptr, column, v.ptr, length), "setVector")
+
+    def getVector(self, v, length: int, V, column: int):
+        check(self._lib.oaph_get_vector(self._h, v.ptr, length, V.ptr, column), "getVector")
+
+    # ---- reductions
+    def magnitude(self, m) -> float:
+        out = C.c_double()
+        check(self._lib.oaph_magnitude(self._h, C.byref(out), m.ptr), "magnitude")
+        return out.value
+
+    def sum(self, m) -> tuple[float, float]:
+        re, im = C.c_double(), C.c_double()
+        check(self._lib.oaph_sum(self._h, C.byref(re), C.byref(im), m.ptr), "sum")
+        return re.value, im.value
+
+    # ---- small dense
+    def QRGR(self, Q, R, H, aux0=None, aux1=None, aux2=None, aux3=None):
+        check(self._lib.oaph_qrgr(self._h, Q.ptr, R.ptr, H.ptr, _m(aux0), _m(aux1), _m(aux2),
+                                  _m(aux3)), "QRGR")
+
+    def isUpperTriangular(self, m) -> bool:
+        r = C.c_int()
+        check(self._lib.oaph_is_upper_triangular(self._h, m.ptr, C.byref(r)), "isUpperTriangular")
+        return r.value == 1
+
+    def calcTriangularH(self, H, Q, R=None, *aux):
+        a = list(aux) + [None] * (6 - len(aux))
+        check(self._lib.oaph_calc_triangular_h(self._h, H.ptr, Q.ptr, _m(R), *[_m(x) for x in a]),
+              "calcTriangularH")
+
+
+def set_threads_blocks(w: int, h: int, limit: int = 1024):
+    """utils::mapper::SetThreadsBlocks (oapUtils/ThreadsMapper.cpp:36-74) -> (blocks, threads)."""
+    lib = load()
+    b = (U32 * 2)()
+    t = (U32 * 2)()
+    check(lib.oaph_set_threads_blocks(b, t, w, h, limit), "SetThreadsBlocks")
+    return (b[0], b[1]), (t[0], t[1])
+
+
+__all__ = ["DeviceMatrix", "CuProceduresApi", "OapError", "set_threads_blocks", "oaph_matrix"]
+
+
+# ---------------------------------------------------------------------------
+# Arnoldi: CuHArnoldi / CuHArnoldiCallback (ArnoldiPackage/ArnoldiProcedures.h:33-241,
+# ArnoldiProceduresImpl.h:45-74)
+# ---------------------------------------------------------------------------
+class QRType:
+    """oap::QRType (ArnoldiPackage/oapGenericArnoldiApi.h:31-36)"""
+    NONE, QRGR, QRHT = -1, 0, 1
+
+
+class VecMultiplicationType:
+    """oap::VecMultiplicationType (oapGenericArnoldiApi.h:38-42)"""
+    TYPE_EIGENVECTOR, TYPE_WV = 0, 1
+
+
+class InitVVectorType:
+    """oap::InitVVectorType (oapGenericArnoldiApi.h:44-48)"""
+    RANDOM, FIRST_VALUE_IS_ONE = 0, 1
+
+
+class ArnUtils:
+    """ArnUtils sort functions, CheckType, TriangularHProcedureType, Type (ArnoldiUtils.h:58-87)"""
+    SortLargestValues, SortLargestReValues, SortLargestImValues = 0, 1, 2
+    SortSmallestValues, SortSmallestReValues, SortSmallestImValues = 3, 4, 5
+    CHECK_INTERNAL, CHECK_FIRST_STOP, CHECK_COUNTER = 0, 1, 2
+    CALC_IN_DEVICE, CALC_IN_HOST, CALC_IN_DEVICE_STEP = 0, 1, 2
+    UNDEFINED, DEVICE, HOST = 0, 1, 2
+
+
+class Precision:
+    FP64, MIXED_FP32 = 0, 1
+
+
+class MatrixInfo:
+    """math::MatrixInfo (oapMatrix/MatrixInfo.h:16-67)."""
+
+    def __init__(self, is_re: bool, is_im: bool, columns: int, rows: int):
+        self.c = oaph_matrix_info(int(is_re), int(is_im), columns, rows)
+
+    @property
+    def rows(self):
+        return self.c.rows
+
+
+class CuHArnoldiCallback:
+    """CuHArnoldiCallback over liboaph.so.
+
+    multiply(w, v, api, user_data, mt): w = A v, where w and v are
+    single-column DeviceMatrixView objects owned by the solver and api is the
+    solver's CuProceduresApi (same stream).  Set either a callback or a dense
+    device operator (setMatrix, the CuHArnoldiDefault behaviour).
+    """
+
+    def __init__(self):
+        from ._lib import CHECK_FN, MULTIPLY_FN
+        self._lib = load()
+        h = C.c_void_p()
+        check(self._lib.oaph_arnoldi_create(C.byref(h)), "oaph_arnoldi_create")
+        self._h = h
+        self.api = CuProceduresApi(self._lib.oaph_arnoldi_api(h))
+        self._MULTIPLY_FN = MULTIPLY_FN
+        self._CHECK_FN = CHECK_FN
+        self._cb = None
+        self._ccb = None
+        self._outs = None
+        self._error = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            if self._h:
+                self._lib.oaph_arnoldi_destroy(self._h)
+                self._h = C.c_void_p()
+        except Exception:
+            pass
+
+    # ---- setters (ArnoldiProcedures.h:41-62)
+    def setRho(self, rho=1.0 / 3.14):
+        check(self._lib.oaph_arnoldi_set_rho(self._h, rho), "setRho")
+
+    def setBLimit(self, blimit):
+        check(self._lib.oaph_arnoldi_set_blimit(self._h, blimit), "setBLimit")
+
+    def setQRType(self, qrtype=QRType.QRGR):
+        check(self._lib.oaph_arnoldi_set_qr_type(self._h, qrtype), "setQRType")
+
+    def setVecInitType(self, t):
+        check(self._lib.oaph_arnoldi_set_vec_init_type(self._h, t), "setVecInitType")
+
+    def setSortType(self, t):
+        check(self._lib.oaph_arnoldi_set_sort_type(self._h, t), "setSortType")
+
+    def setCheckType(self, t):
+        check(self._lib.oaph_arnoldi_set_check_type(self._h, t), "setCheckType")
+
+    def setCheckCounts(self, n):
+        check(self._lib.oaph_arnoldi_set_check_counts(self._h, n), "setCheckCounts")
+
+    def setCheckTolerance(self, tol):
+        check(self._lib.oaph_arnoldi_set_check_tolerance(self._h, tol), "setCheckTolerance")
+
+    def setOutputType(self, t):
+        check(self._lib.oaph_arnoldi_set_output_type(self._h, t), "setOutputType")
+
+    def setCalcTraingularHType(self, t):
+        check(self._lib.oaph_arnoldi_set_calc_triangular_h_type(self._h, t),
+              "setCalcTraingularHType")
+
+    # extensions
+    def setSeed(self, seed):
+        check(self._lib.oaph_arnoldi_set_seed(self._h, seed), "setSeed")
+
+    def setNormA(self, norm):
+        check(self._lib.oaph_arnoldi_set_norm_a(self._h, norm), "setNormA")
+
+    def setMaxRestarts(self, n):
+        check(self._lib.oaph_arnoldi_set_max_restarts(self._h, n), "setMaxRestarts")
+
+    def setPrecision(self, p):
+        check(self._lib.oaph_arnoldi_set_precision(self._h, p), "setPrecision")
+
+    def setStartVector(self, v):
+        if v is None:
+            check(self._lib.oaph_arnoldi_set_start_vector(self._h, None, None, 0), "setStartVector")
+            return
+        v = np.asarray(v)
+        re = np.ascontiguousarray(v.real, dtype=np.float64)
+        im = np.ascontiguousarray(v.imag, dtype=np.float64)
+        check(self._lib.oaph_arnoldi_set_start_vector(self._h, _dptr(re), _dptr(im), v.size),
+              "setStartVector")
+
+    def setMatrix(self, A: DeviceMatrix):
+        self._A = A
+        check(self._lib.oaph_arnoldi_set_matrix(self._h, A.ptr), "setMatrix")
+
+    def setCallback(self, fn, user_data=None):
+        """fn(w, v, api, user_data, mt) with w, v DeviceMatrixView."""
+        if fn is None:
+            self._cb = None
+            check(self._lib.oaph_arnoldi_set_callback(self._h, self._MULTIPLY_FN(), None),
+                  "setCallback")
+            return
+
+        def tramp(wp, vp, _api, _u, mt):
+            try:
+                fn(DeviceMatrixView(wp), DeviceMatrixView(vp), self.api, user_data, mt)
+            except Exception as e:  # surface after execute returns
+                self._error = e
+
+        self._cb = self._MULTIPLY_FN(tramp)
+        check(self._lib.oaph_arnoldi_set_callback(self._h, self._cb, None), "setCallback")
+
+    def setCheckCallback(self, fn, user_data=None):
+        def tramp(re, im, vp, index, mx, _u):
+            return int(bool(fn(re, im, DeviceMatrixView(vp), index, mx, user_data)))
+
+        self._ccb = self._CHECK_FN(tramp)
+        check(self._lib.oaph_arnoldi_set_check_callback(self._h, self._ccb, None),
+              "setCheckCallback")
+
+    def setOutputsEigenvalues(self, re: np.ndarray, im: np.ndarray):
+        self._ev = (re, im)
+        check(self._lib.oaph_arnoldi_set_outputs_eigenvalues(self._h, _dptr(re), _dptr(im)),
+              "setOutputsEigenvalues")
+
+    def setOutputsEigenvectors(self, vectors):
+        """vectors: list of DeviceMatrix (n x 1) or None."""
+        arr = (MP * len(vectors))(*[v.ptr if v is not None else MP() for v in vectors])
+        self._outs = (arr, vectors)
+        check(self._lib.oaph_arnoldi_set_outputs_eigenvectors(self._h, arr),
+              "setOutputsEigenvectors")
+
+    # ---- run
+    def execute(self, hdim: int, wanted: int, info: MatrixInfo, mtype=ArnUtils.DEVICE,
+                raise_on_not_converged=True) -> int:
+        self._error = None
+        st = self._lib.oaph_arnoldi_execute(self._h, hdim, wanted, C.byref(info.c), mtype)
+        if self._error is not None:
+            raise self._error
+        if st == 7 and not raise_on_not_converged:
+            return st
+        check(st, "execute")
+        return st
+
+    def testOutcome(self, index: int) -> float:
+        out = C.c_double()
+        check(self._lib.oaph_arnoldi_test_outcome(self._h, index, C.byref(out)), "testOutcome")
+        return out.value
+
+    def stats(self) -> dict:
+        from ._lib import oaph_arnoldi_stats
+        s = oaph_arnoldi_stats()
+        check(self._lib.oaph_arnoldi_get_stats(self._h, C.byref(s)), "get_stats")
+        return {f: getattr(s, f) for f, _ in s._fields_}
+
+
+class DeviceMatrixView:
+    """A non-owning view of a solver-owned device matrix (the w/v of the callback)."""
+
+    def __init__(self, p):
+        self._p = p
+
+    @property
+    def ptr(self):
+        return self._p
+
+    @property
+    def columns(self):
+        return self._p.contents.columns
+
+    @property
+    def rows(self):
+        return self._p.contents.rows
+
+    @property
+    def is_re(self):
+        return bool(self._p.contents.reValues)
+
+    @property
+    def is_im(self):
+        return bool(self._p.contents.imValues)
+
+    download = DeviceMatrix.download
+    upload = DeviceMatrix.upload
+    to_complex = DeviceMatrix.to_complex
+
+
+__all__ += ["CuHArnoldiCallback", "MatrixInfo", "QRType", "VecMultiplicationType",
+            "InitVVectorType", "ArnUtils", "Precision", "DeviceMatrixView"]

@@ -1,0 +1,642 @@
+// arnoldi.hip -- host driver of the implicitly restarted Arnoldi solver and
+// the C ABI of CuHArnoldi / CuHArnoldiCallback
+// (ArnoldiPackage/ArnoldiProcedures.h:33-241, ArnoldiProceduresImpl.h:45-74).
+//
+// Step sequence (restated from CuHArnoldi::execute/step,
+// ArnoldiProcedures.cpp:129-215, with the fixes that let it reach FP64
+// residuals; see the oracle's orc_iram for the CPU restatement):
+//   v0 -> [ w = A v_j (callback or dense operator);  CGS2 against V[:, 0..j];
+//           H[:, j] = h;  beta = ||f||;  v_{j+1} = f / beta ]  j = start..m-1
+//   -> one-wave restart kernel: Schur form of H, Ritz estimates
+//      beta |e_m^T y|, wanted order, exact shifts -> (H+, Q)
+//   -> stop when the k wanted estimates are <= tol * norm_a
+//   -> V <- V Q[:, 0..kk), f <- V Q[:, kk] H+[kk][kk-1] + f Q[m-1][kk-1]
+// The host waits on the device once per restart cycle (to read the status);
+// inside a cycle everything is enqueued on the solver's stream.
+//
+// Mixed precision (OAPH_PREC_MIXED_FP32, dense operator only): the cycles run
+// with an FP32 copy of the operator until the estimates reach the FP32 floor,
+// then the solver restarts in FP64 from the sum of the wanted Ritz vectors
+// and polishes to the requested tolerance; every returned pair is checked
+// with an explicit FP64 residual.
+#include <chrono>
+#include <vector>
+
+#include "iram.h"
+
+using namespace oaph;
+using namespace oaph::iram;
+
+struct oaph_arnoldi {
+  oaph_api* api = nullptr;
+  // settings (defaults: ArnoldiProcedures.cpp:35-53)
+  double rho = 1.0 / 3.14;  // DGKS threshold; CGS2 always re-orthogonalises
+  double blimit = 1e-5;
+  int qrtype = OAPH_QRTYPE_NONE;
+  int init = OAPH_INIT_RANDOM;
+  int sort = OAPH_SORT_SMALLEST_VALUES;
+  int check = OAPH_CHECK_INTERNAL;
+  uint32_t check_counts = 0;
+  double tol = 1e-10;
+  int out_type = OAPH_UNDEFINED;
+  int tri_type = OAPH_CALC_IN_HOST;
+  uint64_t seed = 0x5EED5EEDull;
+  double norm_a = 0.0;  // 0: estimate
+  bool norm_a_user = false;
+  uint32_t max_restarts = 1000;
+  int precision = OAPH_PREC_FP64;
+  oaph_multiply_fn mfn = nullptr;
+  void* muser = nullptr;
+  oaph_check_fn cfn = nullptr;
+  void* cuser = nullptr;
+  const oaph_matrix* A = nullptr;
+  double* out_re = nullptr;
+  double* out_im = nullptr;
+  oaph_matrix** out_vecs = nullptr;
+  std::vector<double> start_re, start_im;
+  // state
+  uint32_t n = 0, m = 0, k = 0;
+  bool cplx = true;
+  DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32;
+  size_t a32_elems = 0;
+  const oaph_matrix* a32_of = nullptr;
+  Status host_status{};
+  std::vector<double> ev_re, ev_im, resid;
+  oaph_arnoldi_stats stats{};
+  bool executed = false;
+};
+
+namespace {
+
+oaph_status alloc(oaph_arnoldi* s) {
+  const size_t n = s->n, m = s->m, k = s->k;
+  OAPH_TRY(s->V.reserve(2 * (m + 1) * n * sizeof(double)));
+  OAPH_TRY(s->V2.reserve(2 * (m + 1) * n * sizeof(double)));
+  OAPH_TRY(s->W.reserve(2 * n * sizeof(double)));
+  OAPH_TRY(s->F.reserve(2 * n * sizeof(double)));
+  OAPH_TRY(s->H.reserve(2 * m * m * sizeof(double)));
+  OAPH_TRY(s->Q.reserve(4 * m * m * sizeof(double)));  // Q and the X work of the restart kernel
+  OAPH_TRY(s->Y.reserve(2 * m * k * sizeof(double)));
+  OAPH_TRY(s->X.reserve(2 * k * n * sizeof(double)));
+  OAPH_TRY(s->AX.reserve(2 * k * n * sizeof(double)));
+  OAPH_TRY(s->coef.reserve(6 * (m + 1) * sizeof(double2)));
+  size_t pb = orth_partials_bytes(s->n, s->m);
+  size_t pp = pairs_partials_bytes(s->n, s->k);
+  OAPH_TRY(s->partials.reserve(pb > pp ? pb : pp));
+  if (!s->counter.ptr) {
+    OAPH_TRY(s->counter.reserve(256));
+    OAPH_HIP(hipMemset(s->counter.ptr, 0, 256));
+  }
+  OAPH_TRY(s->status.reserve(sizeof(Status)));
+  return OAPH_OK;
+}
+
+// views of the solver's planes
+inline double* Vre(oaph_arnoldi* s, DevBuf& b, uint32_t c) {
+  return b.as<double>() + (size_t)c * s->n;
+}
+inline double* Vim(oaph_arnoldi* s, DevBuf& b, uint32_t c) {
+  return b.as<double>() + (size_t)(s->m + 1) * s->n + (size_t)c * s->n;
+}
+
+Work work_of(oaph_arnoldi* s) {
+  Work w;
+  w.n = s->n;
+  w.m = s->m;
+  w.Vre = Vre(s, s->V, 0);
+  w.Vim = Vim(s, s->V, 0);
+  w.Hre = s->H.as<double>();
+  w.Him = s->H.as<double>() + (size_t)s->m * s->m;
+  w.partials = s->partials.as<double>();
+  w.counter = s->counter.as<unsigned>();
+  w.st = s->status.as<Status>();
+  return w;
+}
+
+// y = A x with x, y n-vectors (complex planes); counts the matvec
+oaph_status apply_operator(oaph_arnoldi* s, double* xre, double* xim, double* yre, double* yim,
+                           int type, bool low) {
+  hipStream_t st = s->api->stream;
+  if (s->A) {
+    const oaph_matrix* A = s->A;
+    if (low) {
+      const float* a32 = s->A32.as<float>();
+      OAPH_TRY(launch_cgemv32(st, s->n, s->n, A->columns, a32, a32 + s->a32_elems, xre, xim, yre,
+                              yim));
+      s->stats.matvecs_low++;
+    } else {
+      OAPH_TRY(launch_zgemv(st, s->n, s->n, A->columns, A->reValues, A->imValues, xre, xim, yre,
+                            yim));
+    }
+  } else {
+    OAPH_CHECK(s->mfn != nullptr, OAPH_ERR_STATE);
+    oaph_matrix w{1, s->n, 1, s->n, yre, yim};
+    oaph_matrix v{1, s->n, 1, s->n, xre, xim};
+    s->mfn(&w, &v, s->api, s->muser, type);
+    OAPH_HIP(hipGetLastError());
+  }
+  s->stats.matvecs++;
+  return OAPH_OK;
+}
+
+oaph_status read_status(oaph_arnoldi* s) {
+  OAPH_HIP(hipMemcpyAsync(&s->host_status, s->status.ptr, sizeof(Status), hipMemcpyDeviceToHost,
+                          s->api->stream));
+  OAPH_HIP(hipStreamSynchronize(s->api->stream));
+  return OAPH_OK;
+}
+
+// One Arnoldi step: column j of H and v_{j+1} (CGS2; ArnoldiProcedures.cpp:416-459)
+oaph_status arnoldi_step(oaph_arnoldi* s, uint32_t j, bool low) {
+  hipStream_t st = s->api->stream;
+  Work w = work_of(s);
+  double* wre = s->W.as<double>();
+  double* wim = wre + s->n;
+  double* fre = s->F.as<double>();
+  double* fim = fre + s->n;
+  double2* c1 = s->coef.as<double2>();
+  double2* c2 = c1 + (s->m + 1);
+  OAPH_TRY(apply_operator(s, Vre(s, s->V, j), Vim(s, s->V, j), wre, wim, OAPH_TYPE_WV, low));
+  const uint32_t nc = j + 1;
+  OAPH_TRY(orth(st, 0, w, nc, wre, wim, nullptr, nullptr, nullptr, nullptr, c1, 0, 0));
+  OAPH_TRY(orth(st, 1, w, nc, wre, wim, fre, fim, c1, nullptr, c2, 0, 0));
+  OAPH_TRY(orth(st, 2, w, nc, fre, fim, fre, fim, c2, c1, nullptr, (int)j, (int)j + 1));
+  if (j + 1 < s->m) OAPH_TRY(normalize(st, w, fre, fim, Vre(s, s->V, j + 1), Vim(s, s->V, j + 1)));
+  return OAPH_OK;
+}
+
+// orthonormalise F against V[:, 0..nc) and store it as column nc (breakdown restart)
+oaph_status fresh_column(oaph_arnoldi* s, uint32_t nc, uint64_t seed) {
+  hipStream_t st = s->api->stream;
+  Work w = work_of(s);
+  double* fre = s->F.as<double>();
+  double* fim = fre + s->n;
+  double* wre = s->W.as<double>();
+  double* wim = wre + s->n;
+  double2* c1 = s->coef.as<double2>();
+  double2* c2 = c1 + (s->m + 1);
+  OAPH_TRY(random_vector(st, s->n, seed, true, wre, wim));
+  OAPH_TRY(orth(st, 0, w, nc, wre, wim, nullptr, nullptr, nullptr, nullptr, c1, 0, 0));
+  OAPH_TRY(orth(st, 1, w, nc, wre, wim, fre, fim, c1, nullptr, c2, 0, 0));
+  // H[nc][nc-1] stays 0: norm only (ncols = 0), written at hrow = m (nowhere)
+  OAPH_TRY(orth(st, 2, w, nc, fre, fim, fre, fim, c2, c1, nullptr, -1, (int)s->m));
+  return normalize(st, w, fre, fim, Vre(s, s->V, nc), Vim(s, s->V, nc));
+}
+
+// Runs restart cycles from column `start` until convergence at `tol` (relative
+// to norm, or to max |Ritz value| when norm <= 0).  On return the status holds
+// the final ordering and Y the wanted Ritz vectors of H.
+oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint32_t* cycles,
+                       bool* converged) {
+  hipStream_t st = s->api->stream;
+  uint32_t start = 0;
+  *converged = false;
+  for (uint32_t cycle = 0;; ++cycle) {
+    for (uint32_t j = start; j < s->m; ++j) OAPH_TRY(arnoldi_step(s, j, low));
+    RestartArgs ra;
+    ra.Hre = s->H.as<double>();
+    ra.Him = ra.Hre + (size_t)s->m * s->m;
+    ra.Qre = s->Q.as<double>();
+    ra.Qim = ra.Qre + (size_t)s->m * s->m;
+    ra.Xre = ra.Qim + (size_t)s->m * s->m;
+    ra.Xim = ra.Xre + (size_t)s->m * s->m;
+    ra.Yre = s->Y.as<double>();
+    ra.Yim = ra.Yre + (size_t)s->m * s->k;
+    ra.st = s->status.as<Status>();
+    ra.m = (int)s->m;
+    ra.mact = (int)s->m;
+    ra.k = (int)s->k;
+    ra.which = s->sort;
+    ra.do_shifts = 1;
+    ra.tol = tol;
+    ra.anorm = norm;
+    // a breakdown seen during this factorisation limits the active block
+    OAPH_TRY(read_status(s));
+    int bd = s->host_status.breakdown;
+    if (bd > 0) {
+      if ((uint32_t)bd < s->k || bd < 2) {
+        // invariant subspace smaller than k: continue from a fresh vector
+        int zero = 0;
+        OAPH_HIP(hipMemcpyAsync(&s->status.as<Status>()->breakdown, &zero, sizeof(int),
+                                hipMemcpyHostToDevice, st));
+        OAPH_TRY(fresh_column(s, (uint32_t)bd, s->seed + 7919u * (cycle + 1)));
+        start = (uint32_t)bd;
+        if (cycle >= s->max_restarts) break;
+        continue;
+      }
+      ra.mact = bd;
+      ra.do_shifts = 0;
+    }
+    OAPH_TRY(restart(st, ra));
+    OAPH_TRY(read_status(s));
+    s->stats.restarts++;
+    const Status& hs = s->host_status;
+    if (hs.schur_fail) {
+      oaph::set_last_message("Hessenberg QR did not converge");
+      return OAPH_ERR_NOT_CONVERGED;
+    }
+    bool stop = false;
+    if (bd > 0) {
+      stop = true;  // lucky breakdown: the leading block is invariant
+      *converged = true;
+    } else if (s->check == OAPH_CHECK_FIRST_STOP) {
+      stop = true;
+      *converged = hs.nconv >= (int)s->k;
+    } else if (s->check == OAPH_CHECK_COUNTER) {
+      stop = cycle + 1 >= s->check_counts;
+      *converged = hs.nconv >= (int)s->k;
+    } else if (hs.nconv >= (int)s->k) {
+      stop = true;
+      *converged = true;
+    }
+    if (stop || cycle + 1 >= s->max_restarts) {
+      *cycles = cycle + 1;
+      return OAPH_OK;
+    }
+    // V <- V Q[:, 0..kk), f update with column kk; then v_kk = f / ||f||
+    const uint32_t kk = (uint32_t)hs.kk;
+    double* fre = s->F.as<double>();
+    double* fim = fre + s->n;
+    OAPH_TRY(vq(st, s->n, s->m, kk + 1, Vre(s, s->V, 0), Vim(s, s->V, 0), ra.Qre, ra.Qim, s->m,
+                Vre(s, s->V2, 0), Vim(s, s->V2, 0), (int)kk, fre, fim, ra.st));
+    std::swap(s->V, s->V2);
+    Work w = work_of(s);
+    OAPH_TRY(orth(st, 2, w, 0, fre, fim, fre, fim, nullptr, nullptr, nullptr, (int)kk - 1,
+                  (int)kk));
+    OAPH_TRY(normalize(st, w, fre, fim, Vre(s, s->V, kk), Vim(s, s->V, kk)));
+    start = kk;
+  }
+  *cycles = s->max_restarts;
+  return OAPH_OK;
+}
+
+// X[:, t] = V Y[:, t] for the k wanted pairs
+oaph_status ritz_vectors(oaph_arnoldi* s, uint32_t mact) {
+  double* Yre = s->Y.as<double>();
+  double* Yim = Yre + (size_t)s->m * s->k;
+  double* Xre = s->X.as<double>();
+  double* Xim = Xre + (size_t)s->k * s->n;
+  return vq(s->api->stream, s->n, mact, s->k, Vre(s, s->V, 0), Vim(s, s->V, 0), Yre, Yim, s->k,
+            Xre, Xim, -1, nullptr, nullptr, nullptr);
+}
+
+oaph_status start_vector(oaph_arnoldi* s) {
+  hipStream_t st = s->api->stream;
+  double* v0r = Vre(s, s->V, 0);
+  double* v0i = Vim(s, s->V, 0);
+  size_t bytes = (size_t)s->n * sizeof(double);
+  if (!s->start_re.empty()) {
+    OAPH_HIP(hipMemcpyAsync(v0r, s->start_re.data(), bytes, hipMemcpyHostToDevice, st));
+    if (!s->start_im.empty())
+      OAPH_HIP(hipMemcpyAsync(v0i, s->start_im.data(), bytes, hipMemcpyHostToDevice, st));
+    else
+      OAPH_HIP(hipMemsetAsync(v0i, 0, bytes, st));
+  } else if (s->init == OAPH_INIT_FIRST_VALUE_IS_ONE) {
+    // initVvector_fvis1 (ArnoldiProcedures.cpp:279-287)
+    OAPH_HIP(hipMemsetAsync(v0r, 0, bytes, st));
+    OAPH_HIP(hipMemsetAsync(v0i, 0, bytes, st));
+    double one = 1.0;
+    OAPH_HIP(hipMemcpyAsync(v0r, &one, sizeof(double), hipMemcpyHostToDevice, st));
+    OAPH_HIP(hipStreamSynchronize(st));
+  } else {
+    // initVvector_rand (ArnoldiProcedures.cpp:289-319): uniform in [-1, 1],
+    // seeded (the reference draws from std::random_device)
+    OAPH_TRY(random_vector(st, s->n, s->seed, false, v0r, v0i));
+  }
+  return OAPH_OK;
+}
+
+// normalise V[:, 0] in place
+oaph_status normalize_v0(oaph_arnoldi* s) {
+  hipStream_t st = s->api->stream;
+  Work w = work_of(s);
+  double* fre = s->F.as<double>();
+  double* fim = fre + s->n;
+  OAPH_HIP(hipMemcpyAsync(fre, Vre(s, s->V, 0), (size_t)s->n * sizeof(double),
+                          hipMemcpyDeviceToDevice, st));
+  OAPH_HIP(hipMemcpyAsync(fim, Vim(s, s->V, 0), (size_t)s->n * sizeof(double),
+                          hipMemcpyDeviceToDevice, st));
+  OAPH_TRY(orth(st, 2, w, 0, fre, fim, fre, fim, nullptr, nullptr, nullptr, -1, (int)s->m));
+  return normalize(st, w, fre, fim, Vre(s, s->V, 0), Vim(s, s->V, 0));
+}
+
+oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
+  hipStream_t st = s->api->stream;
+  const uint32_t k = s->k, n = s->n;
+  OAPH_TRY(ritz_vectors(s, mact));
+  double* Xre = s->X.as<double>();
+  double* Xim = Xre + (size_t)k * n;
+  double* AXre = s->AX.as<double>();
+  double* AXim = AXre + (size_t)k * n;
+  for (uint32_t t = 0; t < k; ++t)
+    OAPH_TRY(apply_operator(s, Xre + (size_t)t * n, Xim + (size_t)t * n, AXre + (size_t)t * n,
+                            AXim + (size_t)t * n, OAPH_TYPE_EIGENVECTOR, false));
+  double2* lam = s->coef.as<double2>() + 2 * (s->m + 1);
+  double2* out = lam + (s->m + 1);
+  double* part = s->partials.as<double>();
+  unsigned* cnt = s->counter.as<unsigned>() + 16;
+  OAPH_TRY(pairs(st, 0, n, k, Xre, Xim, AXre, AXim, lam, out, part, cnt));
+  std::vector<double2> lam_h(k), out_h(k);
+  OAPH_HIP(hipMemcpyAsync(lam_h.data(), lam, k * sizeof(double2), hipMemcpyDeviceToHost, st));
+  OAPH_TRY(pairs(st, 1, n, k, Xre, Xim, AXre, AXim, lam, out, part, cnt));
+  OAPH_HIP(hipMemcpyAsync(out_h.data(), out, k * sizeof(double2), hipMemcpyDeviceToHost, st));
+  OAPH_HIP(hipStreamSynchronize(st));
+  s->ev_re.assign(k, 0.0);
+  s->ev_im.assign(k, 0.0);
+  s->resid.assign(k, 0.0);
+  double maxr = 0;
+  for (uint32_t t = 0; t < k; ++t) {
+    // Rayleigh quotient x^H A x / x^H x (FP64) refines the Ritz value
+    s->ev_re[t] = lam_h[t].x;
+    s->ev_im[t] = lam_h[t].y;
+    double nx = sqrt(out_h[t].y);
+    s->resid[t] = nx > 0 ? sqrt(out_h[t].x) / nx : INFINITY;
+    maxr = fmax(maxr, s->resid[t]);
+  }
+  s->stats.max_resid = maxr;
+  // outputs (extractOutput, ArnoldiProcedures.cpp:229-258)
+  for (uint32_t t = 0; t < k; ++t) {
+    if (s->out_re) s->out_re[t] = s->ev_re[t];
+    if (s->out_im) s->out_im[t] = s->ev_im[t];
+  }
+  if (s->out_vecs) {
+    for (uint32_t t = 0; t < k; ++t) {
+      oaph_matrix* o = s->out_vecs[t];
+      if (!o) continue;
+      OAPH_CHECK((size_t)o->columns * o->rows == n, OAPH_ERR_SHAPE);
+      double inv = 1.0 / sqrt(out_h[t].y);
+      oaph_matrix src{1, n, 1, n, Xre + (size_t)t * n, Xim + (size_t)t * n};
+      if (s->out_type == OAPH_HOST) {
+        std::vector<double> tmp(n);
+        OAPH_HIP(hipMemcpy(tmp.data(), src.reValues, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (o->reValues)
+          for (uint32_t i = 0; i < n; ++i) o->reValues[i] = tmp[i] * inv;
+        OAPH_HIP(hipMemcpy(tmp.data(), src.imValues, n * sizeof(double), hipMemcpyDeviceToHost));
+        if (o->imValues)
+          for (uint32_t i = 0; i < n; ++i) o->imValues[i] = tmp[i] * inv;
+      } else {
+        OAPH_TRY(launch_scale(st, o, &src, inv, inv, false));
+      }
+      if (s->cfn) s->cfn(s->ev_re[t], s->ev_im[t], o, t, k, s->cuser);
+    }
+    OAPH_HIP(hipStreamSynchronize(st));
+  }
+  return OAPH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+oaph_status oaph_arnoldi_create(oaph_arnoldi** out) {
+  OAPH_CHECK(out, OAPH_ERR_ARG);
+  OAPH_TRY(ensure_init());
+  oaph_arnoldi* s = new oaph_arnoldi();
+  oaph_status st = oaph_api_create(&s->api);
+  if (st != OAPH_OK) {
+    delete s;
+    return st;
+  }
+  *out = s;
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_destroy(oaph_arnoldi* s) {
+  if (!s) return OAPH_OK;
+  if (s->api) (void)hipStreamSynchronize(s->api->stream);
+  for (DevBuf* b : {&s->V, &s->V2, &s->W, &s->F, &s->H, &s->Q, &s->Y, &s->X, &s->AX, &s->coef,
+                    &s->partials, &s->counter, &s->status, &s->A32})
+    b->release();
+  oaph_api_destroy(s->api);
+  delete s;
+  return OAPH_OK;
+}
+
+oaph_api* oaph_arnoldi_api(oaph_arnoldi* s) { return s ? s->api : nullptr; }
+
+#define SETTER(name, field, type)                            \
+  oaph_status oaph_arnoldi_set_##name(oaph_arnoldi* s, type v) { \
+    OAPH_CHECK(s, OAPH_ERR_ARG);                             \
+    s->field = v;                                            \
+    return OAPH_OK;                                          \
+  }
+SETTER(rho, rho, double)
+SETTER(blimit, blimit, double)
+SETTER(qr_type, qrtype, int)
+SETTER(vec_init_type, init, int)
+SETTER(check_counts, check_counts, uint32_t)
+SETTER(output_type, out_type, int)
+SETTER(calc_triangular_h_type, tri_type, int)
+SETTER(seed, seed, uint64_t)
+SETTER(max_restarts, max_restarts, uint32_t)
+#undef SETTER
+
+oaph_status oaph_arnoldi_set_sort_type(oaph_arnoldi* s, int v) {
+  OAPH_CHECK(s && v >= 0 && v <= 5, OAPH_ERR_ARG);
+  s->sort = v;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_check_type(oaph_arnoldi* s, int v) {
+  OAPH_CHECK(s && v >= 0 && v <= 2, OAPH_ERR_ARG);
+  s->check = v;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_check_tolerance(oaph_arnoldi* s, double v) {
+  OAPH_CHECK(s && v > 0, OAPH_ERR_ARG);
+  s->tol = v;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_norm_a(oaph_arnoldi* s, double v) {
+  OAPH_CHECK(s && v >= 0, OAPH_ERR_ARG);
+  s->norm_a = v;
+  s->norm_a_user = v > 0;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* s, int v) {
+  OAPH_CHECK(s && (v == OAPH_PREC_FP64 || v == OAPH_PREC_MIXED_FP32), OAPH_ERR_ARG);
+  s->precision = v;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_outputs_eigenvalues(oaph_arnoldi* s, double* re, double* im) {
+  OAPH_CHECK(s, OAPH_ERR_ARG);
+  s->out_re = re;
+  s->out_im = im;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_outputs_eigenvectors(oaph_arnoldi* s, oaph_matrix** v) {
+  OAPH_CHECK(s, OAPH_ERR_ARG);
+  s->out_vecs = v;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_callback(oaph_arnoldi* s, oaph_multiply_fn fn, void* user) {
+  OAPH_CHECK(s, OAPH_ERR_ARG);
+  s->mfn = fn;
+  s->muser = user;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_check_callback(oaph_arnoldi* s, oaph_check_fn fn, void* user) {
+  OAPH_CHECK(s, OAPH_ERR_ARG);
+  s->cfn = fn;
+  s->cuser = user;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_matrix(oaph_arnoldi* s, const oaph_matrix* A) {
+  OAPH_CHECK(s, OAPH_ERR_ARG);
+  if (A) {
+    OAPH_CHECK(A->columns == A->rows && A->reValues, OAPH_ERR_SHAPE);
+  }
+  s->A = A;
+  s->a32_of = nullptr;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_start_vector(oaph_arnoldi* s, const double* re, const double* im,
+                                          uint32_t n) {
+  OAPH_CHECK(s, OAPH_ERR_ARG);
+  if (!re) {
+    s->start_re.clear();
+    s->start_im.clear();
+    return OAPH_OK;
+  }
+  s->start_re.assign(re, re + n);
+  if (im) s->start_im.assign(im, im + n);
+  else s->start_im.clear();
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted,
+                                 const oaph_matrix_info* info, int type) {
+  OAPH_CHECK(s && info, OAPH_ERR_ARG);
+  (void)type;
+  // begin() preconditions (ArnoldiProcedures.cpp:146-152)
+  OAPH_CHECK(wanted >= 1 && hdim > wanted, OAPH_ERR_ARG);
+  OAPH_CHECK(hdim <= (uint32_t)kMaxM, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(info->rows >= hdim, OAPH_ERR_SHAPE);
+  OAPH_CHECK(s->A || s->mfn, OAPH_ERR_STATE);
+  if (s->A) OAPH_CHECK(s->A->rows == info->rows, OAPH_ERR_SHAPE);
+  if (!s->start_re.empty()) OAPH_CHECK(s->start_re.size() == info->rows, OAPH_ERR_SHAPE);
+  auto t0 = std::chrono::steady_clock::now();
+  s->n = info->rows;
+  s->m = hdim;
+  s->k = wanted;
+  s->cplx = info->isIm != 0;
+  s->stats = oaph_arnoldi_stats{};
+  s->executed = false;
+  OAPH_TRY(alloc(s));
+  hipStream_t st = s->api->stream;
+  OAPH_HIP(hipMemsetAsync(s->status.ptr, 0, sizeof(Status), st));
+  OAPH_HIP(hipMemsetAsync(s->H.ptr, 0, 2 * (size_t)s->m * s->m * sizeof(double), st));
+  // the tolerance is relative to norm_a: the user's value, ||A||_F of a dense
+  // operator, or (callback without a norm) the largest |Ritz value|
+  double norm = s->norm_a_user ? s->norm_a : 0.0;
+  if (!s->norm_a_user && s->A) {
+    OAPH_TRY(s->partials.reserve(4096 * sizeof(double)));
+    double* res = s->coef.as<double>();
+    OAPH_TRY(launch_sumsq(st, s->A->reValues, s->A->imValues, (size_t)s->n * s->n,
+                          s->partials.as<double>(), s->counter.as<unsigned>() + 32, res));
+    double h = 0;
+    OAPH_HIP(hipMemcpyAsync(&h, res, sizeof(double), hipMemcpyDeviceToHost, st));
+    OAPH_HIP(hipStreamSynchronize(st));
+    norm = sqrt(h);
+    s->norm_a = norm;
+  }
+  const bool mixed = s->precision == OAPH_PREC_MIXED_FP32 && s->A != nullptr;
+  if (mixed && s->a32_of != s->A) {
+    size_t elems = (size_t)s->n * s->A->columns;
+    OAPH_TRY(s->A32.reserve(2 * elems * sizeof(float)));
+    s->a32_elems = elems;
+    float* a32 = s->A32.as<float>();
+    OAPH_TRY(to_f32(st, elems, s->A->reValues, a32));
+    if (s->A->imValues) OAPH_TRY(to_f32(st, elems, s->A->imValues, a32 + elems));
+    else OAPH_HIP(hipMemsetAsync(a32 + elems, 0, elems * sizeof(float), st));
+    s->a32_of = s->A;
+  }
+  OAPH_TRY(start_vector(s));
+  OAPH_TRY(normalize_v0(s));
+  uint32_t cycles = 0;
+  bool converged = false;
+  if (mixed) {
+    // FP32 phase down to the FP32 floor (relative to max |Ritz value|)
+    const double tol_low = 4e-7;
+    OAPH_TRY(run_cycles(s, true, tol_low, 0.0, &cycles, &converged));
+    OAPH_TRY(ritz_vectors(s, s->m));
+    double* Xre = s->X.as<double>();
+    OAPH_TRY(colsum(st, s->n, s->k, Xre, Xre + (size_t)s->k * s->n, Vre(s, s->V, 0),
+                    Vim(s, s->V, 0)));
+    OAPH_TRY(normalize_v0(s));
+    OAPH_HIP(hipMemsetAsync(s->status.ptr, 0, sizeof(Status), st));
+    OAPH_HIP(hipMemsetAsync(s->H.ptr, 0, 2 * (size_t)s->m * s->m * sizeof(double), st));
+  }
+  uint32_t c2 = 0;
+  OAPH_TRY(run_cycles(s, false, s->tol, norm, &c2, &converged));
+  int bd = s->host_status.breakdown;
+  uint32_t mact = bd > 0 ? (uint32_t)bd : s->m;
+  OAPH_TRY(finish(s, mact));
+  s->stats.nconv = (uint32_t)s->host_status.nconv;
+  if (bd > 0) s->stats.nconv = s->k;
+  s->stats.seconds =
+      std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  s->executed = true;
+  if (s->check == OAPH_CHECK_INTERNAL && !converged) return OAPH_ERR_NOT_CONVERGED;
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_test_outcome(oaph_arnoldi* s, uint32_t index, double* out) {
+  OAPH_CHECK(s && out, OAPH_ERR_ARG);
+  OAPH_CHECK(s->executed, OAPH_ERR_STATE);
+  OAPH_CHECK(index < s->resid.size(), OAPH_ERR_ARG);
+  *out = s->resid[index];
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* s, int low, uint32_t iters, double* avg_ms) {
+  OAPH_CHECK(s && avg_ms && iters > 0, OAPH_ERR_ARG);
+  OAPH_CHECK(s->A != nullptr, OAPH_ERR_STATE);
+  const uint32_t n = s->A->rows;
+  hipStream_t st = s->api->stream;
+  if (s->n != n || !s->V.ptr) {
+    s->n = n;
+    s->m = s->m ? s->m : 2;
+    s->k = s->k ? s->k : 1;
+    OAPH_TRY(alloc(s));
+  }
+  if (low && s->a32_of != s->A) {
+    size_t elems = (size_t)n * s->A->columns;
+    OAPH_TRY(s->A32.reserve(2 * elems * sizeof(float)));
+    s->a32_elems = elems;
+    float* a32 = s->A32.as<float>();
+    OAPH_TRY(to_f32(st, elems, s->A->reValues, a32));
+    if (s->A->imValues) OAPH_TRY(to_f32(st, elems, s->A->imValues, a32 + elems));
+    else OAPH_HIP(hipMemsetAsync(a32 + elems, 0, elems * sizeof(float), st));
+    s->a32_of = s->A;
+  }
+  double* xre = Vre(s, s->V, 0);
+  double* xim = Vim(s, s->V, 0);
+  double* yre = s->W.as<double>();
+  double* yim = yre + n;
+  OAPH_TRY(random_vector(st, n, 1, true, xre, xim));
+  auto keep = s->stats;
+  OAPH_TRY(apply_operator(s, xre, xim, yre, yim, OAPH_TYPE_WV, low != 0));
+  hipEvent_t e0, e1;
+  OAPH_HIP(hipEventCreate(&e0));
+  OAPH_HIP(hipEventCreate(&e1));
+  OAPH_HIP(hipEventRecord(e0, st));
+  for (uint32_t i = 0; i < iters; ++i)
+    OAPH_TRY(apply_operator(s, xre, xim, yre, yim, OAPH_TYPE_WV, low != 0));
+  OAPH_HIP(hipEventRecord(e1, st));
+  OAPH_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  OAPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  s->stats = keep;
+  *avg_ms = ms / iters;
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* s, oaph_arnoldi_stats* stats) {
+  OAPH_CHECK(s && stats, OAPH_ERR_ARG);
+  *stats = s->stats;
+  return OAPH_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,104 @@
+// internal.h -- shared declarations of the gfx950 implementation behind oaph.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+#include "../../include/oaph.h"
+
+namespace oaph {
+
+// Thread-local text of the last HIP failure (oaph_last_error()).
+void set_last_error(const char* where, hipError_t e);
+void set_last_message(const std::string& msg);
+
+#define OAPH_HIP(expr)                                  \
+  do {                                                  \
+    hipError_t e_ = (expr);                             \
+    if (e_ != hipSuccess) {                             \
+      ::oaph::set_last_error(#expr, e_);                \
+      return OAPH_ERR_HIP;                              \
+    }                                                   \
+  } while (0)
+
+#define OAPH_CHECK(cond, code) \
+  do {                         \
+    if (!(cond)) return (code);\
+  } while (0)
+
+#define OAPH_TRY(expr)              \
+  do {                              \
+    oaph_status s_ = (expr);        \
+    if (s_ != OAPH_OK) return s_;   \
+  } while (0)
+
+oaph_status ensure_init();
+
+// A device scratch buffer that only grows.
+struct DevBuf {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  oaph_status reserve(size_t n);
+  void release();
+  template <typename T>
+  T* as() const { return static_cast<T*>(ptr); }
+};
+
+}  // namespace oaph
+
+// oaph_api: one HIP stream plus scratch for reductions (the reference's
+// CuProceduresApi owns two device scalars and growable TBuffers,
+// oapMatrixCuda/CuProceduresApi.cpp:64-89).
+struct oaph_api {
+  hipStream_t stream = nullptr;
+  oaph::DevBuf partials;   // per-block partial sums
+  oaph::DevBuf counters;   // last-block-done counters (kept zero between launches)
+  oaph::DevBuf scalars;    // device-side results
+  oaph::DevBuf pinned_host;  // not used for device memory; see host_scalars
+  double* host_scalars = nullptr;  // pinned, 16 doubles
+  oaph::DevBuf work;       // general device workspace (QR, triangular-H)
+};
+
+namespace oaph {
+
+inline size_t numel(const oaph_matrix* m) { return (size_t)m->columns * m->rows; }
+
+// ---- kernel launchers (each enqueues on `s`, returns OAPH_OK or HIP error) ----
+// elementwise.hip
+oaph_status launch_transpose(hipStream_t s, oaph_matrix* out, const oaph_matrix* a, bool conj);
+oaph_status launch_set_diagonal(hipStream_t s, oaph_matrix* m, double re, double im);
+oaph_status launch_binary(hipStream_t s, int op, oaph_matrix* out, const oaph_matrix* a,
+                          const oaph_matrix* b);  // 0 add, 1 sub, 2 hadamard, 3 hadamard_vec
+oaph_status launch_scale(hipStream_t s, oaph_matrix* out, const oaph_matrix* a, double re,
+                         double im, bool re_only);
+oaph_status launch_tensor(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
+                          const oaph_matrix* b, const uint32_t dims[3][2]);
+oaph_status launch_set_vector(hipStream_t s, oaph_matrix* V, uint32_t column, const oaph_matrix* v,
+                              uint32_t length);
+oaph_status launch_get_vector(hipStream_t s, oaph_matrix* v, uint32_t length, const oaph_matrix* V,
+                              uint32_t column);
+// reduce.hip
+oaph_status launch_sumsq(hipStream_t s, const double* re, const double* im, size_t n,
+                         double* partials, unsigned* counter, double* result);
+oaph_status launch_sum2(hipStream_t s, const double* re, const double* im, size_t n,
+                        double* partials, unsigned* counter, double* result);
+int reduce_blocks_for(size_t n);
+// gemv.hip
+oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
+                        const oaph_matrix* b, uint32_t W, uint32_t H, uint32_t K, bool dims_mode);
+oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
+                         const double* are, const double* aim, const double* xre,
+                         const double* xim, double* yre, double* yim);
+oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
+                           const float* are, const float* aim, const double* xre,
+                           const double* xim, double* yre, double* yim);
+// small.hip
+oaph_status launch_qrgr(hipStream_t s, oaph_matrix* Q, oaph_matrix* R, const oaph_matrix* A);
+oaph_status launch_is_upper_triangular(hipStream_t s, const oaph_matrix* m, int* dflag);
+oaph_status launch_calc_triangular_h(hipStream_t s, oaph_matrix* H, oaph_matrix* Q,
+                                     double* work, uint32_t max_steps);
+
+}  // namespace oaph

@@ -1,0 +1,77 @@
+// iram.h -- shared declarations of the Arnoldi solver (host driver + kernels).
+#pragma once
+
+#include "internal.h"
+
+namespace oaph {
+namespace iram {
+
+constexpr int kMaxM = 64;  // Krylov dimension limit of the one-wave restart kernel
+
+// Device-resident solver status; the host reads it once per restart cycle.
+struct Status {
+  int nconv;
+  int kk;
+  int breakdown;   // first index j+1 whose Arnoldi residual vanished (0 = none)
+  int schur_fail;
+  double beta;     // norm of the latest residual vector f
+  double beta_k_re, beta_k_im, sigma_re, sigma_im;
+  double maxabs;   // max |Ritz value|
+  double tolabs;   // absolute residual tolerance used for nconv
+  double ev_re[kMaxM], ev_im[kMaxM], est[kMaxM];  // Ritz values in wanted order, estimates
+  int order[kMaxM];
+};
+
+// Device buffers the orthogonalisation kernels use.
+struct Work {
+  uint32_t n = 0, m = 0;
+  double* Vre = nullptr;  // column c at Vre + c*n
+  double* Vim = nullptr;
+  double* Hre = nullptr;  // m x m, row-major, ld m
+  double* Him = nullptr;
+  double* partials = nullptr;
+  unsigned* counter = nullptr;
+  Status* st = nullptr;
+};
+
+struct RestartArgs {
+  double* Hre;
+  double* Him;
+  double* Qre;
+  double* Qim;
+  double* Yre;  // m x k wanted Ritz vectors of H
+  double* Yim;
+  double* Xre;  // m x m work
+  double* Xim;
+  Status* st;
+  int m;        // leading dimension of H, Q, X
+  int mact;     // active size
+  int k;
+  int which;
+  int do_shifts;
+  double tol;
+  double anorm;  // <= 0: relative to max |Ritz value|
+};
+
+oaph_status orth(hipStream_t s, int mode, const Work& w, uint32_t ncols, const double* xre,
+                 const double* xim, double* fre, double* fim, const double2* sub,
+                 const double2* add, double2* result, int hcol, int hrow);
+size_t orth_partials_bytes(uint32_t n, uint32_t m);
+oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const double* fim,
+                      double* vre, double* vim);
+oaph_status restart(hipStream_t s, const RestartArgs& a);
+oaph_status vq(hipStream_t s, uint32_t n, uint32_t mrows, uint32_t ncols, const double* Vre,
+               const double* Vim, const double* Mre, const double* Mim, uint32_t ldm,
+               double* Ore, double* Oim, int fcol, double* fre, double* fim, const Status* st);
+oaph_status pairs(hipStream_t s, int mode, uint32_t n, uint32_t k, const double* Xre,
+                  const double* Xim, const double* Yre, const double* Yim, double2* lambda,
+                  double2* out, double* partials, unsigned* counter);
+size_t pairs_partials_bytes(uint32_t n, uint32_t k);
+oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst);
+oaph_status colsum(hipStream_t s, uint32_t n, uint32_t k, const double* Xre, const double* Xim,
+                   double* vre, double* vim);
+oaph_status random_vector(hipStream_t s, uint32_t n, uint64_t seed, bool cplx, double* re,
+                          double* im);
+
+}  // namespace iram
+}  // namespace oaph

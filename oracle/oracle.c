@@ -1,0 +1,912 @@
+/*
+ * oracle.c -- CPU restatement of oap's complex procedures and its implicitly
+ * restarted Arnoldi path.  TEST INFRASTRUCTURE ONLY (see oracle.h): the HIP
+ * library never links or calls this code; it is the checker.
+ *
+ * Each function cites the reference file:line it restates.  Where the
+ * reference is wrong for complex input or leaves a result undefined, the
+ * restatement documents the divergence next to the code.
+ *
+ * Build: gcc -O2 -ffp-contract=off -fPIC -shared (oracle/Makefile).  FP
+ * contraction is off so that sums accumulate in the reference's sequential
+ * order with separately rounded products, like the reference's own host
+ * build (g++ without -mfma).
+ */
+#include "oracle.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IDX(m, c, r) ((size_t)(c) + (size_t)(r) * (m)->columns)
+
+/* ------------------------------------------------------------------ */
+/* launch geometry: oapUtils/ThreadsMapper.cpp:29-74                    */
+/* ------------------------------------------------------------------ */
+static void increase_block(uint32_t blocks[2], const uint32_t threads[2], int index,
+                           uint32_t limit) {
+  /* ThreadsMapper.cpp:29-34: grow the block count until the grid covers limit */
+  while (threads[index] * blocks[index] < limit) ++blocks[index];
+}
+
+void orc_set_threads_blocks(uint32_t blocks[2], uint32_t threads[2], uint32_t w, uint32_t h,
+                            uint32_t limit) {
+  /* ThreadsMapper.cpp:36-74.  sqrt of the limit truncated to an integer. */
+  uint32_t sq = (uint32_t)sqrt((double)limit);
+  blocks[0] = 1;
+  blocks[1] = 1;
+  if (w * h < limit) {
+    threads[0] = w;
+    threads[1] = h;
+  } else if (sq <= w && sq <= h) {
+    threads[0] = sq;
+    threads[1] = sq;
+    increase_block(blocks, threads, 0, w);
+    increase_block(blocks, threads, 1, h);
+  } else if (sq <= w && sq > h) {
+    threads[0] = sq;
+    threads[1] = h;
+    increase_block(blocks, threads, 0, w);
+  } else if (sq <= h && sq > w) {
+    threads[0] = w;
+    threads[1] = sq;
+    increase_block(blocks, threads, 1, h);
+  } else {
+    double factor = w / (double)(w + h);
+    uint32_t lim2 = 2 * sq;
+    threads[0] = (uint32_t)(factor * lim2);
+    threads[1] = (uint32_t)((1 - factor) * lim2);
+    increase_block(blocks, threads, 0, w);
+    increase_block(blocks, threads, 1, h);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* dotProduct: oapMatrixCuda/CuProcedures/CuDotProductSpecificProcedures.h:26-141 */
+/* ------------------------------------------------------------------ */
+int orc_dot_product(orc_matrix* out, const orc_matrix* a, const orc_matrix* b) {
+  if (a->columns != b->rows || out->columns != b->columns || out->rows != a->rows) return 1;
+  const int isre = out->re != NULL, isim = out->im != NULL;
+  /* Im-only output: the reference writes -sum(a_im b_im) into the absent re
+   * plane (CuDotProductSpecificProcedures.h:45-62) -- undefined; rejected. */
+  if (!isre) return 2;
+  if (isim && (!a->re || !a->im || !b->re || !b->im)) return 3;
+  if (!isim && (!a->re || !b->re)) return 3;
+  const uint32_t K = a->columns;
+  for (uint32_t y = 0; y < out->rows; ++y) {
+    for (uint32_t x = 0; x < out->columns; ++x) {
+      double re = 0, im = 0;
+      for (uint32_t k = 0; k < K; ++k) {
+        size_t ia = (size_t)k + (size_t)K * y, ib = (size_t)k * b->columns + x;
+        if (isim) {
+          /* :75-84, four separately accumulated products in this order */
+          re += a->re[ia] * b->re[ib];
+          re -= a->im[ia] * b->im[ib];
+          im += a->re[ia] * b->im[ib];
+          im += a->im[ia] * b->re[ib];
+        } else {
+          re += a->re[ia] * b->re[ib]; /* :38-41 */
+        }
+      }
+      out->re[IDX(out, x, y)] = re;
+      if (isim) out->im[IDX(out, x, y)] = im;
+    }
+  }
+  return 0;
+}
+
+/* dotProduct(dims): CuDotProductDimProcedures.h:30-48 with
+ * CuDotProductGenericProcedures.h:26-80: output window dims[0] = {cols, rows},
+ * reduction length dims[1][0], full-matrix strides. */
+int orc_dot_product_dims(orc_matrix* out, const orc_matrix* a, const orc_matrix* b,
+                         const uint32_t dims[3][2]) {
+  const uint32_t W = dims[0][0], H = dims[0][1], K = dims[1][0];
+  if (dims[1][0] != dims[2][1] || dims[0][0] != dims[2][0] || dims[0][1] != dims[1][1]) return 1;
+  if (W > out->columns || H > out->rows || dims[1][0] > a->columns || dims[1][1] > a->rows ||
+      dims[2][0] > b->columns || dims[2][1] > b->rows)
+    return 1;
+  const int isre = out->re != NULL, isim = out->im != NULL;
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) {
+      double re = 0, im = 0;
+      for (uint32_t k = 0; k < K; ++k) {
+        size_t ia = (size_t)k + (size_t)a->columns * y, ib = (size_t)x + (size_t)b->columns * k;
+        if (isre && isim) {
+          /* maux_calcRealValue: one expression per plane */
+          re += a->re[ia] * b->re[ib] - a->im[ia] * b->im[ib];
+          im += a->re[ia] * b->im[ib] + a->im[ia] * b->re[ib];
+        } else if (isre) {
+          re += a->re[ia] * b->re[ib];
+        } else if (isim) {
+          im -= a->im[ia] * b->im[ib]; /* maux_calcImValue */
+        }
+      }
+      if (isre) out->re[IDX(out, x, y)] = re;
+      if (isim) out->im[IDX(out, x, y)] = im;
+    }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* transpose: CuProceduresApi.cpp:268-286, CuTransposeProcedures.h:106-158 */
+/* conjugateTranspose: CuProceduresApi.cpp:288-307, CuConjugateTransposeProcedures.h:106-158 */
+/* ------------------------------------------------------------------ */
+static int transpose_impl(orc_matrix* out, const orc_matrix* a, double imsign) {
+  if (out->rows != a->columns || out->columns != a->rows) return 1;
+  const int vec = (out->columns == 1 && a->rows == 1) || (out->rows == 1 && a->columns == 1);
+  for (uint32_t y = 0; y < out->rows; ++y)
+    for (uint32_t x = 0; x < out->columns; ++x) {
+      size_t io = IDX(out, x, y), ia = (size_t)x * a->columns + y;
+      if (out->re) out->re[io] = a->re[ia];
+      /* vector <-> row is a plain device copy in the reference, so the
+       * conjugate variant does not negate there (CuProceduresApi.cpp:298-301) */
+      if (out->im) out->im[io] = vec ? a->im[ia] : imsign * a->im[ia];
+    }
+  return 0;
+}
+int orc_transpose(orc_matrix* out, const orc_matrix* a) { return transpose_impl(out, a, 1.0); }
+int orc_conjugate_transpose(orc_matrix* out, const orc_matrix* a) {
+  return transpose_impl(out, a, -1.0);
+}
+
+/* tensorProduct(dims): CuTensorProductDimProcedures.h:26-114.  Re plane only;
+ * the reference's Im/Real bodies are empty, so the im plane is left as is. */
+int orc_tensor_product(orc_matrix* out, const orc_matrix* a, const orc_matrix* b,
+                       const uint32_t dims[3][2]) {
+  const uint32_t W = dims[0][0], H = dims[0][1];
+  const uint32_t bc = dims[2][0], br = dims[2][1];
+  if (W != dims[1][0] * bc || H != dims[1][1] * br) return 1;
+  if (!out->re || !a->re || !b->re) return 0; /* nothing is written (see above) */
+  if (out->im) return 0; /* complex output: the reference body is empty */
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) {
+      double v0 = a->re[(size_t)(x / bc) + (size_t)a->columns * (y / br)];
+      double v1 = b->re[(size_t)(x % bc) + (size_t)b->columns * (y % br)];
+      out->re[IDX(out, x, y)] = v0 * v1;
+    }
+  return 0;
+}
+
+/* hadamardProduct: CuHadamardProductProcedures.h:26-106 (per plane, not complex) */
+int orc_hadamard_product(orc_matrix* out, const orc_matrix* a, const orc_matrix* b) {
+  size_t n = (size_t)out->columns * out->rows;
+  for (size_t i = 0; i < n; ++i) {
+    if (out->re) out->re[i] = a->re[i] * b->re[i];
+    if (out->im) out->im[i] = a->im[i] * b->im[i];
+  }
+  return 0;
+}
+/* hadamardProductVec: CuPartialHadamardProductProcedures.h:26-109, b is a column
+ * vector broadcast along each row (index1 = row). */
+int orc_hadamard_product_vec(orc_matrix* out, const orc_matrix* a, const orc_matrix* b) {
+  if (b->rows != out->rows) return 1;
+  for (uint32_t y = 0; y < out->rows; ++y)
+    for (uint32_t x = 0; x < out->columns; ++x) {
+      size_t i = IDX(out, x, y);
+      if (out->re) out->re[i] = a->re[i] * b->re[y];
+      if (out->im) out->im[i] = a->im[i] * b->im[y];
+    }
+  return 0;
+}
+
+/* add / substract: CuAdditionProcedures.h:88-107, CuSubstractionProcedures.h:92-110 */
+int orc_add(orc_matrix* out, const orc_matrix* a, const orc_matrix* b) {
+  size_t n = (size_t)out->columns * out->rows;
+  for (size_t i = 0; i < n; ++i) {
+    if (out->re) out->re[i] = a->re[i] + b->re[i];
+    if (out->im) out->im[i] = a->im[i] + b->im[i];
+  }
+  return 0;
+}
+int orc_substract(orc_matrix* out, const orc_matrix* a, const orc_matrix* b) {
+  size_t n = (size_t)out->columns * out->rows;
+  for (size_t i = 0; i < n; ++i) {
+    if (out->re) out->re[i] = a->re[i] - b->re[i];
+    if (out->im) out->im[i] = a->im[i] - b->im[i];
+  }
+  return 0;
+}
+
+/* multiplyReConstant: CuMatrixKernels.cu:119-123 -> CuMultiplicationProcedures.h:28-40.
+ * Scales the re plane only; the im plane of out is not written. */
+int orc_multiply_re_constant(orc_matrix* out, const orc_matrix* a, double re) {
+  size_t n = (size_t)out->columns * out->rows;
+  if (!out->re) return 0;
+  for (size_t i = 0; i < n; ++i) out->re[i] = a->re[i] * re;
+  return 0;
+}
+/* multiplyConstant: CuMultiplicationProcedures.h:56-116, per plane: re*alpha_re, im*alpha_im */
+int orc_multiply_constant(orc_matrix* out, const orc_matrix* a, double re, double im) {
+  size_t n = (size_t)out->columns * out->rows;
+  for (size_t i = 0; i < n; ++i) {
+    if (out->re) out->re[i] = a->re[i] * re;
+    if (out->im) out->im[i] = a->im[i] * im;
+  }
+  return 0;
+}
+
+/* setDiagonal / setIdentity / setZero: CuSetMatrixProcedures.h:31-97 */
+int orc_set_diagonal(orc_matrix* m, double re, double im) {
+  for (uint32_t y = 0; y < m->rows; ++y)
+    for (uint32_t x = 0; x < m->columns; ++x) {
+      if (m->re) m->re[IDX(m, x, y)] = x == y ? re : 0.0;
+      if (m->im) m->im[IDX(m, x, y)] = x == y ? im : 0.0;
+    }
+  return 0;
+}
+int orc_set_identity(orc_matrix* m) { return orc_set_diagonal(m, 1.0, 0.0); }
+int orc_set_zero(orc_matrix* m) { return orc_set_diagonal(m, 0.0, 0.0); }
+
+/* setVector / getVector: CuVectorUtils.h:27-69 with the grid of
+ * GenericProceduresApi.h:610-641 (v's columns x rows, rows < length). */
+int orc_set_vector(orc_matrix* V, uint32_t column, const orc_matrix* v, uint32_t length) {
+  for (uint32_t y = 0; y < v->rows && y < length; ++y)
+    for (uint32_t x = 0; x < v->columns; ++x) {
+      size_t i1 = (size_t)y * V->columns + column + x, i2 = (size_t)y * v->columns + x;
+      if (V->re && v->re) V->re[i1] = v->re[i2];
+      if (V->im && v->im) V->im[i1] = v->im[i2];
+    }
+  return 0;
+}
+int orc_get_vector(orc_matrix* v, uint32_t length, const orc_matrix* V, uint32_t column) {
+  for (uint32_t y = 0; y < v->rows && y < length; ++y)
+    for (uint32_t x = 0; x < v->columns; ++x) {
+      size_t i1 = (size_t)y * V->columns + column + x, i2 = (size_t)y * v->columns + x;
+      if (V->re && v->re) v->re[i2] = V->re[i1];
+      if (V->im && v->im) v->im[i2] = V->im[i1];
+    }
+  return 0;
+}
+
+/* magnitude: CuProceduresApi.cpp:342-345 -> CuMagnitudeOptProcedures.h:30-113;
+ * sqrt(sum re^2 + im^2).  Summed in index order here; the reference sums
+ * per-block trees then block partials on the host (ulp-level difference). */
+double orc_magnitude(const orc_matrix* m) {
+  size_t n = (size_t)m->columns * m->rows;
+  double s = 0;
+  for (size_t i = 0; i < n; ++i) {
+    double t = 0;
+    if (m->re) t += m->re[i] * m->re[i];
+    if (m->im) t += m->im[i] * m->im[i];
+    s += t;
+  }
+  return sqrt(s);
+}
+/* sum: CuProceduresApi.cpp:358-388, GenericProceduresApi.h:207-286 */
+void orc_sum(const orc_matrix* m, double* re, double* im) {
+  size_t n = (size_t)m->columns * m->rows;
+  double sr = 0, si = 0;
+  for (size_t i = 0; i < n; ++i) {
+    if (m->re) sr += m->re[i];
+    if (m->im) si += m->im[i];
+  }
+  *re = sr;
+  *im = si;
+}
+
+/* ------------------------------------------------------------------ */
+/* Givens QR: CuQRProcedures_GR.h:38-150 (device, m*m <= 1024) and
+ * HostMatrixKernels.h:41-166 (host-orchestrated, larger m).
+ *  - rotation order: column fa = 0.., row fb = m-1 down to fa+1
+ *  - skip when |A[fb][fa]| of the INPUT matrix is below 1e-5 (device path,
+ *    m*m <= 1024) or 1e-10 (host path), as the reference does
+ *  - c = R1[fa][fa]/r, s = R1[fb][fa]/r; R = G R1, Q = Q1 G^T
+ * Divergences (fixes): r uses double sqrt (the device path uses sqrtf);
+ * complex entries use the unitary rotation G = [[conj c, conj s], [-s, c]]
+ * with Q = Q1 G^H, which equals the reference's G for real data; the final
+ * rotation is always kept (the reference's ping-pong drops it when the number
+ * of applied rotations is even: CuQRProcedures_GR.h:140-144,
+ * HostMatrixKernels.h:159-163); with no rotation R = A, Q = I. */
+/* ------------------------------------------------------------------ */
+int orc_qrgr(orc_matrix* Q, orc_matrix* R, const orc_matrix* A) {
+  const uint32_t m = A->columns;
+  if (A->rows != m || !A->re || !Q->re || !R->re) return 1;
+  const int cplx = A->im != NULL;
+  if (cplx && (!Q->im || !R->im)) return 1;
+  const double tol = (m * m <= 1024) ? 1e-5 : 1e-10;
+  size_t mm = (size_t)m * m;
+  double *rr = R->re, *ri = cplx ? R->im : NULL;
+  double *qr = Q->re, *qi = cplx ? Q->im : NULL;
+  memcpy(rr, A->re, mm * sizeof(double));
+  if (cplx) memcpy(ri, A->im, mm * sizeof(double));
+  for (size_t i = 0; i < mm; ++i) {
+    qr[i] = (i % (m + 1)) == 0 ? 1.0 : 0.0;
+    if (cplx) qi[i] = 0.0;
+  }
+  for (uint32_t fa = 0; fa < m; ++fa) {
+    for (uint32_t fb = m - 1; fb > fa; --fb) {
+      double v = A->re[fa + (size_t)fb * m];
+      if (cplx) v = hypot(v, A->im[fa + (size_t)fb * m]); /* complex: modulus */
+      if (-tol < v && v < tol) continue;
+      double cr = rr[fa + (size_t)fa * m], sr = rr[fa + (size_t)fb * m];
+      double ci = cplx ? ri[fa + (size_t)fa * m] : 0, si = cplx ? ri[fa + (size_t)fb * m] : 0;
+      double r = sqrt(cr * cr + sr * sr + ci * ci + si * si);
+      cr /= r; sr /= r; ci /= r; si /= r;
+      /* rows fa, fb of R: new_fa = conj(c) R_fa + conj(s) R_fb; new_fb = -s R_fa + c R_fb */
+      for (uint32_t x = 0; x < m; ++x) {
+        size_t ia = x + (size_t)fa * m, ib = x + (size_t)fb * m;
+        double ar = rr[ia], br = rr[ib];
+        double ai = cplx ? ri[ia] : 0, bi = cplx ? ri[ib] : 0;
+        rr[ia] = cr * ar + ci * ai + sr * br + si * bi;
+        rr[ib] = -(sr * ar - si * ai) + (cr * br - ci * bi);
+        if (cplx) {
+          ri[ia] = cr * ai - ci * ar + sr * bi - si * br;
+          ri[ib] = -(sr * ai + si * ar) + (cr * bi + ci * br);
+        }
+      }
+      /* columns fa, fb of Q: Q G^H, G^H = [[c, -conj s], [s, conj c]] */
+      for (uint32_t y = 0; y < m; ++y) {
+        size_t ia = fa + (size_t)y * m, ib = fb + (size_t)y * m;
+        double ar = qr[ia], br = qr[ib];
+        double ai = cplx ? qi[ia] : 0, bi = cplx ? qi[ib] : 0;
+        qr[ia] = (ar * cr - ai * ci) + (br * sr - bi * si);
+        qr[ib] = -(ar * sr + ai * si) + (br * cr + bi * ci);
+        if (cplx) {
+          qi[ia] = (ar * ci + ai * cr) + (br * si + bi * sr);
+          qi[ib] = -(ai * sr - ar * si) + (bi * cr - br * ci);
+        }
+      }
+    }
+  }
+  return 0;
+}
+
+/* isUpperTriangular: CuIsUpperTriangularProcedures.h:29-89, absolute 1e-3 on
+ * the first subdiagonal of every present plane. */
+int orc_is_upper_triangular(const orc_matrix* m) {
+  const double lim = 0.001;
+  for (uint32_t fa = 0; fa + 1 < m->columns; ++fa) {
+    size_t i = fa + (size_t)m->columns * (fa + 1);
+    if (m->re && !(-lim < m->re[i] && m->re[i] < lim)) return 0;
+    if (m->im && !(-lim < m->im[i] && m->im[i] < lim)) return 0;
+  }
+  return 1;
+}
+
+/* calcTriangularH: CuTriangularH.h:31-59 (device, 10000 steps) and
+ * oapGenericArnoldiApi.h:268-296 (host, 600 steps): H <- R Q until
+ * isUpperTriangular.  Divergence (fix): Q accumulates Q_1 Q_2 ... Q_k so that
+ * H_in = Q H_out Q^H (the reference multiplies on the left and, for an even
+ * step count, returns the previous product). */
+int orc_calc_triangular_h(orc_matrix* H, orc_matrix* Q, uint32_t max_steps) {
+  const uint32_t m = H->columns;
+  size_t mm = (size_t)m * m;
+  int cplx = H->im != NULL;
+  double* buf = (double*)calloc(8 * mm, sizeof(double));
+  orc_matrix q1 = {m, m, buf, cplx ? buf + mm : NULL};
+  orc_matrix r1 = {m, m, buf + 2 * mm, cplx ? buf + 3 * mm : NULL};
+  orc_matrix acc = {m, m, buf + 4 * mm, cplx ? buf + 5 * mm : NULL};
+  orc_matrix tmp = {m, m, buf + 6 * mm, cplx ? buf + 7 * mm : NULL};
+  orc_set_identity(&acc);
+  uint32_t steps = 0;
+  while (steps < max_steps && !orc_is_upper_triangular(H)) {
+    orc_qrgr(&q1, &r1, H);
+    orc_dot_product(H, &r1, &q1);
+    orc_dot_product(&tmp, &acc, &q1);
+    memcpy(acc.re, tmp.re, mm * sizeof(double));
+    if (cplx) memcpy(acc.im, tmp.im, mm * sizeof(double));
+    ++steps;
+  }
+  memcpy(Q->re, acc.re, mm * sizeof(double));
+  if (cplx && Q->im) memcpy(Q->im, acc.im, mm * sizeof(double));
+  free(buf);
+  return (int)steps;
+}
+
+/* ------------------------------------------------------------------ */
+/* Complex shifted Hessenberg QR (replaces the reference's unshifted
+ * iteration to a 1e-3 absolute triangularity, oapGenericArnoldiApi.h:250-298,
+ * which cannot separate complex pairs).  Textbook single-shift QR with
+ * Wilkinson shifts, deflation on |h_{l,l-1}| <= eps(|h_{l-1,l-1}|+|h_{l,l}|),
+ * exceptional shifts every 10 stalled sweeps, rotations in the convention
+ * G = [[c, s], [-conj s, c]] with real c (Anderson, LAPACK Working Note 150). */
+/* ------------------------------------------------------------------ */
+static void cgivens(double fr, double fi, double gr, double gi, double* c, double* sr,
+                    double* si) {
+  double af = hypot(fr, fi), ag = hypot(gr, gi);
+  if (ag == 0) { *c = 1; *sr = 0; *si = 0; return; }
+  if (af == 0) { *c = 0; *sr = gr / ag; *si = -gi / ag; return; }
+  double rho = hypot(af, ag);
+  *c = af / rho;
+  /* s = (f/|f|) conj(g) / rho */
+  double ur = fr / af, ui = fi / af;
+  *sr = (ur * gr + ui * gi) / rho;
+  *si = (ui * gr - ur * gi) / rho;
+}
+
+#define H_(i, j) ((size_t)(i) * m + (j))
+
+static void rot_rows(int m, double* hre, double* him, int i, int j0, int j1, double c,
+                     double sr, double si) {
+  for (int j = j0; j <= j1; ++j) {
+    double ar = hre[H_(i, j)], ai = him[H_(i, j)], br = hre[H_(i + 1, j)], bi = him[H_(i + 1, j)];
+    /* new_a = c a + s b ; new_b = -conj(s) a + c b */
+    hre[H_(i, j)] = c * ar + (sr * br - si * bi);
+    him[H_(i, j)] = c * ai + (sr * bi + si * br);
+    hre[H_(i + 1, j)] = -(sr * ar + si * ai) + c * br;
+    him[H_(i + 1, j)] = -(sr * ai - si * ar) + c * bi;
+  }
+}
+static void rot_cols(int m, double* hre, double* him, int j, int r0, int r1, double c,
+                     double sr, double si) {
+  for (int r = r0; r <= r1; ++r) {
+    double ar = hre[H_(r, j)], ai = him[H_(r, j)], br = hre[H_(r, j + 1)], bi = him[H_(r, j + 1)];
+    /* [a b] G^H = [a c + b conj(s), -a s + b c] */
+    hre[H_(r, j)] = ar * c + (br * sr + bi * si);
+    him[H_(r, j)] = ai * c + (bi * sr - br * si);
+    hre[H_(r, j + 1)] = -(ar * sr - ai * si) + br * c;
+    him[H_(r, j + 1)] = -(ar * si + ai * sr) + bi * c;
+  }
+}
+
+int orc_hess_schur(int m, double* hre, double* him, double* zre, double* zim, double* wr,
+                   double* wi) {
+  const double eps = 2.220446049250313e-16;
+  if (zre) {
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) { zre[H_(i, j)] = i == j; zim[H_(i, j)] = 0; }
+  }
+  for (int i = 2; i < m; ++i)
+    for (int j = 0; j < i - 1; ++j) { hre[H_(i, j)] = 0; him[H_(i, j)] = 0; }
+  int ihi = m - 1, iter = 0, total = 0, status = 0;
+  while (ihi >= 0) {
+    int l;
+    for (l = ihi; l > 0; --l) {
+      double s = fabs(hre[H_(l - 1, l - 1)]) + fabs(him[H_(l - 1, l - 1)]) + fabs(hre[H_(l, l)]) +
+                 fabs(him[H_(l, l)]);
+      double sub = fabs(hre[H_(l, l - 1)]) + fabs(him[H_(l, l - 1)]);
+      if (sub <= eps * s || sub < 1e-300) { hre[H_(l, l - 1)] = 0; him[H_(l, l - 1)] = 0; break; }
+    }
+    if (l == ihi) { --ihi; iter = 0; continue; }
+    if (++total > 100 * m) { status = -1; break; }
+    double mr, mi;
+    if (iter > 0 && iter % 10 == 0) {
+      /* exceptional shift */
+      mr = hre[H_(ihi, ihi)] + 0.75 * fabs(hre[H_(ihi, ihi - 1)]);
+      mi = him[H_(ihi, ihi)];
+    } else {
+      /* Wilkinson: eigenvalue of the trailing 2x2 closer to h[ihi][ihi] */
+      double ar = hre[H_(ihi - 1, ihi - 1)], ai = him[H_(ihi - 1, ihi - 1)];
+      double br = hre[H_(ihi - 1, ihi)], bi = him[H_(ihi - 1, ihi)];
+      double cr = hre[H_(ihi, ihi - 1)], ci = him[H_(ihi, ihi - 1)];
+      double dr = hre[H_(ihi, ihi)], di = him[H_(ihi, ihi)];
+      double hr = 0.5 * (ar - dr), hi = 0.5 * (ai - di);
+      /* disc = sqrt(h^2 + b c) */
+      double qr = hr * hr - hi * hi + (br * cr - bi * ci), qi = 2 * hr * hi + (br * ci + bi * cr);
+      double mod = hypot(qr, qi);
+      double sr = sqrt(0.5 * (mod + qr)), si = sqrt(0.5 * (mod - qr));
+      if (qi < 0) si = -si;
+      /* mu = d + h -/+ disc, choose the root closer to d: d + h - sign*disc */
+      double t1r = hr + sr, t1i = hi + si, t2r = hr - sr, t2i = hi - si;
+      if (t1r * t1r + t1i * t1i < t2r * t2r + t2i * t2i) { mr = dr + t1r; mi = di + t1i; }
+      else { mr = dr + t2r; mi = di + t2i; }
+    }
+    ++iter;
+    for (int kk = l; kk < ihi; ++kk) {
+      double fr, fi, gr, gi;
+      if (kk == l) {
+        fr = hre[H_(l, l)] - mr; fi = him[H_(l, l)] - mi;
+        gr = hre[H_(l + 1, l)]; gi = him[H_(l + 1, l)];
+      } else {
+        fr = hre[H_(kk, kk - 1)]; fi = him[H_(kk, kk - 1)];
+        gr = hre[H_(kk + 1, kk - 1)]; gi = him[H_(kk + 1, kk - 1)];
+      }
+      double c, sr, si;
+      cgivens(fr, fi, gr, gi, &c, &sr, &si);
+      int j0 = kk > l ? kk - 1 : kk;
+      rot_rows(m, hre, him, kk, j0, m - 1, c, sr, si);
+      if (kk > l) { hre[H_(kk + 1, kk - 1)] = 0; him[H_(kk + 1, kk - 1)] = 0; }
+      int r1 = kk + 2 < ihi ? kk + 2 : ihi;
+      rot_cols(m, hre, him, kk, 0, r1, c, sr, si);
+      if (zre) rot_cols(m, zre, zim, kk, 0, m - 1, c, sr, si);
+    }
+  }
+  for (int i = 0; i < m; ++i) { wr[i] = hre[H_(i, i)]; wi[i] = him[H_(i, i)]; }
+  return status;
+}
+
+void orc_trevc(int m, const double* tre, const double* tim, double* xre, double* xim) {
+  double tnorm = 0;
+  for (int i = 0; i < m; ++i)
+    for (int j = i; j < m; ++j) tnorm = fmax(tnorm, hypot(tre[H_(i, j)], tim[H_(i, j)]));
+  double small = fmax(tnorm * 2.220446049250313e-16, 1e-300);
+  for (int i = 0; i < m; ++i) {
+    for (int j = 0; j < m; ++j) { xre[H_(j, i)] = 0; xim[H_(j, i)] = 0; }
+    xre[H_(i, i)] = 1;
+    double lr = tre[H_(i, i)], li = tim[H_(i, i)];
+    for (int j = i - 1; j >= 0; --j) {
+      double sr = 0, si = 0;
+      for (int l = j + 1; l <= i; ++l) {
+        sr += tre[H_(j, l)] * xre[H_(l, i)] - tim[H_(j, l)] * xim[H_(l, i)];
+        si += tre[H_(j, l)] * xim[H_(l, i)] + tim[H_(j, l)] * xre[H_(l, i)];
+      }
+      double dr = tre[H_(j, j)] - lr, di = tim[H_(j, j)] - li;
+      if (hypot(dr, di) < small) { dr = small; di = 0; }
+      double den = dr * dr + di * di;
+      /* x_j = -s / d */
+      xre[H_(j, i)] = -(sr * dr + si * di) / den;
+      xim[H_(j, i)] = -(si * dr - sr * di) / den;
+    }
+  }
+}
+#undef H_
+
+/* ------------------------------------------------------------------ */
+/* dense matvec (the reference's CuHArnoldiDefault::multiply =
+ * dotProduct(w, A, v), ArnoldiProceduresImpl.cpp:26-30)                */
+/* ------------------------------------------------------------------ */
+typedef struct mv_job {
+  int n, r0, r1;
+  const double *are, *aim, *xre, *xim;
+  double *yre, *yim;
+} mv_job;
+
+static void* mv_worker(void* p) {
+  mv_job* j = (mv_job*)p;
+  int n = j->n;
+  for (int r = j->r0; r < j->r1; ++r) {
+    const double* ar = j->are + (size_t)r * n;
+    const double* ai = j->aim ? j->aim + (size_t)r * n : NULL;
+    double sr = 0, si = 0;
+    for (int c = 0; c < n; ++c) {
+      double xr = j->xre[c], xi = j->xim ? j->xim[c] : 0;
+      double a_r = ar[c], a_i = ai ? ai[c] : 0;
+      sr += a_r * xr - a_i * xi;
+      si += a_r * xi + a_i * xr;
+    }
+    j->yre[r] = sr;
+    if (j->yim) j->yim[r] = si;
+  }
+  return NULL;
+}
+
+void orc_dense_matvec(int n, const double* are, const double* aim, const double* xre,
+                      const double* xim, double* yre, double* yim, int threads) {
+  if (threads < 1) threads = 1;
+  if (threads > 256) threads = 256;
+  pthread_t tid[256];
+  mv_job jobs[256];
+  int per = (n + threads - 1) / threads;
+  for (int t = 0; t < threads; ++t) {
+    int r0 = t * per, r1 = r0 + per > n ? n : r0 + per;
+    jobs[t] = (mv_job){n, r0, r1, are, aim, xre, xim, yre, yim};
+    if (threads == 1) mv_worker(&jobs[t]);
+    else pthread_create(&tid[t], NULL, mv_worker, &jobs[t]);
+  }
+  if (threads > 1)
+    for (int t = 0; t < threads; ++t) pthread_join(tid[t], NULL);
+}
+
+/* ------------------------------------------------------------------ */
+/* seeded generators                                                    */
+/* ------------------------------------------------------------------ */
+uint64_t orc_splitmix64(uint64_t x) {
+  /* SplitMix64 output function (Steele, Lea, Flood 2014) */
+  uint64_t z = x;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+double orc_uniform(uint64_t seed, uint64_t idx) {
+  uint64_t z = orc_splitmix64(seed + 0x9E3779B97F4A7C15ULL * (idx + 1));
+  return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+}
+double orc_normal(uint64_t seed, uint64_t idx) {
+  double u1 = orc_uniform(seed, 2 * idx), u2 = orc_uniform(seed, 2 * idx + 1);
+  return sqrt(-2.0 * log(1.0 - u1)) * cos(6.283185307179586 * u2);
+}
+void orc_gen_c0(int n, uint64_t seed, double* are) {
+  double sc = 0.05 / sqrt((double)n);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c)
+      are[(size_t)r * n + c] = (r == c ? 1.0 / (1.0 + r) : 0.0) + sc * orc_normal(seed, (uint64_t)r * n + c);
+}
+void orc_gen_c1(int n, uint64_t seed, double eps, double decay, double* are, double* aim) {
+  double sc = eps / sqrt((double)n);
+  for (int r = 0; r < n; ++r)
+    for (int c = 0; c < n; ++c) {
+      uint64_t i = (uint64_t)r * n + c;
+      are[i] = sc * orc_normal(seed, 2 * i);
+      aim[i] = sc * orc_normal(seed, 2 * i + 1);
+    }
+  for (int k = 0; k < n; ++k) {
+    double u = orc_uniform(seed ^ 0xD1A60000ULL, (uint64_t)k);
+    double mag = exp(-(double)k / decay);
+    are[(size_t)k * n + k] += mag * cos(6.283185307179586 * u);
+    aim[(size_t)k * n + k] += mag * sin(6.283185307179586 * u);
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Implicitly restarted Arnoldi.  Restates the step sequence of
+ * CuHArnoldi::execute/step (ArnoldiPackage/ArnoldiProcedures.cpp:129-215):
+ * init (oapGenericArnoldiApi.h:61-71) -> factorization j = start..m-1 with one
+ * re-orthogonalisation pass (ArnoldiProcedures.cpp:407-461) -> Ritz values of
+ * H sorted by the wanted order (:351-398, ArnoldiUtils.cpp:24-50) -> p = m-k
+ * exact shifts (oapGenericArnoldiApi.h:139-223) -> V <- V Q
+ * (ArnoldiProcedures.cpp:509-519) -> f <- V[:,k] H[k][k-1] + f Q[m-1][k-1]
+ * (oapGenericArnoldiApi.h:73-102) -> stop check (:470-507).
+ * Fixes that make the method converge to FP64 residuals: V^H (not V^T),
+ * complex scaling, correctly accumulated Q, Ritz values from the shifted
+ * complex Hessenberg QR above, Ritz vectors V y instead of basis columns,
+ * residual-estimate stop |beta_m e_m^T y| <= tol*anorm, seeded start vector,
+ * the ARPACK rule k_eff = k + min(nconv, p/2) against stagnation, and a
+ * second classical Gram-Schmidt pass on every step (DGKS). */
+/* ------------------------------------------------------------------ */
+typedef struct { double re, im; } cplx_t;
+
+static int ritz_before(const cplx_t* a, const cplx_t* b, int which) {
+  /* ArnUtils::Sort* comparators: 0 LargestValues, 1 LargestRe, 2 LargestIm,
+   * 3 SmallestValues, 4 SmallestRe, 5 SmallestIm */
+  double ma = a->re * a->re + a->im * a->im, mb = b->re * b->re + b->im * b->im;
+  switch (which) {
+    case 0: return ma > mb;
+    case 1: return a->re > b->re;
+    case 2: return a->im > b->im;
+    case 3: return ma < mb;
+    case 4: return a->re < b->re;
+    default: return a->im < b->im;
+  }
+}
+
+/* order[0..m) = indices of theta sorted by `which`; stable insertion sort */
+static void ritz_order(int m, const double* wr, const double* wi, int which, int* order) {
+  for (int i = 0; i < m; ++i) order[i] = i;
+  for (int i = 1; i < m; ++i) {
+    int v = order[i], j = i - 1;
+    cplx_t cv = {wr[v], wi[v]};
+    while (j >= 0) {
+      cplx_t cj = {wr[order[j]], wi[order[j]]};
+      if (!ritz_before(&cv, &cj, which)) break;
+      order[j + 1] = order[j];
+      --j;
+    }
+    order[j + 1] = v;
+  }
+}
+
+/* one implicit single-shift QR sweep on the k x k leading block of the
+ * m-stride Hessenberg H, accumulating Q (m x m, stride m) on the right */
+static void shifted_sweep(int m, int kdim, double* hre, double* him, double* qre, double* qim,
+                          double mr, double mi) {
+#define H_(i, j) ((size_t)(i) * m + (j))
+  for (int kk = 0; kk < kdim - 1; ++kk) {
+    double fr, fi, gr, gi;
+    if (kk == 0) {
+      fr = hre[H_(0, 0)] - mr; fi = him[H_(0, 0)] - mi;
+      gr = hre[H_(1, 0)]; gi = him[H_(1, 0)];
+    } else {
+      fr = hre[H_(kk, kk - 1)]; fi = him[H_(kk, kk - 1)];
+      gr = hre[H_(kk + 1, kk - 1)]; gi = him[H_(kk + 1, kk - 1)];
+    }
+    double c, sr, si;
+    cgivens(fr, fi, gr, gi, &c, &sr, &si);
+    int j0 = kk > 0 ? kk - 1 : 0;
+    rot_rows(m, hre, him, kk, j0, kdim - 1, c, sr, si);
+    if (kk > 0) { hre[H_(kk + 1, kk - 1)] = 0; him[H_(kk + 1, kk - 1)] = 0; }
+    int r1 = kk + 2 < kdim - 1 ? kk + 2 : kdim - 1;
+    rot_cols(m, hre, him, kk, 0, r1, c, sr, si);
+    rot_cols(m, qre, qim, kk, 0, kdim - 1, c, sr, si);
+  }
+#undef H_
+}
+
+static double vnorm(int n, const double* re, const double* im) {
+  double s = 0;
+  for (int i = 0; i < n; ++i) s += re[i] * re[i] + im[i] * im[i];
+  return sqrt(s);
+}
+
+/* f -= V[:, 0..j] (V^H f), accumulating the coefficients into h */
+static void cgs_pass(int n, int j, const double* vre, const double* vim, double* fre,
+                     double* fim, double* hre, double* him) {
+  for (int c = 0; c <= j; ++c) {
+    const double *ar = vre + (size_t)c * n, *ai = vim + (size_t)c * n;
+    double sr = 0, si = 0;
+    for (int i = 0; i < n; ++i) {
+      sr += ar[i] * fre[i] + ai[i] * fim[i];
+      si += ar[i] * fim[i] - ai[i] * fre[i];
+    }
+    hre[c] += sr;
+    him[c] += si;
+  }
+  /* classical: subtract with the coefficients of this pass only */
+}
+
+int orc_iram(const orc_iram_opts* o, orc_matvec_fn mv, void* user, double* ev_re,
+             double* ev_im, double* x_re, double* x_im, double* resid, orc_iram_result* res) {
+  const int n = o->n, m = o->m, k = o->k;
+  if (k < 1 || m <= k || m > n) return 1;
+  size_t nm = (size_t)n * (m + 1);
+  double* vre = (double*)calloc(nm, sizeof(double));
+  double* vim = (double*)calloc(nm, sizeof(double));
+  double* wre = (double*)calloc((size_t)n * 4, sizeof(double));
+  double *wim = wre + n, *fre = wre + 2 * (size_t)n, *fim = wre + 3 * (size_t)n;
+  size_t mm = (size_t)m * m;
+  double* sm = (double*)calloc(mm * 8 + (size_t)m * 8, sizeof(double));
+  double *hre = sm, *him = sm + mm, *tre = sm + 2 * mm, *tim = sm + 3 * mm;
+  double *zre = sm + 4 * mm, *zim = sm + 5 * mm, *qre = sm + 6 * mm, *qim = sm + 7 * mm;
+  double *wr = sm + 8 * mm, *wi = wr + m, *cr = wr + 2 * m, *ci = wr + 3 * m;
+  double *yre = wr + 4 * m, *yim = wr + 5 * m, *est = wr + 6 * m;
+  double* xre_h = (double*)calloc(mm * 2, sizeof(double));
+  double* xim_h = xre_h + mm;
+  double* vtmp_re = (double*)calloc((size_t)n * m * 2, sizeof(double));
+  double* vtmp_im = vtmp_re + (size_t)n * m;
+  int* order = (int*)calloc((size_t)m, sizeof(int));
+  int matvecs = 0, restarts = 0, nconv = 0, status = 0;
+
+  /* start vector: initVvector_rand (ArnoldiProcedures.cpp:289-319), uniform
+   * in [-1, 1], normalised; seeded (reference: std::random_device). */
+  for (int i = 0; i < n; ++i) {
+    vre[i] = 2.0 * orc_uniform(o->seed, (uint64_t)i) - 1.0;
+    vim[i] = o->real_start ? 0.0 : 2.0 * orc_uniform(o->seed ^ 0xA5A5ULL, (uint64_t)i) - 1.0;
+  }
+  {
+    double nv = vnorm(n, vre, vim);
+    for (int i = 0; i < n; ++i) { vre[i] /= nv; vim[i] /= nv; }
+  }
+  int start = 0;
+  double beta = 0;
+  for (;;) {
+    /* Arnoldi factorisation columns start..m-1 */
+    for (int j = start; j < m; ++j) {
+      double *vj_re = vre + (size_t)j * n, *vj_im = vim + (size_t)j * n;
+      mv(user, vj_re, vj_im, wre, wim);
+      ++matvecs;
+      for (int i = 0; i <= j; ++i) { cr[i] = 0; ci[i] = 0; }
+      memcpy(fre, wre, (size_t)n * sizeof(double));
+      memcpy(fim, wim, (size_t)n * sizeof(double));
+      for (int pass = 0; pass < 2; ++pass) {
+        double sr_[512], si_[512];
+        double *sr = m <= 512 ? sr_ : (double*)calloc((size_t)m, sizeof(double));
+        double *si = m <= 512 ? si_ : (double*)calloc((size_t)m, sizeof(double));
+        for (int i = 0; i <= j; ++i) { sr[i] = 0; si[i] = 0; }
+        cgs_pass(n, j, vre, vim, fre, fim, sr, si);
+        for (int c = 0; c <= j; ++c) {
+          const double *ar = vre + (size_t)c * n, *ai = vim + (size_t)c * n;
+          for (int i = 0; i < n; ++i) {
+            fre[i] -= ar[i] * sr[c] - ai[i] * si[c];
+            fim[i] -= ar[i] * si[c] + ai[i] * sr[c];
+          }
+          cr[c] += sr[c];
+          ci[c] += si[c];
+        }
+        if (sr != sr_) { free(sr); free(si); }
+      }
+      for (int i = 0; i <= j; ++i) { hre[(size_t)i * m + j] = cr[i]; him[(size_t)i * m + j] = ci[i]; }
+      beta = vnorm(n, fre, fim);
+      if (j + 1 < m) {
+        if (beta < 1e-14 * (1.0 + vnorm(j + 1, cr, ci))) {
+          /* breakdown (ArnoldiProcedures.cpp:429-433 stops here); restart the
+           * next column from a fresh random vector orthogonal to V */
+          for (int i = 0; i < n; ++i) {
+            fre[i] = 2.0 * orc_uniform(o->seed + 77 + (uint64_t)j, (uint64_t)i) - 1.0;
+            fim[i] = 0;
+          }
+          for (int pass = 0; pass < 2; ++pass) {
+            double sr[512] = {0}, si[512] = {0};
+            cgs_pass(n, j, vre, vim, fre, fim, sr, si);
+            for (int c = 0; c <= j; ++c) {
+              const double *ar = vre + (size_t)c * n, *ai = vim + (size_t)c * n;
+              for (int i = 0; i < n; ++i) {
+                fre[i] -= ar[i] * sr[c] - ai[i] * si[c];
+                fim[i] -= ar[i] * si[c] + ai[i] * sr[c];
+              }
+            }
+          }
+          double nb = vnorm(n, fre, fim);
+          for (int i = 0; i < n; ++i) { vre[(size_t)(j + 1) * n + i] = fre[i] / nb; vim[(size_t)(j + 1) * n + i] = fim[i] / nb; }
+          hre[(size_t)(j + 1) * m + j] = 0; him[(size_t)(j + 1) * m + j] = 0;
+          beta = 0;
+        } else {
+          for (int i = 0; i < n; ++i) { vre[(size_t)(j + 1) * n + i] = fre[i] / beta; vim[(size_t)(j + 1) * n + i] = fim[i] / beta; }
+          hre[(size_t)(j + 1) * m + j] = beta; him[(size_t)(j + 1) * m + j] = 0;
+        }
+      }
+    }
+    /* Ritz values/vectors of H */
+    memcpy(tre, hre, mm * sizeof(double));
+    memcpy(tim, him, mm * sizeof(double));
+    if (orc_hess_schur(m, tre, tim, zre, zim, wr, wi) != 0) { status = 2; break; }
+    orc_trevc(m, tre, tim, xre_h, xim_h);
+    ritz_order(m, wr, wi, o->which, order);
+    nconv = 0;
+    for (int t = 0; t < m; ++t) {
+      int i = order[t];
+      /* y = Z x_i, normalised; estimate = beta |y[m-1]| */
+      double ny = 0;
+      for (int r = 0; r < m; ++r) {
+        double ar = 0, ai = 0;
+        for (int l = 0; l <= i; ++l) {
+          ar += zre[(size_t)r * m + l] * xre_h[(size_t)l * m + i] - zim[(size_t)r * m + l] * xim_h[(size_t)l * m + i];
+          ai += zre[(size_t)r * m + l] * xim_h[(size_t)l * m + i] + zim[(size_t)r * m + l] * xre_h[(size_t)l * m + i];
+        }
+        if (r == m - 1) { yre[0] = ar; yim[0] = ai; }
+        ny += ar * ar + ai * ai;
+      }
+      est[t] = beta * hypot(yre[0], yim[0]) / sqrt(ny);
+      if (t < k && est[t] <= o->tol * o->anorm) ++nconv;
+    }
+    if (nconv >= k || restarts >= o->max_restarts) break;
+    /* exact-shift restart with p = m - kk unwanted Ritz values */
+    int kk = k + (nconv < (m - k) / 2 ? nconv : (m - k) / 2);
+    if (kk == 1 && m >= 6) kk = m / 2;
+    else if (kk == 1 && m > 2) kk = 2;
+    for (int i = 0; i < m; ++i)
+      for (int j = 0; j < m; ++j) { qre[(size_t)i * m + j] = i == j; qim[(size_t)i * m + j] = 0; }
+    for (int t = kk; t < m; ++t) shifted_sweep(m, m, hre, him, qre, qim, wr[order[t]], wi[order[t]]);
+    /* V <- V Q[:, 0..kk] ; f <- V Q[:, kk] H[kk][kk-1] + f Q[m-1][kk-1] */
+    double bk_r = hre[(size_t)kk * m + kk - 1], bk_i = him[(size_t)kk * m + kk - 1];
+    double sg_r = qre[(size_t)(m - 1) * m + kk - 1], sg_i = qim[(size_t)(m - 1) * m + kk - 1];
+    for (int c = 0; c <= kk; ++c) {
+      double* tr = vtmp_re + (size_t)c * n;
+      double* ti = vtmp_im + (size_t)c * n;
+      for (int i = 0; i < n; ++i) { tr[i] = 0; ti[i] = 0; }
+      for (int l = 0; l < m; ++l) {
+        double q_r = qre[(size_t)l * m + c], q_i = qim[(size_t)l * m + c];
+        const double *ar = vre + (size_t)l * n, *ai = vim + (size_t)l * n;
+        for (int i = 0; i < n; ++i) {
+          tr[i] += ar[i] * q_r - ai[i] * q_i;
+          ti[i] += ar[i] * q_i + ai[i] * q_r;
+        }
+      }
+    }
+    for (int i = 0; i < n; ++i) {
+      double a_r = vtmp_re[(size_t)kk * n + i], a_i = vtmp_im[(size_t)kk * n + i];
+      double nr = (a_r * bk_r - a_i * bk_i) + (fre[i] * sg_r - fim[i] * sg_i);
+      double ni = (a_r * bk_i + a_i * bk_r) + (fre[i] * sg_i + fim[i] * sg_r);
+      fre[i] = nr;
+      fim[i] = ni;
+    }
+    memcpy(vre, vtmp_re, (size_t)n * kk * sizeof(double));
+    memcpy(vim, vtmp_im, (size_t)n * kk * sizeof(double));
+    for (int i = kk; i < m; ++i)
+      for (int j = 0; j < m; ++j) { hre[(size_t)i * m + j] = 0; him[(size_t)i * m + j] = 0; }
+    for (int i = 0; i < kk; ++i)
+      for (int j = kk; j < m; ++j) { hre[(size_t)i * m + j] = 0; him[(size_t)i * m + j] = 0; }
+    beta = vnorm(n, fre, fim);
+    for (int i = 0; i < n; ++i) { vre[(size_t)kk * n + i] = fre[i] / beta; vim[(size_t)kk * n + i] = fim[i] / beta; }
+    hre[(size_t)kk * m + kk - 1] = beta; him[(size_t)kk * m + kk - 1] = 0;
+    start = kk;
+    ++restarts;
+  }
+  /* Ritz vectors x_t = V Z x_i for the k wanted values, explicit residuals */
+  for (int t = 0; t < k && status == 0; ++t) {
+    int i = order[t];
+    ev_re[t] = wr[i];
+    ev_im[t] = wi[i];
+    for (int r = 0; r < m; ++r) {
+      double ar = 0, ai = 0;
+      for (int l = 0; l <= i; ++l) {
+        ar += zre[(size_t)r * m + l] * xre_h[(size_t)l * m + i] - zim[(size_t)r * m + l] * xim_h[(size_t)l * m + i];
+        ai += zre[(size_t)r * m + l] * xim_h[(size_t)l * m + i] + zim[(size_t)r * m + l] * xre_h[(size_t)l * m + i];
+      }
+      yre[r] = ar;
+      yim[r] = ai;
+    }
+    double *xr = x_re + (size_t)t * n, *xi = x_im + (size_t)t * n;
+    for (int q = 0; q < n; ++q) {
+      double sr = 0, si = 0;
+      for (int r = 0; r < m; ++r) {
+        sr += vre[(size_t)r * n + q] * yre[r] - vim[(size_t)r * n + q] * yim[r];
+        si += vre[(size_t)r * n + q] * yim[r] + vim[(size_t)r * n + q] * yre[r];
+      }
+      xr[q] = sr;
+      xi[q] = si;
+    }
+    double nx = vnorm(n, xr, xi);
+    for (int q = 0; q < n; ++q) { xr[q] /= nx; xi[q] /= nx; }
+    mv(user, xr, xi, wre, wim);
+    double s = 0;
+    for (int q = 0; q < n; ++q) {
+      double rr = wre[q] - (ev_re[t] * xr[q] - ev_im[t] * xi[q]);
+      double ri = wim[q] - (ev_re[t] * xi[q] + ev_im[t] * xr[q]);
+      s += rr * rr + ri * ri;
+    }
+    resid[t] = sqrt(s);
+  }
+  if (res) { res->nconv = nconv; res->restarts = restarts; res->matvecs = matvecs; }
+  free(vre); free(vim); free(wre); free(sm); free(xre_h); free(vtmp_re); free(order);
+  return status;
+}

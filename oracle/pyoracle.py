@@ -1,0 +1,313 @@
+"""ctypes wrappers of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg import
+this module, as the checker.  `Oracle` wraps liboracle.so (the C restatement
+in oracle.c); `Ref` wraps _ref/libref.so (the reference's own host sources,
+built only where the reference tree is present).
+
+Matrices are passed as complex numpy arrays of shape (rows, columns) plus
+flags for which planes exist (a plane that does not exist is NULL).
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+DP = C.POINTER(C.c_double)
+U32 = C.c_uint32
+
+
+class orc_matrix(C.Structure):
+    _fields_ = [("columns", U32), ("rows", U32), ("re", DP), ("im", DP)]
+
+
+class orc_iram_opts(C.Structure):
+    _fields_ = [("n", C.c_int), ("m", C.c_int), ("k", C.c_int), ("which", C.c_int),
+                ("tol", C.c_double), ("anorm", C.c_double), ("max_restarts", C.c_int),
+                ("seed", C.c_uint64), ("real_start", C.c_int)]
+
+
+class orc_iram_result(C.Structure):
+    _fields_ = [("nconv", C.c_int), ("restarts", C.c_int), ("matvecs", C.c_int)]
+
+
+MATVEC_FN = C.CFUNCTYPE(None, C.c_void_p, DP, DP, DP, DP)
+
+
+class Planes:
+    """Host matrix with optional planes (keeps the numpy buffers alive)."""
+
+    def __init__(self, rows: int, cols: int, is_re=True, is_im=True, z=None):
+        self.rows, self.cols = rows, cols
+        self.re = np.zeros((rows, cols)) if is_re else None
+        self.im = np.zeros((rows, cols)) if is_im else None
+        if z is not None:
+            z = np.asarray(z).reshape(rows, cols)
+            if self.re is not None:
+                self.re[:] = np.real(z)
+            if self.im is not None:
+                self.im[:] = np.imag(z)
+
+    @classmethod
+    def of(cls, z, is_re=True, is_im=True):
+        z = np.asarray(z)
+        if z.ndim == 1:
+            z = z.reshape(-1, 1)
+        return cls(z.shape[0], z.shape[1], is_re, is_im, z)
+
+    def c(self) -> orc_matrix:
+        return orc_matrix(self.cols, self.rows, _p(self.re), _p(self.im))
+
+    def z(self) -> np.ndarray:
+        out = np.zeros((self.rows, self.cols), dtype=np.complex128)
+        if self.re is not None:
+            out.real = self.re
+        if self.im is not None:
+            out.imag = self.im
+        return out
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(DP)
+
+
+def _dims(d):
+    arr = ((U32 * 2) * 3)()
+    for i in range(3):
+        for j in range(2):
+            arr[i][j] = int(d[i][j])
+    return arr
+
+
+class Oracle:
+    def __init__(self, path: str | Path | None = None):
+        p = Path(path) if path else HERE / "liboracle.so"
+        if not p.exists():
+            raise FileNotFoundError(f"{p} missing: run `make -C oracle`")
+        self.lib = C.CDLL(str(p))
+        L = self.lib
+        MP = C.POINTER(orc_matrix)
+        for name in ("orc_dot_product", "orc_transpose", "orc_conjugate_transpose",
+                     "orc_hadamard_product", "orc_hadamard_product_vec", "orc_add", "orc_substract"):
+            getattr(L, name).argtypes = [MP, MP] + ([MP] if name not in (
+                "orc_transpose", "orc_conjugate_transpose") else [])
+            getattr(L, name).restype = C.c_int
+        L.orc_dot_product_dims.argtypes = [MP, MP, MP, (U32 * 2) * 3]
+        L.orc_tensor_product.argtypes = [MP, MP, MP, (U32 * 2) * 3]
+        L.orc_multiply_re_constant.argtypes = [MP, MP, C.c_double]
+        L.orc_multiply_constant.argtypes = [MP, MP, C.c_double, C.c_double]
+        L.orc_set_diagonal.argtypes = [MP, C.c_double, C.c_double]
+        L.orc_set_identity.argtypes = [MP]
+        L.orc_set_vector.argtypes = [MP, U32, MP, U32]
+        L.orc_get_vector.argtypes = [MP, U32, MP, U32]
+        L.orc_magnitude.argtypes = [MP]
+        L.orc_magnitude.restype = C.c_double
+        L.orc_sum.argtypes = [MP, DP, DP]
+        L.orc_qrgr.argtypes = [MP, MP, MP]
+        L.orc_is_upper_triangular.argtypes = [MP]
+        L.orc_calc_triangular_h.argtypes = [MP, MP, U32]
+        L.orc_set_threads_blocks.argtypes = [C.POINTER(U32), C.POINTER(U32), U32, U32, U32]
+        L.orc_hess_schur.argtypes = [C.c_int] + [DP] * 6
+        L.orc_trevc.argtypes = [C.c_int] + [DP] * 4
+        L.orc_iram.argtypes = [C.POINTER(orc_iram_opts), MATVEC_FN, C.c_void_p] + [DP] * 5 + [
+            C.POINTER(orc_iram_result)]
+        L.orc_dense_matvec.argtypes = [C.c_int] + [DP] * 6 + [C.c_int]
+        L.orc_uniform.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_uniform.restype = C.c_double
+        L.orc_normal.argtypes = [C.c_uint64, C.c_uint64]
+        L.orc_normal.restype = C.c_double
+        L.orc_splitmix64.argtypes = [C.c_uint64]
+        L.orc_splitmix64.restype = C.c_uint64
+        L.orc_gen_c0.argtypes = [C.c_int, C.c_uint64, DP]
+        L.orc_gen_c1.argtypes = [C.c_int, C.c_uint64, C.c_double, C.c_double, DP, DP]
+
+    # ---- procedures (return the output Planes)
+    def dot_product(self, out: Planes, a: Planes, b: Planes, dims=None) -> int:
+        if dims is None:
+            return self.lib.orc_dot_product(C.byref(out.c()), C.byref(a.c()), C.byref(b.c()))
+        return self.lib.orc_dot_product_dims(C.byref(out.c()), C.byref(a.c()), C.byref(b.c()),
+                                             _dims(dims))
+
+    def transpose(self, out: Planes, a: Planes, conj=False) -> int:
+        f = self.lib.orc_conjugate_transpose if conj else self.lib.orc_transpose
+        return f(C.byref(out.c()), C.byref(a.c()))
+
+    def tensor_product(self, out, a, b, dims) -> int:
+        return self.lib.orc_tensor_product(C.byref(out.c()), C.byref(a.c()), C.byref(b.c()),
+                                           _dims(dims))
+
+    def binary(self, op: str, out, a, b) -> int:
+        f = {"add": self.lib.orc_add, "substract": self.lib.orc_substract,
+             "hadamard": self.lib.orc_hadamard_product,
+             "hadamard_vec": self.lib.orc_hadamard_product_vec}[op]
+        return f(C.byref(out.c()), C.byref(a.c()), C.byref(b.c()))
+
+    def multiply_constant(self, out, a, re, im=0.0, re_only=False) -> int:
+        if re_only:
+            return self.lib.orc_multiply_re_constant(C.byref(out.c()), C.byref(a.c()), re)
+        return self.lib.orc_multiply_constant(C.byref(out.c()), C.byref(a.c()), re, im)
+
+    def set_diagonal(self, m, re, im) -> int:
+        return self.lib.orc_set_diagonal(C.byref(m.c()), re, im)
+
+    def set_vector(self, V, column, v, length) -> int:
+        return self.lib.orc_set_vector(C.byref(V.c()), column, C.byref(v.c()), length)
+
+    def get_vector(self, v, length, V, column) -> int:
+        return self.lib.orc_get_vector(C.byref(v.c()), length, C.byref(V.c()), column)
+
+    def magnitude(self, m) -> float:
+        return self.lib.orc_magnitude(C.byref(m.c()))
+
+    def sum(self, m):
+        r, i = C.c_double(), C.c_double()
+        self.lib.orc_sum(C.byref(m.c()), C.byref(r), C.byref(i))
+        return r.value, i.value
+
+    def qrgr(self, Q, R, A) -> int:
+        return self.lib.orc_qrgr(C.byref(Q.c()), C.byref(R.c()), C.byref(A.c()))
+
+    def is_upper_triangular(self, m) -> bool:
+        return bool(self.lib.orc_is_upper_triangular(C.byref(m.c())))
+
+    def calc_triangular_h(self, H, Q, max_steps=10000) -> int:
+        return self.lib.orc_calc_triangular_h(C.byref(H.c()), C.byref(Q.c()), max_steps)
+
+    def set_threads_blocks(self, w, h, limit=1024):
+        b, t = (U32 * 2)(), (U32 * 2)()
+        self.lib.orc_set_threads_blocks(b, t, w, h, limit)
+        return (b[0], b[1]), (t[0], t[1])
+
+    # ---- eigen-machinery
+    def hess_schur(self, H: np.ndarray):
+        m = H.shape[0]
+        hr = np.ascontiguousarray(H.real, dtype=np.float64).copy()
+        hi = np.ascontiguousarray(H.imag, dtype=np.float64).copy()
+        zr, zi = np.zeros((m, m)), np.zeros((m, m))
+        wr, wi = np.zeros(m), np.zeros(m)
+        st = self.lib.orc_hess_schur(m, _p(hr), _p(hi), _p(zr), _p(zi), _p(wr), _p(wi))
+        return st, hr + 1j * hi, zr + 1j * zi, wr + 1j * wi
+
+    def iram(self, matvec, n, m, k, which=0, tol=1e-10, anorm=1.0, max_restarts=1000, seed=1,
+             real_start=True):
+        """matvec(x complex (n,)) -> complex (n,).  Returns (evals, evecs (k, n), resid, stats)."""
+
+        def cb(_u, xr, xi, yr, yi):
+            x = np.ctypeslib.as_array(xr, (n,)) + 1j * np.ctypeslib.as_array(xi, (n,))
+            y = matvec(x)
+            np.ctypeslib.as_array(yr, (n,))[:] = y.real
+            np.ctypeslib.as_array(yi, (n,))[:] = y.imag
+
+        fn = MATVEC_FN(cb)
+        o = orc_iram_opts(n, m, k, which, tol, anorm, max_restarts, seed, int(real_start))
+        er, ei = np.zeros(k), np.zeros(k)
+        xr, xi = np.zeros(k * n), np.zeros(k * n)
+        rs = np.zeros(k)
+        res = orc_iram_result()
+        st = self.lib.orc_iram(C.byref(o), fn, None, _p(er), _p(ei), _p(xr), _p(xi), _p(rs),
+                               C.byref(res))
+        return (st, er + 1j * ei, (xr + 1j * xi).reshape(k, n), rs,
+                {"nconv": res.nconv, "restarts": res.restarts, "matvecs": res.matvecs})
+
+    def dense_matvec(self, are, aim, x, threads=1):
+        n = are.shape[0]
+        xr = np.ascontiguousarray(x.real)
+        xi = np.ascontiguousarray(x.imag)
+        yr, yi = np.zeros(n), np.zeros(n)
+        self.lib.orc_dense_matvec(n, _p(are), _p(aim), _p(xr), _p(xi), _p(yr), _p(yi), threads)
+        return yr + 1j * yi
+
+    # ---- generators
+    def gen_c0(self, n=64, seed=0x5EED0000):
+        a = np.zeros((n, n))
+        self.lib.orc_gen_c0(n, seed, _p(a))
+        return a
+
+    def gen_c1(self, n=8192, seed=0x5EED0001, eps=1e-2, decay=500.0):
+        ar, ai = np.zeros((n, n)), np.zeros((n, n))
+        self.lib.orc_gen_c1(n, seed, eps, decay, _p(ar), _p(ai))
+        return ar, ai
+
+    def uniform(self, seed, idx) -> float:
+        return self.lib.orc_uniform(seed, idx)
+
+    def normal(self, seed, idx) -> float:
+        return self.lib.orc_normal(seed, idx)
+
+    def splitmix64(self, x) -> int:
+        return self.lib.orc_splitmix64(x)
+
+
+class Ref:
+    """The reference's own host code (oracle/_ref/libref.so)."""
+
+    PATH = HERE / "_ref" / "libref.so"
+
+    @classmethod
+    def available(cls) -> bool:
+        return cls.PATH.exists()
+
+    def __init__(self):
+        self.lib = C.CDLL(str(self.PATH))
+
+    @staticmethod
+    def _args(p: Planes):
+        return [int(p.re is not None), int(p.im is not None), p.cols, p.rows, _p(p.re), _p(p.im)]
+
+    def set_threads_blocks(self, w, h, limit=1024):
+        b, t = (U32 * 2)(), (U32 * 2)()
+        self.lib.ref_set_threads_blocks(b, t, U32(w), U32(h), U32(limit))
+        return (b[0], b[1]), (t[0], t[1])
+
+    def dot_product(self, out, a, b, dims=None):
+        if dims is None:
+            self.lib.ref_dot_product(*self._args(out), *self._args(a), *self._args(b))
+        else:
+            d = (U32 * 6)(*[int(v) for row in dims for v in row])
+            self.lib.ref_dot_product_dims(*self._args(out), *self._args(a), *self._args(b), d)
+
+    def transpose(self, out, a, conj=False):
+        f = self.lib.ref_conjugate_transpose if conj else self.lib.ref_transpose
+        f(*self._args(out), *self._args(a))
+
+    def tensor_product(self, out, a, b, dims):
+        d = (U32 * 6)(*[int(v) for row in dims for v in row])
+        self.lib.ref_tensor_product(*self._args(out), *self._args(a), *self._args(b), d)
+
+    def sum(self, a):
+        r, i = C.c_double(), C.c_double()
+        self.lib.ref_sum(*self._args(a), C.byref(r), C.byref(i))
+        return r.value, i.value
+
+    def set_vector(self, V, column, v, length):
+        self.lib.ref_set_vector(*self._args(V), U32(column), *self._args(v), U32(length))
+
+    def get_vector(self, v, length, V, column):
+        self.lib.ref_get_vector(*self._args(v), U32(length), *self._args(V), U32(column))
+
+    def set_identity(self, m):
+        self.lib.ref_set_identity(*self._args(m))
+
+    def binary(self, op, out, a, b):
+        code = {"add": 0, "substract": 1, "hadamard": 2, "hadamard_vec": 3}[op]
+        self.lib.ref_binary(C.c_int(code), *self._args(out), *self._args(a), *self._args(b))
+
+    def multiply_constant(self, out, a, re, im=0.0, re_only=False):
+        self.lib.ref_multiply_constant(C.c_int(int(re_only)), *self._args(out), *self._args(a),
+                                       C.c_double(re), C.c_double(im))
+
+    def set_diagonal(self, m, re, im):
+        self.lib.ref_set_diagonal(*self._args(m), C.c_double(re), C.c_double(im))
+
+    def magnitude(self, a) -> float:
+        self.lib.ref_magnitude.restype = C.c_double
+        return self.lib.ref_magnitude(*self._args(a))
+
+    def qrgr(self, Q, R, A):
+        self.lib.ref_qrgr(*self._args(Q), *self._args(R), *self._args(A))
+
+    def is_upper_triangular(self, a) -> bool:
+        return bool(self.lib.ref_is_upper_triangular(*self._args(a)))

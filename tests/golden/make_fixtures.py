@@ -1,0 +1,172 @@
+"""Writes the committed golden fixtures under tests/golden/.
+
+Run in the build container (where the reference tree and numpy/scipy are
+present):  python tests/golden/make_fixtures.py [--c1]
+
+Two kinds of fixtures:
+  * data the reference's own tests hold, extracted as numbers from its test
+    headers (read as text, never compiled or executed here):
+      - oapQRTestSamples/qrtest1-9.h, oapHostTests/host_qrtest1-4.h  -> qr.json
+      - oapArnoldiDeviceTests/schur1-3.h                             -> schur.json
+      - oapCMatrixDataTests/CMatrixData1-4.h with the (wanted, hdim)
+        pairs of oapArnoldiPackageMatricesTests.cpp:224-266          -> cmatrixdata.json
+      - oapTestsData/oapDotProductTests_Data_1-4.h and
+        oapTensorProductTests_Data_1.h with the shapes their tests use -> products.json
+      - the expected eigenvalues of the data1-7 callback tests
+        (oapArnoldiPackageCallbackTest.cpp:153-187; operators absent)  -> callback_eigs.json
+  * dense LAPACK eigenvalues (numpy) of the seeded synthetic matrices the
+    oracle generates: c0_eigs.json (64 x 64 real), c1_top8.json (8192 x 8192
+    complex, optional, slow: --c1).
+"""
+from __future__ import annotations
+
+import json
+import re
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+REF = Path("/root/reference")
+sys.path.insert(0, str(ROOT))
+
+NUM = r"[-+]?(?:\d+\.?\d*|\.\d+)(?:[eE][-+]?\d+)?"
+
+
+def numbers(text: str) -> list[float]:
+    return [float(x) for x in re.findall(NUM, text)]
+
+
+def c_string(src: str, name: str) -> str:
+    """concatenated C string literal assigned to `name`"""
+    m = re.search(r"\b" + name + r"\s*=\s*((?:\"[^\"]*\"\s*)+);", src, re.S)
+    if not m:
+        raise KeyError(name)
+    return "".join(re.findall(r"\"([^\"]*)\"", m.group(1)))
+
+
+def c_array(src: str, name: str) -> list[float]:
+    m = re.search(r"\b" + name + r"\s*\[[^\]]*\]\s*=\s*\{([^}]*)\}", src, re.S)
+    if not m:
+        raise KeyError(name)
+    return numbers(m.group(1))
+
+
+def matrix_string(s: str) -> list[float]:
+    """oap's text matrix "[a, b, ...]" (square when no dimensions are given);
+    "x <repeats N times>" (gdb's run-length form, used by schur3.h) expands to N copies"""
+    body = s[s.index("[") + 1:s.rindex("]")]
+    body = re.sub(r"(" + NUM + r")\s*<repeats\s+(\d+)\s+times>",
+                  lambda m: ",".join([m.group(1)] * int(m.group(2))), body)
+    return numbers(body)
+
+
+def qr_fixtures():
+    out = []
+    for i in range(1, 10):
+        p = REF / "oapQRTestSamples" / f"qrtest{i}.h"
+        src = p.read_text()
+        vals = matrix_string(c_string(src, "matrix"))
+        m = int(round(len(vals) ** 0.5))
+        out.append({"name": f"qrtest{i}", "m": m, "matrix": vals})
+    for i in range(1, 5):
+        p = REF / "oapHostTests" / f"host_qrtest{i}.h"
+        src = p.read_text()
+        vals = matrix_string(c_string(src, "matrix"))
+        m = int(round(len(vals) ** 0.5))
+        rec = {"name": f"host_qrtest{i}", "m": m, "matrix": vals}
+        for key in ("qref", "rref"):
+            try:
+                rec[key] = matrix_string(c_string(src, key))
+            except KeyError:
+                pass
+        out.append(rec)
+    return out
+
+
+def schur_fixtures():
+    out = []
+    for i in range(1, 4):
+        src = (REF / "oapArnoldiDeviceTests" / f"schur{i}.h").read_text()
+        a = matrix_string(c_string(src, "matrix"))
+        e = matrix_string(c_string(src, "eq_matrix"))
+        m = int(round(len(a) ** 0.5))
+        out.append({"name": f"schur{i}", "m": m, "matrix": a, "eq_matrix": e})
+    return out
+
+
+def cmatrixdata_fixtures():
+    # (data file, wanted, hdim, tolerance) as oapArnoldiPackageMatricesTests.cpp:224-266 runs them
+    runs = [("CMatrixData1", 2, 6), ("CMatrixData2", 3, 6), ("CMatrixData3", 6, 12),
+            ("CMatrixData4", 6, 12)]
+    cases = []
+    for name, wanted, hdim in runs:
+        src = (REF / "oapCMatrixDataTests" / f"{name}.h").read_text()
+        cols = int(re.search(r"columns\s*=\s*(\d+)", src).group(1))
+        rows = int(re.search(r"rows\s*=\s*(\d+)", src).group(1))
+        cases.append({"name": name, "columns": cols, "rows": rows, "wanted": wanted,
+                      "hdim": hdim, "matrix": c_array(src, "matrix"),
+                      "eigenvalues": c_array(src, "eigenvalues")})
+    return {"cases": cases}
+
+
+def products_fixtures():
+    d = {}
+    # Data_4 holds inputs only (no expected output): not a golden vector
+    for i in range(1, 4):
+        src = (REF / "oapTestsData" / f"oapDotProductTests_Data_{i}.h").read_text()
+        d[f"dot_{i}"] = {k: c_array(src, k) for k in ("t_reValues1", "t_reValues2",
+                                                      "t_outputValues")}
+    src = (REF / "oapTestsData" / "oapTensorProductTests_Data_1.h").read_text()
+    d["tensor_1"] = {k: c_array(src, k) for k in ("t_reValues1", "t_reValues2",
+                                                  "t_outputValues")}
+    return d
+
+
+def callback_eigs():
+    src = (REF / "oapArnoldiDeviceTests" / "oapArnoldiPackageCallbackTest.cpp").read_text()
+    vals = re.findall(r"executeArnoldiTest\s*\(\s*(" + NUM + r")\s*,\s*\"([^\"]+)\"", src)
+    return {"expected": [{"value": float(v), "data": d} for v, d in vals],
+            "note": "operators (.tdata) are absent offline; values only"}
+
+
+def c0_eigs():
+    from oracle.pyoracle import Oracle
+    A = Oracle().gen_c0(64, 0x5EED0000)
+    ev = np.linalg.eigvals(A)
+    ev = ev[np.argsort(-np.abs(ev), kind="stable")]
+    return {"n": 64, "seed": 0x5EED0000, "re": ev.real.tolist(), "im": ev.imag.tolist(),
+            "fro": float(np.linalg.norm(A)), "sum": float(A.sum())}
+
+
+def c1_top8():
+    import scipy.sparse.linalg as sla
+    from oracle.pyoracle import Oracle
+    n = 8192
+    ar, ai = Oracle().gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    del ar, ai
+    ev, _ = sla.eigs(A, k=8, which="LM", ncv=48, tol=1e-13, maxiter=20000)
+    ev = ev[np.argsort(-np.abs(ev), kind="stable")]
+    return {"n": n, "seed": 0x5EED0001, "re": ev.real.tolist(), "im": ev.imag.tolist(),
+            "fro": float(np.linalg.norm(A)), "method": "ARPACK zneupd tol 1e-13"}
+
+
+def main():
+    HERE.mkdir(exist_ok=True)
+    if REF.exists():
+        (HERE / "qr.json").write_text(json.dumps(qr_fixtures()))
+        (HERE / "schur.json").write_text(json.dumps(schur_fixtures()))
+        (HERE / "cmatrixdata.json").write_text(json.dumps(cmatrixdata_fixtures()))
+        (HERE / "products.json").write_text(json.dumps(products_fixtures()))
+        (HERE / "callback_eigs.json").write_text(json.dumps(callback_eigs()))
+    (HERE / "c0_eigs.json").write_text(json.dumps(c0_eigs()))
+    if "--c1" in sys.argv:
+        (HERE / "c1_top8.json").write_text(json.dumps(c1_top8()))
+    print("fixtures written to", HERE)
+
+
+if __name__ == "__main__":
+    main()

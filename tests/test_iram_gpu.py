@@ -1,0 +1,197 @@
+"""The GPU implicitly restarted Arnoldi solver (CuHArnoldiCallback over
+liboaph.so) against the dense FP64 eigensolver (LAPACK through numpy on the
+same matrix), the oracle's IRAM restatement, and explicit FP64 residuals.
+
+Bar: eigenvalues within 1e-10 relative of the dense oracle and residuals
+||A x - lambda x||_2 <= 1e-10 ||A||_F for every returned pair.
+"""
+import json
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oap_amd.api import (ArnUtils, CuHArnoldiCallback, DeviceMatrix, InitVVectorType, MatrixInfo,
+                         OapError, Precision, QRType)
+
+pytestmark = pytest.mark.gpu
+GOLDEN = Path(__file__).resolve().parent / "golden"
+
+
+def top_k(ev, k, which):
+    key = {ArnUtils.SortLargestValues: lambda z: -abs(z),
+           ArnUtils.SortSmallestReValues: lambda z: z.real,
+           ArnUtils.SortLargestReValues: lambda z: -z.real,
+           ArnUtils.SortSmallestValues: lambda z: abs(z)}[which]
+    return sorted(ev, key=key)[:k]
+
+
+def match(got, want, rel):
+    """every wanted eigenvalue has a computed one within rel * max|lambda|"""
+    scale = max(abs(w) for w in want)
+    got = list(got)
+    for w in want:
+        d = min(abs(g - w) for g in got)
+        assert d <= rel * scale, (w, d)
+
+
+def solver(hdim_tol=1e-10, which=ArnUtils.SortLargestValues, seed=7):
+    s = CuHArnoldiCallback()
+    s.setQRType(QRType.QRGR)
+    s.setSortType(which)
+    s.setCheckTolerance(hdim_tol)
+    s.setOutputType(ArnUtils.DEVICE)
+    s.setSeed(seed)
+    return s
+
+
+def run_dense(A, k, m, which=ArnUtils.SortLargestValues, precision=Precision.FP64, vectors=True):
+    n = A.shape[0]
+    cplx = np.iscomplexobj(A)
+    dA = DeviceMatrix.from_numpy(A.real, A.imag if cplx else None)
+    s = solver(which=which)
+    s.setMatrix(dA)
+    s.setPrecision(precision)
+    re, im = np.zeros(k), np.zeros(k)
+    s.setOutputsEigenvalues(re, im)
+    vecs = [DeviceMatrix(1, n) for _ in range(k)] if vectors else None
+    if vecs:
+        s.setOutputsEigenvectors(vecs)
+    s.execute(m, k, MatrixInfo(True, cplx, n, n))
+    return s, re + 1j * im, vecs
+
+
+def check_residuals(s, A, ev, vecs, k):
+    fro = np.linalg.norm(A)
+    for t in range(k):
+        assert s.testOutcome(t) <= 1e-10 * fro, (t, s.testOutcome(t), fro)
+        if vecs is not None:
+            x = vecs[t].to_complex()[:, 0]
+            r = np.linalg.norm(A @ x - ev[t] * x) / np.linalg.norm(x)
+            assert r <= 1e-10 * fro, (t, r)
+
+
+def test_c0_real_64(oracle):
+    # the reference's CPU-runnable case: real 64 x 64, k=4 largest |lambda|, m=16
+    A = oracle.gen_c0(64, 0x5EED0000)
+    fix = json.loads((GOLDEN / "c0_eigs.json").read_text())
+    want = np.array(fix["re"]) + 1j * np.array(fix["im"])
+    s, ev, vecs = run_dense(A, 4, 16)
+    match(ev, want[:4], 1e-10)
+    check_residuals(s, A, ev, vecs, 4)
+    # the oracle's IRAM restatement agrees
+    st, oev, _, ores, _ = oracle.iram(lambda x: A @ x, 64, 16, 4, 0, 1e-10, np.linalg.norm(A))
+    assert st == 0
+    match(ev, oev, 1e-10)
+
+
+@pytest.mark.parametrize("n", [512, 2048])
+def test_c1_like_complex(oracle, n):
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    s, ev, vecs = run_dense(A, 8, 48)
+    want = top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues)
+    match(ev, want, 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+    st = s.stats()
+    assert st["matvecs"] > 0 and st["restarts"] >= 1
+
+
+def test_mixed_precision_reaches_fp64_residual(oracle):
+    n = 1024
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    s, ev, vecs = run_dense(A, 8, 48, precision=Precision.MIXED_FP32)
+    want = top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues)
+    match(ev, want, 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+    assert s.stats()["matvecs_low"] > 0
+
+
+def test_callback_mode_matches_dense(oracle):
+    n = 512
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    dA = DeviceMatrix.from_numpy(A.real, A.imag)
+    calls = {"wv": 0, "ev": 0}
+
+    def multiply(w, v, api, user, mt):
+        calls["wv" if mt == 1 else "ev"] += 1
+        api.dotProduct(w, user, v)  # CuHArnoldiDefault::multiply
+
+    s = solver()
+    s.setCallback(multiply, dA)
+    re, im = np.zeros(6), np.zeros(6)
+    s.setOutputsEigenvalues(re, im)
+    s.execute(32, 6, MatrixInfo(True, True, n, n))
+    want = top_k(np.linalg.eigvals(A), 6, ArnUtils.SortLargestValues)
+    match(re + 1j * im, want, 1e-10)
+    assert calls["wv"] > 0 and calls["ev"] == 6
+    for t in range(6):
+        assert s.testOutcome(t) <= 1e-10 * np.linalg.norm(A)
+
+
+@pytest.mark.parametrize("which", [ArnUtils.SortSmallestReValues, ArnUtils.SortLargestReValues])
+def test_sort_types(rng, which):
+    n = 300
+    A = np.diag(np.linspace(-3, 5, n)) + 0.01 * rng.standard_normal((n, n))
+    s, ev, vecs = run_dense(A, 3, 24, which=which)
+    want = top_k(np.linalg.eigvals(A), 3, which)
+    match(ev, want, 1e-10)
+    check_residuals(s, A, ev, vecs, 3)
+
+
+def test_cmatrixdata_fixtures():
+    # oapArnoldiDeviceTests/oapArnoldiPackageMatricesTests.cpp:224-266:
+    # diagonal matrices with known spectra, (wanted, hdim) pairs
+    fix = json.loads((GOLDEN / "cmatrixdata.json").read_text())
+    for case in fix["cases"]:
+        A = np.array(case["matrix"]).reshape(case["rows"], case["columns"])
+        k, m = case["wanted"], case["hdim"]
+        s, ev, vecs = run_dense(A, k, m)
+        want = sorted(case["eigenvalues"], key=lambda z: -abs(z))[:k]
+        np.testing.assert_allclose(sorted(np.abs(ev)), sorted(np.abs(want)), rtol=1e-10)
+
+
+def test_breakdown_first_value_is_one():
+    # e0 is an eigenvector of a diagonal matrix: the Krylov space breaks down
+    # at once; the solver continues from a fresh orthogonal vector
+    n = 64
+    d = np.arange(1, n + 1, dtype=float)
+    A = np.diag(d)
+    dA = DeviceMatrix.from_numpy(A, None)
+    s = solver()
+    s.setMatrix(dA)
+    s.setVecInitType(InitVVectorType.FIRST_VALUE_IS_ONE)
+    re, im = np.zeros(3), np.zeros(3)
+    s.setOutputsEigenvalues(re, im)
+    s.execute(12, 3, MatrixInfo(True, False, n, n))
+    np.testing.assert_allclose(sorted(re), [62, 63, 64], rtol=1e-10)
+
+
+def test_check_counter_and_first_stop(oracle):
+    A = oracle.gen_c0(64, 0x5EED0000)
+    dA = DeviceMatrix.from_numpy(A, None)
+    s = solver()
+    s.setMatrix(dA)
+    s.setCheckType(ArnUtils.CHECK_COUNTER)
+    s.setCheckCounts(2)
+    re, im = np.zeros(4), np.zeros(4)
+    s.setOutputsEigenvalues(re, im)
+    s.execute(16, 4, MatrixInfo(True, False, 64, 64))
+    assert s.stats()["restarts"] == 2
+    s.setCheckType(ArnUtils.CHECK_FIRST_STOP)
+    s.execute(16, 4, MatrixInfo(True, False, 64, 64))
+    assert s.stats()["restarts"] == 1
+
+
+def test_argument_errors():
+    s = solver()
+    with pytest.raises(OapError):  # no operator and no callback
+        s.execute(16, 4, MatrixInfo(True, False, 64, 64))
+    dA = DeviceMatrix.from_numpy(np.eye(64), None)
+    s.setMatrix(dA)
+    with pytest.raises(OapError):  # hdim must exceed wanted
+        s.execute(4, 4, MatrixInfo(True, False, 64, 64))
+    with pytest.raises(OapError):  # Krylov dimension above the one-wave limit
+        s.execute(65, 4, MatrixInfo(True, False, 64, 64))
