@@ -204,14 +204,17 @@ __device__ void rot_cols(Mat M, int j, int r0, int r1, double c, cd s) {
   }
 }
 
-// one implicit single-shift sweep over the leading kdim block of T with
-// shift mu, accumulating Q on the right
-__device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu) {
-  for (int kk = 0; kk < kdim - 1; ++kk) {
+// one implicit single-shift QR step with shift mu on the leading kdim block of
+// T, accumulating Q on the right.  The block is split at negligible
+// subdiagonals and the bulge chased inside each unreduced block (a sweep
+// started at the top of a reducible T stays the identity).  Same steps as
+// shifted_sweep in oracle/oracle.c.
+__device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu) {
+  for (int kk = lo; kk < hi; ++kk) {
     cd f, g;
-    if (kk == 0) {
-      f = csub(T.get(0, 0), mu);
-      g = T.get(1, 0);
+    if (kk == lo) {
+      f = csub(T.get(lo, lo), mu);
+      g = T.get(lo + 1, lo);
     } else {
       f = T.get(kk, kk - 1);
       g = T.get(kk + 1, kk - 1);
@@ -220,13 +223,33 @@ __device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu) {
     cd s;
     cgivens(f, g, c, s);
     __syncthreads();
-    rot_rows(T, kk, kk > 0 ? kk - 1 : 0, kdim - 1, c, s);
+    rot_rows(T, kk, kk > lo ? kk - 1 : kk, kdim - 1, c, s);
     __syncthreads();
-    if (kk > 0 && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
-    int r1 = kk + 2 < kdim - 1 ? kk + 2 : kdim - 1;
+    if (kk > lo && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
+    int r1 = kk + 2 < hi ? kk + 2 : hi;
     rot_cols(T, kk, 0, r1, c, s);
     rot_cols(Q, kk, 0, kdim - 1, c, s);
     __syncthreads();
+  }
+}
+
+__device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu) {
+  const double eps = 2.220446049250313e-16;
+  int lo = 0;
+  while (lo < kdim) {
+    int hi = lo;
+    while (hi + 1 < kdim) {
+      cd sd = T.get(hi + 1, hi), a = T.get(hi, hi), b = T.get(hi + 1, hi + 1);
+      double sub = fabs(sd.re) + fabs(sd.im);
+      double d = fabs(a.re) + fabs(a.im) + fabs(b.re) + fabs(b.im);
+      if (sub <= eps * d || sub < 1e-300) break;
+      ++hi;
+    }
+    __syncthreads();
+    if (hi + 1 < kdim && threadIdx.x == 0) T.set(hi + 1, hi, {0, 0});
+    __syncthreads();
+    if (hi > lo) chase(T, Q, kdim, lo, hi, mu);
+    lo = hi + 1;
   }
 }
 

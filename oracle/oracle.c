@@ -666,28 +666,56 @@ static void ritz_order(int m, const double* wr, const double* wi, int which, int
   }
 }
 
-/* one implicit single-shift QR sweep on the k x k leading block of the
- * m-stride Hessenberg H, accumulating Q (m x m, stride m) on the right */
-static void shifted_sweep(int m, int kdim, double* hre, double* him, double* qre, double* qim,
-                          double mr, double mi) {
+/* one implicit single-shift QR step with shift mu on the kdim x kdim leading
+ * block of the m-stride Hessenberg H, accumulating Q (stride m) on the right.
+ * The block is split at negligible subdiagonals (|h_{i+1,i}| <= eps
+ * (|h_ii| + |h_{i+1,i+1}|), set to zero) and the bulge is chased inside each
+ * unreduced block, as ARPACK's znapps does: a sweep started at the top of a
+ * reducible H would stay the identity and the shift would be lost. */
+static void chase(int m, int kdim, int lo, int hi, double* hre, double* him, double* qre,
+                  double* qim, double mr, double mi) {
 #define H_(i, j) ((size_t)(i) * m + (j))
-  for (int kk = 0; kk < kdim - 1; ++kk) {
+  for (int kk = lo; kk < hi; ++kk) {
     double fr, fi, gr, gi;
-    if (kk == 0) {
-      fr = hre[H_(0, 0)] - mr; fi = him[H_(0, 0)] - mi;
-      gr = hre[H_(1, 0)]; gi = him[H_(1, 0)];
+    if (kk == lo) {
+      fr = hre[H_(lo, lo)] - mr; fi = him[H_(lo, lo)] - mi;
+      gr = hre[H_(lo + 1, lo)]; gi = him[H_(lo + 1, lo)];
     } else {
       fr = hre[H_(kk, kk - 1)]; fi = him[H_(kk, kk - 1)];
       gr = hre[H_(kk + 1, kk - 1)]; gi = him[H_(kk + 1, kk - 1)];
     }
     double c, sr, si;
     cgivens(fr, fi, gr, gi, &c, &sr, &si);
-    int j0 = kk > 0 ? kk - 1 : 0;
+    int j0 = kk > lo ? kk - 1 : kk;
     rot_rows(m, hre, him, kk, j0, kdim - 1, c, sr, si);
-    if (kk > 0) { hre[H_(kk + 1, kk - 1)] = 0; him[H_(kk + 1, kk - 1)] = 0; }
-    int r1 = kk + 2 < kdim - 1 ? kk + 2 : kdim - 1;
+    if (kk > lo) { hre[H_(kk + 1, kk - 1)] = 0; him[H_(kk + 1, kk - 1)] = 0; }
+    int r1 = kk + 2 < hi ? kk + 2 : hi;
     rot_cols(m, hre, him, kk, 0, r1, c, sr, si);
     rot_cols(m, qre, qim, kk, 0, kdim - 1, c, sr, si);
+  }
+#undef H_
+}
+
+static void shifted_sweep(int m, int kdim, double* hre, double* him, double* qre, double* qim,
+                          double mr, double mi) {
+#define H_(i, j) ((size_t)(i) * m + (j))
+  const double eps = 2.220446049250313e-16;
+  int lo = 0;
+  while (lo < kdim) {
+    int hi = lo;
+    while (hi + 1 < kdim) {
+      double sub = fabs(hre[H_(hi + 1, hi)]) + fabs(him[H_(hi + 1, hi)]);
+      double d = fabs(hre[H_(hi, hi)]) + fabs(him[H_(hi, hi)]) + fabs(hre[H_(hi + 1, hi + 1)]) +
+                 fabs(him[H_(hi + 1, hi + 1)]);
+      if (sub <= eps * d || sub < 1e-300) {
+        hre[H_(hi + 1, hi)] = 0;
+        him[H_(hi + 1, hi)] = 0;
+        break;
+      }
+      ++hi;
+    }
+    if (hi > lo) chase(m, kdim, lo, hi, hre, him, qre, qim, mr, mi);
+    lo = hi + 1;
   }
 #undef H_
 }
@@ -739,6 +767,11 @@ int orc_iram(const orc_iram_opts* o, orc_matvec_fn mv, void* user, double* ev_re
   /* start vector: initVvector_rand (ArnoldiProcedures.cpp:289-319), uniform
    * in [-1, 1], normalised; seeded (reference: std::random_device). */
   for (int i = 0; i < n; ++i) {
+    if (o->v0re) {
+      vre[i] = o->v0re[i];
+      vim[i] = o->v0im ? o->v0im[i] : 0.0;
+      continue;
+    }
     vre[i] = 2.0 * orc_uniform(o->seed, (uint64_t)i) - 1.0;
     vim[i] = o->real_start ? 0.0 : 2.0 * orc_uniform(o->seed ^ 0xA5A5ULL, (uint64_t)i) - 1.0;
   }

@@ -104,6 +104,8 @@ typedef struct orc_iram_opts {
   int max_restarts;
   uint64_t seed;       /* start vector seed (SplitMix64 stream) */
   int real_start;      /* nonzero: start vector has zero imaginary part */
+  const double* v0re;  /* optional start vector (NULL: seeded random) */
+  const double* v0im;
 } orc_iram_opts;
 
 typedef struct orc_iram_result {

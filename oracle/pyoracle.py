@@ -27,7 +27,7 @@ class orc_matrix(C.Structure):
 class orc_iram_opts(C.Structure):
     _fields_ = [("n", C.c_int), ("m", C.c_int), ("k", C.c_int), ("which", C.c_int),
                 ("tol", C.c_double), ("anorm", C.c_double), ("max_restarts", C.c_int),
-                ("seed", C.c_uint64), ("real_start", C.c_int)]
+                ("seed", C.c_uint64), ("real_start", C.c_int), ("v0re", DP), ("v0im", DP)]
 
 
 class orc_iram_result(C.Structure):
@@ -192,7 +192,7 @@ class Oracle:
         return st, hr + 1j * hi, zr + 1j * zi, wr + 1j * wi
 
     def iram(self, matvec, n, m, k, which=0, tol=1e-10, anorm=1.0, max_restarts=1000, seed=1,
-             real_start=True):
+             real_start=True, v0=None):
         """matvec(x complex (n,)) -> complex (n,).  Returns (evals, evecs (k, n), resid, stats)."""
 
         def cb(_u, xr, xi, yr, yi):
@@ -203,6 +203,10 @@ class Oracle:
 
         fn = MATVEC_FN(cb)
         o = orc_iram_opts(n, m, k, which, tol, anorm, max_restarts, seed, int(real_start))
+        if v0 is not None:
+            v0r = np.ascontiguousarray(np.real(v0), dtype=np.float64)
+            v0i = np.ascontiguousarray(np.imag(v0), dtype=np.float64)
+            o.v0re, o.v0im = _p(v0r), _p(v0i)
         er, ei = np.zeros(k), np.zeros(k)
         xr, xi = np.zeros(k * n), np.zeros(k * n)
         rs = np.zeros(k)

@@ -55,3 +55,9 @@ def test_threads_blocks_golden_table():
     assert t[0] * t[1] == 1024 and b[0] * t[0] >= 5123 and b[1] * t[1] >= 2133
     b, t = set_threads_blocks(400, 20)
     assert b[0] * t[0] >= 400 and b[1] * t[1] >= 20
+
+
+def test_cpp_facade_example_is_built():
+    # the facade header compiles as C++17 against the C ABI only
+    exe = ROOT / "tests" / "cpp" / "facade_example"
+    assert exe.exists()

@@ -195,3 +195,21 @@ def test_argument_errors():
         s.execute(4, 4, MatrixInfo(True, False, 64, 64))
     with pytest.raises(OapError):  # Krylov dimension above the one-wave limit
         s.execute(65, 4, MatrixInfo(True, False, 64, 64))
+
+
+def test_cpp_facade_example(oracle):
+    # a reference-style C++ caller (tests/cpp/facade_example.cpp) through the
+    # header-only facade include/oap/oap.h and the C ABI
+    import subprocess
+    exe = Path(__file__).resolve().parent / "cpp" / "facade_example"
+    assert exe.exists(), "build the library first (python -m oap_amd.build)"
+    A = oracle.gen_c0(64, 0x5EED0000)
+    text = "64\n" + " ".join(repr(float(v)) for v in A.ravel())
+    out = subprocess.run([str(exe)], input=text, capture_output=True, text=True, timeout=120)
+    assert out.returncode == 0, out.stderr
+    res = json.loads(out.stdout)
+    ev = np.array(res["re"]) + 1j * np.array(res["im"])
+    fix = json.loads((GOLDEN / "c0_eigs.json").read_text())
+    want = (np.array(fix["re"]) + 1j * np.array(fix["im"]))[:4]
+    match(ev, want, 1e-10)
+    assert max(res["resid"]) <= 1e-10 * fix["fro"]

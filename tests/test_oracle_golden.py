@@ -163,3 +163,18 @@ def test_generators_pinned(oracle):
     assert 0 <= u.min() and u.max() < 1 and abs(u.mean() - 0.5) < 0.01
     g = np.array([oracle.normal(7, i) for i in range(20000)])
     assert abs(g.mean()) < 0.03 and abs(g.std() - 1) < 0.03
+
+
+def test_iram_breakdown_first_value_is_one(oracle):
+    # initVvector_fvis1 (ArnoldiProcedures.cpp:279-287) on a diagonal matrix:
+    # e0 is an eigenvector, the factorisation breaks down at once; the
+    # restart must chase its shifts inside the unreduced trailing block
+    n = 64
+    d = np.arange(1, n + 1, dtype=float)
+    A = np.diag(d)
+    v0 = np.zeros(n)
+    v0[0] = 1.0
+    st, ev, X, res, info = oracle.iram(lambda x: A @ x, n, 12, 3, 0, 1e-10,
+                                       np.linalg.norm(A), v0=v0)
+    assert st == 0 and info["nconv"] == 3
+    np.testing.assert_allclose(sorted(ev.real), [62, 63, 64], rtol=1e-10)
