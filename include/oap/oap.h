@@ -279,6 +279,9 @@ class CuHArnoldi {
   void setNormA(floatt n) { oap::check(oaph_arnoldi_set_norm_a(m_a, n), "setNormA"); }
   void setMaxRestarts(uintt n) { oap::check(oaph_arnoldi_set_max_restarts(m_a, n), "setMaxRestarts"); }
   void setPrecision(int p) { oap::check(oaph_arnoldi_set_precision(m_a, p), "setPrecision"); }
+  void setMixedTolerance(double t) {
+    oap::check(oaph_arnoldi_set_mixed_tolerance(m_a, t), "setMixedTolerance");
+  }
   oaph_arnoldi_stats stats() const {
     oaph_arnoldi_stats s;
     oap::check(oaph_arnoldi_get_stats(m_a, &s), "stats");

@@ -262,6 +262,9 @@ oaph_status oaph_arnoldi_set_seed(oaph_arnoldi* a, uint64_t seed);
 oaph_status oaph_arnoldi_set_norm_a(oaph_arnoldi* a, double norm_a);
 oaph_status oaph_arnoldi_set_max_restarts(oaph_arnoldi* a, uint32_t max_restarts);
 oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* a, int precision);
+/* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
+ * the largest Ritz value (default 1e-6) */
+oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* a, double tol_low);
 /* start vector for the next execute (n complex values, NULL im = real) */
 oaph_status oaph_arnoldi_set_start_vector(oaph_arnoldi* a, const double* re, const double* im,
                                           uint32_t n);
@@ -279,6 +282,10 @@ oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
  * operator application for the dense operator set with
  * oaph_arnoldi_set_matrix: FP64 (low = 0) or the FP32 copy (low = 1). */
 oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* a, int low, uint32_t iters, double* avg_ms);
+/* Shader-clock stamps of the phases of the last restart kernel: load+Schur,
+ * eigenvectors+estimates, ordering, wanted Ritz vectors, exact shifts
+ * (6 values; consecutive differences are the phase durations in cycles). */
+oaph_status oaph_arnoldi_restart_profile(oaph_arnoldi* a, long long* clk6);
 
 /* ------------------------------------------------------------------ */
 /* measurement support (no reference counterpart)                      */
@@ -292,6 +299,58 @@ oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1
  * the api's stream), after one warm-up call. */
 oaph_status oaph_time_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
                                   const oaph_matrix* b, uint32_t iters, double* avg_ms);
+
+/* ------------------------------------------------------------------ */
+/* Batched covariance eigenproblems.  Problem i is C_i = X_i^T X_i / s with
+ * X_i an s x n_i FP32 row-major sample matrix: the M M^T operator the
+ * reference forms with RecToSquareApi / SquareMatrixApi
+ * (oapMatrixCuda/RecToSquareApi.cpp:57-141, oapMatrixCuda/SquareMatrixApi.cpp)
+ * applied implicitly, solved for the `wanted` extreme eigenpairs by the same
+ * implicitly restarted Arnoldi iteration as oaph_arnoldi_execute
+ * (ArnoldiPackage/ArnoldiProcedures.cpp:129-140), all problems advancing in
+ * one persistent launch (one workgroup per problem).  Limits: n_i in
+ * [2, 1024], s <= 1024, hdim <= 24, n_i >= hdim.                           */
+/* ------------------------------------------------------------------ */
+typedef struct oaph_batch oaph_batch;
+
+/* Sizes n_i = 64 + floor(961 u_i) of the seeded synthetic batch (u_i the
+ * first uniform of problem i's SplitMix64 stream). */
+oaph_status oaph_batch_c3_sizes(uint32_t count, uint64_t seed, uint32_t* n_out);
+oaph_status oaph_batch_create(oaph_batch** out, uint32_t count, const uint32_t* n, uint32_t s);
+oaph_status oaph_batch_destroy(oaph_batch* b);
+/* X_i as s x n_i row-major floats (host memory, synchronous copy). */
+oaph_status oaph_batch_upload_x(oaph_batch* b, uint32_t i, const float* X);
+oaph_status oaph_batch_download_x(oaph_batch* b, uint32_t i, float* X);
+/* Synthetic 16-bit grey patches (k/65535, k uniform) plus three planted
+ * rank-1 components of amplitude 5, generated in HBM. */
+oaph_status oaph_batch_synth_c3(oaph_batch* b, uint64_t seed);
+/* Launch variant (threads per workgroup x rows per wave step); see
+ * oaph_batch_variant_info.  Default 1. */
+oaph_status oaph_batch_set_variant(oaph_batch* b, int variant);
+oaph_status oaph_batch_variant_info(int variant, int* threads, int* rows);
+/* Capture the solve launch in a HIP graph and replay it while its
+ * arguments are unchanged (default on). */
+oaph_status oaph_batch_set_graph(oaph_batch* b, int enable);
+/* Solves every problem: residual test beta |e_m^T y| <= tol * max |Ritz|,
+ * sort_type as OAPH_SORT_*.  Synchronous. */
+oaph_status oaph_batch_solve(oaph_batch* b, uint32_t hdim, uint32_t wanted, int sort_type,
+                             double tol, uint32_t max_restarts, uint64_t seed);
+/* Times every variant (`reps` solves each after a warm-up), selects the
+ * fastest and appends it to the text cache at cache_path (NULL: no cache).
+ * With times_ms == NULL a cached choice for the same batch shape and device
+ * is taken without timing.  times_ms, if given, receives one time per
+ * variant. */
+oaph_status oaph_batch_autotune(oaph_batch* b, uint32_t hdim, uint32_t wanted, int sort_type,
+                                double tol, uint32_t max_restarts, uint64_t seed, uint32_t reps,
+                                const char* cache_path, int* chosen, double* times_ms);
+/* Per problem (count x wanted, wanted order): Rayleigh quotients of the
+ * normalised Ritz vectors, explicit FP64 residuals ||C x - lambda x||_2,
+ * converged count, restarts, operator applications; and the kernel time of
+ * the last solve.  Any pointer may be NULL. */
+oaph_status oaph_batch_results(oaph_batch* b, double* evals, double* resid, uint32_t* nconv,
+                               uint32_t* restarts, uint32_t* matvecs, double* kernel_ms);
+/* Ritz vectors of problem i: wanted x n_i doubles, vector t at x + t * n_i. */
+oaph_status oaph_batch_eigenvectors(oaph_batch* b, uint32_t i, double* x);
 
 #ifdef __cplusplus
 }

@@ -124,11 +124,28 @@ _SIGS = {
     "oaph_arnoldi_set_norm_a": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_max_restarts": (C.c_int, [C.c_void_p, U32]),
     "oaph_arnoldi_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "oaph_arnoldi_set_mixed_tolerance": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_start_vector": (C.c_int, [C.c_void_p, DP, DP, U32]),
     "oaph_arnoldi_get_stats": (C.c_int, [C.c_void_p, C.POINTER(oaph_arnoldi_stats)]),
     "oaph_arnoldi_time_matvec": (C.c_int, [C.c_void_p, C.c_int, U32, DP]),
+    "oaph_arnoldi_restart_profile": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "oaph_synth_matrix": (C.c_int, [MP, C.c_int, C.c_uint64, C.c_double, C.c_double]),
     "oaph_time_dot_product": (C.c_int, [C.c_void_p, MP, MP, MP, U32, DP]),
+    "oaph_batch_c3_sizes": (C.c_int, [U32, C.c_uint64, C.POINTER(U32)]),
+    "oaph_batch_create": (C.c_int, [C.POINTER(C.c_void_p), U32, C.POINTER(U32), U32]),
+    "oaph_batch_destroy": (C.c_int, [C.c_void_p]),
+    "oaph_batch_upload_x": (C.c_int, [C.c_void_p, U32, C.POINTER(C.c_float)]),
+    "oaph_batch_download_x": (C.c_int, [C.c_void_p, U32, C.POINTER(C.c_float)]),
+    "oaph_batch_synth_c3": (C.c_int, [C.c_void_p, C.c_uint64]),
+    "oaph_batch_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
+    "oaph_batch_variant_info": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "oaph_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "oaph_batch_solve": (C.c_int, [C.c_void_p, U32, U32, C.c_int, C.c_double, U32, C.c_uint64]),
+    "oaph_batch_autotune": (C.c_int, [C.c_void_p, U32, U32, C.c_int, C.c_double, U32, C.c_uint64,
+                                      U32, C.c_char_p, C.POINTER(C.c_int), DP]),
+    "oaph_batch_results": (C.c_int, [C.c_void_p, DP, DP, C.POINTER(U32), C.POINTER(U32),
+                                     C.POINTER(U32), DP]),
+    "oaph_batch_eigenvectors": (C.c_int, [C.c_void_p, U32, DP]),
 }
 
 _lib = None

@@ -385,6 +385,9 @@ class CuHArnoldiCallback:
     def setPrecision(self, p):
         check(self._lib.oaph_arnoldi_set_precision(self._h, p), "setPrecision")
 
+    def setMixedTolerance(self, tol_low):
+        check(self._lib.oaph_arnoldi_set_mixed_tolerance(self._h, tol_low), "set_mixed_tolerance")
+
     def setStartVector(self, v):
         if v is None:
             check(self._lib.oaph_arnoldi_set_start_vector(self._h, None, None, 0), "setStartVector")
@@ -493,3 +496,120 @@ class DeviceMatrixView:
 
 __all__ += ["CuHArnoldiCallback", "MatrixInfo", "QRType", "VecMultiplicationType",
             "InitVVectorType", "ArnUtils", "Precision", "DeviceMatrixView"]
+
+
+class CovarianceBatch:
+    """Many real symmetric eigenproblems C_i = X_i^T X_i / s solved together
+    on the GPU (one persistent launch, one workgroup per problem).
+
+    The operator is the reference's M M^T product (RecToSquareApi,
+    oapMatrixCuda/RecToSquareApi.cpp:57-141) applied implicitly; the solver
+    is the implicitly restarted Arnoldi iteration of CuHArnoldi
+    (ArnoldiPackage/ArnoldiProcedures.cpp:129-140) with a symmetric
+    tridiagonal projection.
+    """
+
+    def __init__(self, sizes, samples: int):
+        self._lib = load()
+        self.n = np.ascontiguousarray(sizes, dtype=np.uint32)
+        self.s = int(samples)
+        h = C.c_void_p()
+        check(self._lib.oaph_batch_create(C.byref(h), len(self.n),
+                                          self.n.ctypes.data_as(C.POINTER(C.c_uint32)), self.s),
+              "batch_create")
+        self._h = h
+        self.k = 0
+
+    @staticmethod
+    def c3_sizes(count: int, seed: int) -> np.ndarray:
+        out = np.zeros(count, dtype=np.uint32)
+        check(load().oaph_batch_c3_sizes(count, seed, out.ctypes.data_as(C.POINTER(C.c_uint32))),
+              "batch_c3_sizes")
+        return out
+
+    def upload(self, i: int, X: np.ndarray) -> None:
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.shape != (self.s, int(self.n[i])):
+            raise ValueError(f"X_{i} must be {self.s} x {int(self.n[i])}, got {X.shape}")
+        check(self._lib.oaph_batch_upload_x(self._h, i, X.ctypes.data_as(C.POINTER(C.c_float))),
+              "batch_upload_x")
+
+    def download(self, i: int) -> np.ndarray:
+        X = np.zeros((self.s, int(self.n[i])), dtype=np.float32)
+        check(self._lib.oaph_batch_download_x(self._h, i, X.ctypes.data_as(C.POINTER(C.c_float))),
+              "batch_download_x")
+        return X
+
+    def synth_c3(self, seed: int) -> None:
+        check(self._lib.oaph_batch_synth_c3(self._h, seed), "batch_synth_c3")
+
+    def set_variant(self, v: int) -> None:
+        check(self._lib.oaph_batch_set_variant(self._h, v), "batch_set_variant")
+
+    @staticmethod
+    def variants() -> list[tuple[int, int]]:
+        out, v = [], 0
+        lib = load()
+        while True:
+            t, r = C.c_int(), C.c_int()
+            if lib.oaph_batch_variant_info(v, C.byref(t), C.byref(r)) != 0:
+                return out
+            out.append((t.value, r.value))
+            v += 1
+
+    def set_graph(self, enable: bool) -> None:
+        check(self._lib.oaph_batch_set_graph(self._h, int(enable)), "batch_set_graph")
+
+    def solve(self, hdim: int, wanted: int, which: int = 0, tol: float = 1e-10,
+              max_restarts: int = 1000, seed: int = 0x5EED5EED) -> None:
+        check(self._lib.oaph_batch_solve(self._h, hdim, wanted, which, tol, max_restarts, seed),
+              "batch_solve")
+        self.k = wanted
+
+    def autotune(self, hdim: int, wanted: int, which: int = 0, tol: float = 1e-10,
+                 max_restarts: int = 1000, seed: int = 0x5EED5EED, reps: int = 2,
+                 cache: str | None = None, measure: bool = True):
+        """Returns (chosen variant, per-variant ms or None when taken from the cache)."""
+        nv = len(self.variants())
+        times = np.zeros(nv)
+        ch = C.c_int()
+        check(self._lib.oaph_batch_autotune(
+            self._h, hdim, wanted, which, tol, max_restarts, seed, reps,
+            cache.encode() if cache else None, C.byref(ch),
+            times.ctypes.data_as(C.POINTER(C.c_double)) if measure else None), "batch_autotune")
+        return ch.value, (times if measure else None)
+
+    def results(self) -> dict:
+        cnt, k = len(self.n), self.k
+        ev, rs = np.zeros((cnt, k)), np.zeros((cnt, k))
+        nconv = np.zeros(cnt, dtype=np.uint32)
+        rst = np.zeros(cnt, dtype=np.uint32)
+        mv = np.zeros(cnt, dtype=np.uint32)
+        ms = C.c_double()
+        u32 = C.POINTER(C.c_uint32)
+        check(self._lib.oaph_batch_results(
+            self._h, ev.ctypes.data_as(C.POINTER(C.c_double)),
+            rs.ctypes.data_as(C.POINTER(C.c_double)), nconv.ctypes.data_as(u32),
+            rst.ctypes.data_as(u32), mv.ctypes.data_as(u32), C.byref(ms)), "batch_results")
+        return {"evals": ev, "resid": rs, "nconv": nconv, "restarts": rst, "matvecs": mv,
+                "kernel_ms": ms.value}
+
+    def eigenvectors(self, i: int) -> np.ndarray:
+        x = np.zeros((self.k, int(self.n[i])))
+        check(self._lib.oaph_batch_eigenvectors(self._h, i, x.ctypes.data_as(C.POINTER(C.c_double))),
+              "batch_eigenvectors")
+        return x
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.oaph_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+__all__ += ["CovarianceBatch"]

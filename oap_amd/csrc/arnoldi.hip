@@ -45,6 +45,7 @@ struct oaph_arnoldi {
   bool norm_a_user = false;
   uint32_t max_restarts = 1000;
   int precision = OAPH_PREC_FP64;
+  double tol_low = 1e-6;  // FP32 phase tolerance, relative to max |Ritz value|
   oaph_multiply_fn mfn = nullptr;
   void* muser = nullptr;
   oaph_check_fn cfn = nullptr;
@@ -452,6 +453,12 @@ oaph_status oaph_arnoldi_set_norm_a(oaph_arnoldi* s, double v) {
   s->norm_a_user = v > 0;
   return OAPH_OK;
 }
+oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* s, double tol_low) {
+  OAPH_CHECK(s && tol_low > 0 && tol_low < 1, OAPH_ERR_ARG);
+  s->tol_low = tol_low;
+  return OAPH_OK;
+}
+
 oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* s, int v) {
   OAPH_CHECK(s && (v == OAPH_PREC_FP64 || v == OAPH_PREC_MIXED_FP32), OAPH_ERR_ARG);
   s->precision = v;
@@ -555,9 +562,8 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
   uint32_t cycles = 0;
   bool converged = false;
   if (mixed) {
-    // FP32 phase down to the FP32 floor (relative to max |Ritz value|)
-    const double tol_low = 4e-7;
-    OAPH_TRY(run_cycles(s, true, tol_low, 0.0, &cycles, &converged));
+    // FP32 phase (tolerance relative to max |Ritz value|)
+    OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
     OAPH_TRY(ritz_vectors(s, s->m));
     double* Xre = s->X.as<double>();
     OAPH_TRY(colsum(st, s->n, s->k, Xre, Xre + (size_t)s->k * s->n, Vre(s, s->V, 0),
@@ -630,6 +636,14 @@ oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* s, int low, uint32_t iters, d
   (void)hipEventDestroy(e1);
   s->stats = keep;
   *avg_ms = ms / iters;
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_restart_profile(oaph_arnoldi* s, long long* clk6) {
+  OAPH_CHECK(s && clk6, OAPH_ERR_ARG);
+  OAPH_CHECK(s->status.ptr, OAPH_ERR_STATE);
+  OAPH_TRY(read_status(s));
+  for (int i = 0; i < 6; ++i) clk6[i] = s->host_status.clk[i];
   return OAPH_OK;
 }
 

@@ -20,6 +20,7 @@ struct Status {
   double tolabs;   // absolute residual tolerance used for nconv
   double ev_re[kMaxM], ev_im[kMaxM], est[kMaxM];  // Ritz values in wanted order, estimates
   int order[kMaxM];
+  long long clk[8];  // shader-clock stamps of the restart phases (profiling)
 };
 
 // Device buffers the orthogonalisation kernels use.

@@ -16,6 +16,7 @@
 //   k_vq             V Q (tall-skinny x small), optionally f <- (V Q)_kk beta + f sigma
 //   k_pairs<MODE>    Rayleigh quotients and residual norms of Ritz pairs
 #include "iram.h"
+#include "small_dense.h"
 
 namespace oaph {
 namespace iram {
@@ -24,6 +25,7 @@ namespace {
 
 constexpr int kOrthThreads = 256;
 constexpr int kRows = 64;  // rows per block in k_orth
+constexpr int kCPW = kMaxM / 4 + 1;  // basis columns per wave (ncols <= m + 1)
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -59,18 +61,31 @@ __global__ __launch_bounds__(kOrthThreads) void k_orth(
     const double2* add, double2* result, double* partials, unsigned* counter, double* Hre,
     double* Him, uint32_t m, int hcol, int hrow, Status* st) {
   __shared__ double2 lds_part[4][kRows];
+  __shared__ double2 lds_fin[4][kMaxM + 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * kRows + lane;
   const bool in = i < n;
   double xr = in ? xre[i] : 0.0, xi = in ? xim[i] : 0.0;
+  // this wave's basis columns c = wave + 4q, all loads issued before use
+  double vr[kCPW], vi[kCPW];
+#pragma unroll
+  for (int q = 0; q < kCPW; ++q) {
+    const uint32_t c = wave + 4 * q;
+    const bool ok = in && c < ncols;
+    vr[q] = ok ? Vre[(size_t)c * n + i] : 0.0;
+    vi[q] = ok ? Vim[(size_t)c * n + i] : 0.0;
+  }
   if (MODE >= 1) {
     // this wave's share of (V sub)[i]
     double pr = 0, pi = 0;
-    for (uint32_t c = wave; c < ncols; c += 4) {
-      double2 s = sub[c];
-      double vr = in ? Vre[(size_t)c * n + i] : 0.0, vi = in ? Vim[(size_t)c * n + i] : 0.0;
-      pr += vr * s.x - vi * s.y;
-      pi += vr * s.y + vi * s.x;
+#pragma unroll
+    for (int q = 0; q < kCPW; ++q) {
+      const uint32_t c = wave + 4 * q;
+      if (c < ncols) {
+        const double2 s = sub[c];
+        pr += vr[q] * s.x - vi[q] * s.y;
+        pi += vr[q] * s.y + vi[q] * s.x;
+      }
     }
     lds_part[wave][lane] = make_double2(pr, pi);
     __syncthreads();
@@ -85,14 +100,17 @@ __global__ __launch_bounds__(kOrthThreads) void k_orth(
   }
   const size_t pstride = (size_t)(ncols > 0 ? ncols : 1);
   if (MODE <= 1) {
-    for (uint32_t c = wave; c < ncols; c += 4) {
-      double vr = in ? Vre[(size_t)c * n + i] : 0.0, vi = in ? Vim[(size_t)c * n + i] : 0.0;
-      // conj(v) * f
-      double sr = wave_sum(vr * xr + vi * xi);
-      double si = wave_sum(vr * xi - vi * xr);
-      if (lane == 0) {
-        partials[2 * ((size_t)blockIdx.x * pstride + c)] = sr;
-        partials[2 * ((size_t)blockIdx.x * pstride + c) + 1] = si;
+#pragma unroll
+    for (int q = 0; q < kCPW; ++q) {
+      const uint32_t c = wave + 4 * q;
+      if (c < ncols) {
+        // conj(v) * f
+        double sr = wave_sum(vr[q] * xr + vi[q] * xi);
+        double si = wave_sum(vr[q] * xi - vi[q] * xr);
+        if (lane == 0) {
+          partials[2 * ((size_t)blockIdx.x * pstride + c)] = sr;
+          partials[2 * ((size_t)blockIdx.x * pstride + c) + 1] = si;
+        }
       }
     }
   } else {
@@ -103,13 +121,20 @@ __global__ __launch_bounds__(kOrthThreads) void k_orth(
   }
   if (!last_block(counter)) return;
   if (MODE <= 1) {
-    for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) {
+    // lane = column, wave w sums blocks w, w+4, ...; waves combined in order
+    for (uint32_t c = lane; c < ncols; c += 64) {
       double sr = 0, si = 0;
-      for (uint32_t b = 0; b < gridDim.x; ++b) {
+#pragma unroll 8
+      for (uint32_t b = wave; b < gridDim.x; b += 4) {
         sr += __builtin_nontemporal_load(&partials[2 * ((size_t)b * pstride + c)]);
         si += __builtin_nontemporal_load(&partials[2 * ((size_t)b * pstride + c) + 1]);
       }
-      result[c] = make_double2(sr, si);
+      lds_fin[wave][c] = make_double2(sr, si);
+    }
+    __syncthreads();
+    for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) {
+      double2 a = lds_fin[0][c], b = lds_fin[1][c], c2 = lds_fin[2][c], d = lds_fin[3][c];
+      result[c] = make_double2((a.x + b.x) + (c2.x + d.x), (a.y + b.y) + (c2.y + d.y));
     }
   } else {
     if (threadIdx.x < 64) {
@@ -154,328 +179,11 @@ __global__ void k_normalize(uint32_t n, const double* fre, const double* fim, do
   }
 }
 
-// ------------------------------------------------------------------ small dense (one wave)
-struct cd {
-  double re, im;
-};
-__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
-__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
-__device__ __forceinline__ cd csub(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
-__device__ __forceinline__ double cabs2(cd a) { return a.re * a.re + a.im * a.im; }
-
-// G = [[c, s], [-conj s, c]] with real c, G [f; g] = [r; 0] (LAPACK zlartg convention)
-__device__ void cgivens(cd f, cd g, double& c, cd& s) {
-  double af = hypot(f.re, f.im), ag = hypot(g.re, g.im);
-  if (ag == 0) { c = 1; s = {0, 0}; return; }
-  if (af == 0) { c = 0; s = {g.re / ag, -g.im / ag}; return; }
-  double rho = hypot(af, ag);
-  c = af / rho;
-  double ur = f.re / af, ui = f.im / af;
-  s = {(ur * g.re + ui * g.im) / rho, (ui * g.re - ur * g.im) / rho};
-}
-
-// LDS-resident m x m complex matrix, row-major
-struct Mat {
-  double* re;
-  double* im;
-  int ld;
-  __device__ cd get(int r, int c) const { return {re[r * ld + c], im[r * ld + c]}; }
-  __device__ void set(int r, int c, cd v) { re[r * ld + c] = v.re; im[r * ld + c] = v.im; }
-};
-
-// rows i, i+1 <- G rows, columns j0..j1 (lanes split the columns)
-__device__ void rot_rows(Mat M, int i, int j0, int j1, double c, cd s) {
-  for (int j = j0 + (int)threadIdx.x; j <= j1; j += blockDim.x) {
-    cd a = M.get(i, j), b = M.get(i + 1, j);
-    cd na = {c * a.re + (s.re * b.re - s.im * b.im), c * a.im + (s.re * b.im + s.im * b.re)};
-    cd nb = {-(s.re * a.re + s.im * a.im) + c * b.re, -(s.re * a.im - s.im * a.re) + c * b.im};
-    M.set(i, j, na);
-    M.set(i + 1, j, nb);
-  }
-}
-// columns j, j+1 <- columns G^H, rows r0..r1
-__device__ void rot_cols(Mat M, int j, int r0, int r1, double c, cd s) {
-  for (int r = r0 + (int)threadIdx.x; r <= r1; r += blockDim.x) {
-    cd a = M.get(r, j), b = M.get(r, j + 1);
-    cd na = {a.re * c + (b.re * s.re + b.im * s.im), a.im * c + (b.im * s.re - b.re * s.im)};
-    cd nb = {-(a.re * s.re - a.im * s.im) + b.re * c, -(a.re * s.im + a.im * s.re) + b.im * c};
-    M.set(r, j, na);
-    M.set(r, j + 1, nb);
-  }
-}
-
-// one implicit single-shift QR step with shift mu on the leading kdim block of
-// T, accumulating Q on the right.  The block is split at negligible
-// subdiagonals and the bulge chased inside each unreduced block (a sweep
-// started at the top of a reducible T stays the identity).  Same steps as
-// shifted_sweep in oracle/oracle.c.
-__device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu) {
-  for (int kk = lo; kk < hi; ++kk) {
-    cd f, g;
-    if (kk == lo) {
-      f = csub(T.get(lo, lo), mu);
-      g = T.get(lo + 1, lo);
-    } else {
-      f = T.get(kk, kk - 1);
-      g = T.get(kk + 1, kk - 1);
-    }
-    double c;
-    cd s;
-    cgivens(f, g, c, s);
-    __syncthreads();
-    rot_rows(T, kk, kk > lo ? kk - 1 : kk, kdim - 1, c, s);
-    __syncthreads();
-    if (kk > lo && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
-    int r1 = kk + 2 < hi ? kk + 2 : hi;
-    rot_cols(T, kk, 0, r1, c, s);
-    rot_cols(Q, kk, 0, kdim - 1, c, s);
-    __syncthreads();
-  }
-}
-
-__device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu) {
-  const double eps = 2.220446049250313e-16;
-  int lo = 0;
-  while (lo < kdim) {
-    int hi = lo;
-    while (hi + 1 < kdim) {
-      cd sd = T.get(hi + 1, hi), a = T.get(hi, hi), b = T.get(hi + 1, hi + 1);
-      double sub = fabs(sd.re) + fabs(sd.im);
-      double d = fabs(a.re) + fabs(a.im) + fabs(b.re) + fabs(b.im);
-      if (sub <= eps * d || sub < 1e-300) break;
-      ++hi;
-    }
-    __syncthreads();
-    if (hi + 1 < kdim && threadIdx.x == 0) T.set(hi + 1, hi, {0, 0});
-    __syncthreads();
-    if (hi > lo) chase(T, Q, kdim, lo, hi, mu);
-    lo = hi + 1;
-  }
-}
-
-// Complex Schur form of the leading mm block of T (Hessenberg) with Schur
-// vectors Z; the same iteration as orc_hess_schur (oracle/oracle.c).
-__device__ int schur(Mat T, Mat Z, int mm) {
-  const double eps = 2.220446049250313e-16;
-  int ihi = mm - 1, iter = 0, total = 0;
-  while (ihi >= 0) {
-    int l;
-    for (l = ihi; l > 0; --l) {
-      cd a = T.get(l - 1, l - 1), b = T.get(l, l), sd = T.get(l, l - 1);
-      double s = fabs(a.re) + fabs(a.im) + fabs(b.re) + fabs(b.im);
-      double sub = fabs(sd.re) + fabs(sd.im);
-      if (sub <= eps * s || sub < 1e-300) break;
-    }
-    __syncthreads();
-    if (l > 0 && threadIdx.x == 0) T.set(l, l - 1, {0, 0});
-    __syncthreads();
-    if (l == ihi) {
-      --ihi;
-      iter = 0;
-      continue;
-    }
-    if (++total > 100 * mm) return -1;
-    cd mu;
-    if (iter > 0 && iter % 10 == 0) {
-      cd d = T.get(ihi, ihi), sd = T.get(ihi, ihi - 1);
-      mu = {d.re + 0.75 * fabs(sd.re), d.im};
-    } else {
-      cd a = T.get(ihi - 1, ihi - 1), b = T.get(ihi - 1, ihi), c = T.get(ihi, ihi - 1),
-         d = T.get(ihi, ihi);
-      cd h = {0.5 * (a.re - d.re), 0.5 * (a.im - d.im)};
-      cd q = cadd(cmul(h, h), cmul(b, c));
-      double mod = hypot(q.re, q.im);
-      double sr = sqrt(0.5 * (mod + q.re)), si = sqrt(0.5 * (mod - q.re));
-      if (q.im < 0) si = -si;
-      cd t1 = {h.re + sr, h.im + si}, t2 = {h.re - sr, h.im - si};
-      mu = cabs2(t1) < cabs2(t2) ? cadd(d, t1) : cadd(d, t2);
-    }
-    ++iter;
-    for (int kk = l; kk < ihi; ++kk) {
-      cd f, g;
-      if (kk == l) {
-        f = csub(T.get(l, l), mu);
-        g = T.get(l + 1, l);
-      } else {
-        f = T.get(kk, kk - 1);
-        g = T.get(kk + 1, kk - 1);
-      }
-      double c;
-      cd s;
-      cgivens(f, g, c, s);
-      __syncthreads();
-      rot_rows(T, kk, kk > l ? kk - 1 : kk, mm - 1, c, s);
-      __syncthreads();
-      if (kk > l && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
-      int r1 = kk + 2 < ihi ? kk + 2 : ihi;
-      rot_cols(T, kk, 0, r1, c, s);
-      rot_cols(Z, kk, 0, mm - 1, c, s);
-      __syncthreads();
-    }
-  }
-  return 0;
-}
-
-__device__ bool ritz_before(cd a, cd b, int which) {
-  double ma = cabs2(a), mb = cabs2(b);
-  switch (which) {
-    case 0: return ma > mb;
-    case 1: return a.re > b.re;
-    case 2: return a.im > b.im;
-    case 3: return ma < mb;
-    case 4: return a.re < b.re;
-    default: return a.im < b.im;
-  }
-}
-
-// One wave (64 lanes).  Dynamic LDS: T and Z, mact x mact complex each.
-// Global work: X (eigenvectors of T, m x m complex, column i for T[i][i]).
+// The small dense machinery (Schur form, eigenvectors, exact shifts) lives in
+// small_dense.h, shared with the batched solver.
 __global__ __launch_bounds__(64) void k_restart(RestartArgs a) {
   extern __shared__ double smem[];
-  const int mm = a.mact, ld = a.mact;
-  Mat T{smem, smem + mm * ld, ld};
-  Mat Z{smem + 2 * mm * ld, smem + 3 * mm * ld, ld};
-  __shared__ cd ev[kMaxM];
-  __shared__ double est[kMaxM];
-  __shared__ int order[kMaxM];
-  __shared__ double tnorm_s;
-  Status* st = a.st;
-  const int lane = threadIdx.x;
-
-  for (int idx = lane; idx < mm * mm; idx += 64) {
-    int r = idx / mm, c = idx - r * mm;
-    double hr = r <= c + 1 ? a.Hre[(size_t)r * a.m + c] : 0.0;
-    double hi = r <= c + 1 ? a.Him[(size_t)r * a.m + c] : 0.0;
-    T.set(r, c, {hr, hi});
-    Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
-  }
-  __syncthreads();
-  int fail = schur(T, Z, mm);
-  __syncthreads();
-  for (int i = lane; i < mm; i += 64) ev[i] = T.get(i, i);
-  // ||T|| (max modulus of the upper triangle) for the trevc safeguard
-  double tm = 0;
-  for (int idx = lane; idx < mm * mm; idx += 64) {
-    int r = idx / mm, c = idx - r * mm;
-    if (c >= r) tm = fmax(tm, hypot(T.re[r * ld + c], T.im[r * ld + c]));
-  }
-  for (int off = 32; off > 0; off >>= 1) tm = fmax(tm, __shfl_xor(tm, off, 64));
-  if (lane == 0) tnorm_s = tm;
-  __syncthreads();
-  const double small = fmax(tnorm_s * 2.220446049250313e-16, 1e-300);
-  // eigenvectors of T by back substitution: lane i owns column i of X
-  double* Xre = a.Xre;
-  double* Xim = a.Xim;
-  const double beta = st->beta;
-  for (int i = lane; i < mm; i += 64) {
-    cd li = ev[i];
-    for (int j = 0; j < mm; ++j) {
-      Xre[(size_t)j * a.m + i] = j == i ? 1.0 : 0.0;
-      Xim[(size_t)j * a.m + i] = 0.0;
-    }
-    for (int j = i - 1; j >= 0; --j) {
-      cd s = {0, 0};
-      for (int l = j + 1; l <= i; ++l) {
-        cd x = {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]};
-        s = cadd(s, cmul(T.get(j, l), x));
-      }
-      cd d = csub(T.get(j, j), li);
-      if (hypot(d.re, d.im) < small) d = {small, 0.0};
-      double den = cabs2(d);
-      Xre[(size_t)j * a.m + i] = -(s.re * d.re + s.im * d.im) / den;
-      Xim[(size_t)j * a.m + i] = -(s.im * d.re - s.re * d.im) / den;
-    }
-    // y = Z x: ||y|| = ||x|| (Z unitary); residual estimate beta |y[mm-1]| / ||y||
-    double nx = 0;
-    cd ylast = {0, 0};
-    for (int l = 0; l <= i; ++l) {
-      cd x = {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]};
-      nx += cabs2(x);
-      ylast = cadd(ylast, cmul(Z.get(mm - 1, l), x));
-    }
-    est[i] = beta * hypot(ylast.re, ylast.im) / sqrt(nx);
-  }
-  __syncthreads();
-  // stable rank sort by the wanted order
-  for (int i = lane; i < mm; i += 64) {
-    int rank = 0;
-    for (int j = 0; j < mm; ++j) {
-      bool bj = ritz_before(ev[j], ev[i], a.which);
-      bool bi = ritz_before(ev[i], ev[j], a.which);
-      if (bj || (!bi && j < i)) ++rank;
-    }
-    order[rank] = i;
-  }
-  __syncthreads();
-  if (lane == 0) {
-    double maxabs = 0;
-    for (int i = 0; i < mm; ++i) maxabs = fmax(maxabs, sqrt(cabs2(ev[i])));
-    double tolabs = a.tol * (a.anorm > 0 ? a.anorm : fmax(maxabs, 1e-300));
-    int nconv = 0;
-    for (int t = 0; t < a.k && t < mm; ++t)
-      if (est[order[t]] <= tolabs) ++nconv;
-    int p = mm - a.k;
-    int kk = a.k + (nconv < p / 2 ? nconv : p / 2);
-    if (kk == 1 && mm >= 6) kk = mm / 2;
-    else if (kk == 1 && mm > 2) kk = 2;
-    if (kk >= mm) kk = mm - 1;
-    st->nconv = nconv;
-    st->kk = kk;
-    st->schur_fail = fail;
-    st->maxabs = maxabs;
-    st->tolabs = tolabs;
-    for (int t = 0; t < mm; ++t) {
-      st->ev_re[t] = ev[order[t]].re;
-      st->ev_im[t] = ev[order[t]].im;
-      st->est[t] = est[order[t]];
-      st->order[t] = order[t];
-    }
-  }
-  __syncthreads();
-  // wanted Ritz vectors of H: Y[:, t] = Z x_{order[t]} / ||x||, t < k
-  for (int t = 0; t < a.k && t < mm; ++t) {
-    int i = order[t];
-    double nx = 0;
-    for (int l = 0; l <= i; ++l)
-      nx += Xre[(size_t)l * a.m + i] * Xre[(size_t)l * a.m + i] +
-            Xim[(size_t)l * a.m + i] * Xim[(size_t)l * a.m + i];
-    double inv = 1.0 / sqrt(nx);
-    for (int r = lane; r < mm; r += 64) {
-      cd y = {0, 0};
-      for (int l = 0; l <= i; ++l)
-        y = cadd(y, cmul(Z.get(r, l), {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]}));
-      a.Yre[(size_t)r * a.k + t] = y.re * inv;
-      a.Yim[(size_t)r * a.k + t] = y.im * inv;
-    }
-  }
-  __syncthreads();
-  if (!a.do_shifts) return;
-  // exact shifts: T <- H, Q <- I (Q reuses Z), sweeps with the unwanted values
-  const int kk = st->kk;
-  for (int idx = lane; idx < mm * mm; idx += 64) {
-    int r = idx / mm, c = idx - r * mm;
-    double hr = r <= c + 1 ? a.Hre[(size_t)r * a.m + c] : 0.0;
-    double hi = r <= c + 1 ? a.Him[(size_t)r * a.m + c] : 0.0;
-    T.set(r, c, {hr, hi});
-    Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
-  }
-  __syncthreads();
-  for (int t = kk; t < mm; ++t) shifted_sweep(T, Z, mm, ev[order[t]]);
-  __syncthreads();
-  for (int idx = lane; idx < mm * mm; idx += 64) {
-    int r = idx / mm, c = idx - r * mm;
-    a.Qre[(size_t)r * a.m + c] = Z.re[r * ld + c];
-    a.Qim[(size_t)r * a.m + c] = Z.im[r * ld + c];
-    bool keep = r < kk && c < kk;
-    a.Hre[(size_t)r * a.m + c] = keep ? T.re[r * ld + c] : 0.0;
-    a.Him[(size_t)r * a.m + c] = keep ? T.im[r * ld + c] : 0.0;
-  }
-  if (lane == 0) {
-    st->beta_k_re = T.re[kk * ld + kk - 1];
-    st->beta_k_im = T.im[kk * ld + kk - 1];
-    st->sigma_re = Z.re[(mm - 1) * ld + kk - 1];
-    st->sigma_im = Z.im[(mm - 1) * ld + kk - 1];
-  }
+  restart_body(a, smem);
 }
 
 // out[:, c] = sum_l V[:, l] M[l][c], c < ncols (out may not alias V).
@@ -665,13 +373,13 @@ oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const dou
 
 oaph_status restart(hipStream_t s, const RestartArgs& a) {
   OAPH_CHECK(a.mact >= 2 && a.mact <= kMaxM, OAPH_ERR_UNSUPPORTED);
-  size_t lds = (size_t)4 * a.mact * a.mact * sizeof(double);
+  size_t lds = (size_t)4 * a.mact * lds_ld(a.mact) * sizeof(double);
   static bool attr_set = false;
   if (!attr_set) {
-    // T and Z of a 64 x 64 problem need 128 KiB of the 160 KiB LDS
+    // T and Z of a 64 x 64 problem need 130 KiB of the 160 KiB LDS
     OAPH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_restart),
                                  hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 4 * kMaxM * kMaxM * (int)sizeof(double)));
+                                 4 * kMaxM * lds_ld(kMaxM) * (int)sizeof(double)));
     attr_set = true;
   }
   hipLaunchKernelGGL(k_restart, dim3(1), dim3(64), lds, s, a);

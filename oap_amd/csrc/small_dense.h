@@ -1,0 +1,405 @@
+// small_dense.h -- device code for the small dense part of the Arnoldi
+// restart (complex Schur form by shifted QR, eigenvectors, wanted ordering,
+// exact-shift sweeps), shared by the single-problem kernel (k_restart) and
+// the batched solver.  Same iteration as orc_hess_schur / orc_iram in
+// oracle/oracle.c.  Every function is called by all threads of the block:
+// rotations split their row/column updates over the threads, barriers
+// separate the row and column halves of each rotation.
+#pragma once
+
+#include "iram.h"
+
+namespace oaph {
+namespace iram {
+namespace {
+
+struct cd {
+  double re, im;
+};
+__device__ __forceinline__ cd cmul(cd a, cd b) { return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re}; }
+__device__ __forceinline__ cd cadd(cd a, cd b) { return {a.re + b.re, a.im + b.im}; }
+__device__ __forceinline__ cd csub(cd a, cd b) { return {a.re - b.re, a.im - b.im}; }
+__device__ __forceinline__ double cabs2(cd a) { return a.re * a.re + a.im * a.im; }
+
+// Block-wide ordering point.  A one-wave block needs no hardware barrier:
+// the LDS executes a wave's instructions in issue order, and the compiler
+// keeps possibly-aliasing LDS accesses in program order (the fence stops it
+// from moving memory operations across this point).
+__device__ __forceinline__ void bsync() {
+  if (blockDim.x > 64) {
+    __syncthreads();
+  } else {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_wave_barrier();
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  }
+}
+
+// G = [[c, s], [-conj s, c]] with real c, G [f; g] = [r; 0] (LAPACK zlartg
+// convention).  Reciprocal square roots on the common path; hypot-based
+// scaling when a modulus is near the under/overflow thresholds.
+__device__ void cgivens_safe(cd f, cd g, double& c, cd& s);
+__device__ __forceinline__ void cgivens(cd f, cd g, double& c, cd& s) {
+  const double af2 = f.re * f.re + f.im * f.im, ag2 = g.re * g.re + g.im * g.im;
+  if (ag2 == 0.0) { c = 1; s = {0, 0}; return; }
+  if (!(af2 > 1e-280 && ag2 > 1e-280 && af2 < 1e280 && ag2 < 1e280)) {
+    cgivens_safe(f, g, c, s);
+    return;
+  }
+  const double inv_r = rsqrt(af2 + ag2), inv_af = rsqrt(af2);
+  c = af2 * inv_af * inv_r;
+  const double ur = f.re * inv_af, ui = f.im * inv_af;
+  s = {(ur * g.re + ui * g.im) * inv_r, (ui * g.re - ur * g.im) * inv_r};
+}
+__device__ void cgivens_safe(cd f, cd g, double& c, cd& s) {
+  double af = hypot(f.re, f.im), ag = hypot(g.re, g.im);
+  if (ag == 0) { c = 1; s = {0, 0}; return; }
+  if (af == 0) { c = 0; s = {g.re / ag, -g.im / ag}; return; }
+  double rho = hypot(af, ag);
+  c = af / rho;
+  double ur = f.re / af, ui = f.im / af;
+  s = {(ur * g.re + ui * g.im) / rho, (ui * g.re - ur * g.im) / rho};
+}
+
+// LDS-resident m x m complex matrix, row-major
+struct Mat {
+  double* re;
+  double* im;
+  int ld;
+  __device__ cd get(int r, int c) const { return {re[r * ld + c], im[r * ld + c]}; }
+  __device__ void set(int r, int c, cd v) { re[r * ld + c] = v.re; im[r * ld + c] = v.im; }
+};
+
+// rows i, i+1 <- G rows, columns j0..j1 (lanes split the columns)
+__device__ void rot_rows(Mat M, int i, int j0, int j1, double c, cd s) {
+  for (int j = j0 + (int)threadIdx.x; j <= j1; j += blockDim.x) {
+    cd a = M.get(i, j), b = M.get(i + 1, j);
+    cd na = {c * a.re + (s.re * b.re - s.im * b.im), c * a.im + (s.re * b.im + s.im * b.re)};
+    cd nb = {-(s.re * a.re + s.im * a.im) + c * b.re, -(s.re * a.im - s.im * a.re) + c * b.im};
+    M.set(i, j, na);
+    M.set(i + 1, j, nb);
+  }
+}
+// columns j, j+1 <- columns G^H, rows r0..r1
+__device__ void rot_cols(Mat M, int j, int r0, int r1, double c, cd s) {
+  for (int r = r0 + (int)threadIdx.x; r <= r1; r += blockDim.x) {
+    cd a = M.get(r, j), b = M.get(r, j + 1);
+    cd na = {a.re * c + (b.re * s.re + b.im * s.im), a.im * c + (b.im * s.re - b.re * s.im)};
+    cd nb = {-(a.re * s.re - a.im * s.im) + b.re * c, -(a.re * s.im + a.im * s.re) + b.im * c};
+    M.set(r, j, na);
+    M.set(r, j + 1, nb);
+  }
+}
+
+// one implicit single-shift QR step with shift mu on the leading kdim block of
+// T, accumulating Q on the right.  The block is split at negligible
+// subdiagonals and the bulge chased inside each unreduced block (a sweep
+// started at the top of a reducible T stays the identity).  Same steps as
+// shifted_sweep in oracle/oracle.c.
+__device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu) {
+  for (int kk = lo; kk < hi; ++kk) {
+    cd f, g;
+    if (kk == lo) {
+      f = csub(T.get(lo, lo), mu);
+      g = T.get(lo + 1, lo);
+    } else {
+      f = T.get(kk, kk - 1);
+      g = T.get(kk + 1, kk - 1);
+    }
+    double c;
+    cd s;
+    cgivens(f, g, c, s);
+    bsync();
+    rot_rows(T, kk, kk > lo ? kk - 1 : kk, kdim - 1, c, s);
+    bsync();
+    if (kk > lo && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
+    int r1 = kk + 2 < hi ? kk + 2 : hi;
+    rot_cols(T, kk, 0, r1, c, s);
+    rot_cols(Q, kk, 0, kdim - 1, c, s);
+    bsync();
+  }
+}
+
+// Bit i (1 <= i < kdim) set when the subdiagonal entry (i, i-1) is
+// negligible: |t_{i,i-1}| <= eps (|t_{i-1,i-1}| + |t_{i,i}|).  Lane i tests
+// entry i; every wave computes the same mask.
+__device__ __forceinline__ unsigned long long negligible_mask(Mat T, int kdim) {
+  const double eps = 2.220446049250313e-16;
+  const int i = threadIdx.x & 63;
+  bool neg = false;
+  if (i >= 1 && i < kdim) {
+    cd a = T.get(i - 1, i - 1), b = T.get(i, i), sd = T.get(i, i - 1);
+    double d = fabs(a.re) + fabs(a.im) + fabs(b.re) + fabs(b.im);
+    double sub = fabs(sd.re) + fabs(sd.im);
+    neg = sub <= eps * d || sub < 1e-300;
+  }
+  return __ballot(neg);
+}
+
+__device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu) {
+  const unsigned long long mask = negligible_mask(T, kdim);
+  bsync();
+  {
+    const int i = threadIdx.x;
+    if (i < 64 && ((mask >> i) & 1ull)) T.set(i, i - 1, {0, 0});
+  }
+  bsync();
+  int lo = 0;
+  while (lo < kdim) {
+    // the block [lo, hi] ends before the next negligible subdiagonal above lo
+    const unsigned long long above = lo >= 63 ? 0ull : mask & ~((2ull << lo) - 1ull);
+    const int hi = above ? __builtin_ctzll(above) - 1 : kdim - 1;
+    if (hi > lo) chase(T, Q, kdim, lo, hi, mu);
+    lo = hi + 1;
+  }
+}
+
+__device__ int schur(Mat T, Mat Z, int mm) {
+  int ihi = mm - 1, iter = 0, total = 0;
+  while (ihi >= 0) {
+    // largest l <= ihi with a negligible subdiagonal (0 if none)
+    const unsigned long long upto = ihi >= 63 ? ~0ull : ((2ull << ihi) - 1ull);
+    const unsigned long long neg = negligible_mask(T, mm) & upto;
+    const int l = neg ? 63 - __builtin_clzll(neg) : 0;
+    bsync();
+    if (l > 0 && threadIdx.x == 0) T.set(l, l - 1, {0, 0});
+    bsync();
+    if (l == ihi) {
+      --ihi;
+      iter = 0;
+      continue;
+    }
+    if (++total > 100 * mm) return -1;
+    cd mu;
+    if (iter > 0 && iter % 10 == 0) {
+      cd d = T.get(ihi, ihi), sd = T.get(ihi, ihi - 1);
+      mu = {d.re + 0.75 * fabs(sd.re), d.im};
+    } else {
+      cd a = T.get(ihi - 1, ihi - 1), b = T.get(ihi - 1, ihi), c = T.get(ihi, ihi - 1),
+         d = T.get(ihi, ihi);
+      cd h = {0.5 * (a.re - d.re), 0.5 * (a.im - d.im)};
+      cd q = cadd(cmul(h, h), cmul(b, c));
+      double mod = hypot(q.re, q.im);
+      double sr = sqrt(0.5 * (mod + q.re)), si = sqrt(0.5 * (mod - q.re));
+      if (q.im < 0) si = -si;
+      cd t1 = {h.re + sr, h.im + si}, t2 = {h.re - sr, h.im - si};
+      mu = cabs2(t1) < cabs2(t2) ? cadd(d, t1) : cadd(d, t2);
+    }
+    ++iter;
+    for (int kk = l; kk < ihi; ++kk) {
+      cd f, g;
+      if (kk == l) {
+        f = csub(T.get(l, l), mu);
+        g = T.get(l + 1, l);
+      } else {
+        f = T.get(kk, kk - 1);
+        g = T.get(kk + 1, kk - 1);
+      }
+      double c;
+      cd s;
+      cgivens(f, g, c, s);
+      bsync();
+      rot_rows(T, kk, kk > l ? kk - 1 : kk, mm - 1, c, s);
+      bsync();
+      if (kk > l && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
+      int r1 = kk + 2 < ihi ? kk + 2 : ihi;
+      rot_cols(T, kk, 0, r1, c, s);
+      rot_cols(Z, kk, 0, mm - 1, c, s);
+      bsync();
+    }
+  }
+  return 0;
+}
+
+__host__ __device__ constexpr int lds_ld(int m) { return m | 1; }
+
+__device__ bool ritz_before(cd a, cd b, int which) {
+  double ma = cabs2(a), mb = cabs2(b);
+  switch (which) {
+    case 0: return ma > mb;
+    case 1: return a.re > b.re;
+    case 2: return a.im > b.im;
+    case 3: return ma < mb;
+    case 4: return a.re < b.re;
+    default: return a.im < b.im;
+  }
+}
+
+// The restart of one problem by one workgroup (any block size; the one-wave
+// k_restart and the batched solver both call it).  smem: T and Z, mact x
+// mact complex each with the odd leading dimension lds_ld(mact), so that a
+// column walk (one row per lane) spreads over the LDS banks
+// (4 mact lds_ld(mact) doubles).  Global work a.X: eigenvectors of T
+// (m x m complex, column i for T[i][i]).  Outputs: status (nconv, kk, sorted
+// Ritz values, estimates), Y (wanted Ritz vectors of H), and with
+// a.do_shifts the compressed H and the accumulated Q of the exact shifts.
+__device__ void restart_body(const RestartArgs& a, double* smem) {
+  const int mm = a.mact, ld = lds_ld(a.mact);
+  Mat T{smem, smem + mm * ld, ld};
+  Mat Z{smem + 2 * mm * ld, smem + 3 * mm * ld, ld};
+  __shared__ cd ev[kMaxM];
+  __shared__ double est[kMaxM];
+  __shared__ int order[kMaxM];
+  __shared__ double tnorm_s;
+  __shared__ int kk_s;  // kept size, broadcast through LDS (uniform loop bound)
+  Status* st = a.st;
+  const int lane = threadIdx.x, nthr = blockDim.x;
+  __shared__ double tmax_s[1024];
+
+  if (lane == 0) st->clk[0] = clock64();
+  for (int idx = lane; idx < mm * mm; idx += nthr) {
+    int r = idx / mm, c = idx - r * mm;
+    double hr = r <= c + 1 ? a.Hre[(size_t)r * a.m + c] : 0.0;
+    double hi = r <= c + 1 ? a.Him[(size_t)r * a.m + c] : 0.0;
+    T.set(r, c, {hr, hi});
+    Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
+  }
+  __syncthreads();
+  int fail = schur(T, Z, mm);
+  __syncthreads();
+  if (lane == 0) st->clk[1] = clock64();
+  for (int i = lane; i < mm; i += nthr) ev[i] = T.get(i, i);
+  // ||T|| (max modulus of the upper triangle) for the trevc safeguard
+  double tm = 0;
+  for (int idx = lane; idx < mm * mm; idx += nthr) {
+    int r = idx / mm, c = idx - r * mm;
+    if (c >= r) tm = fmax(tm, hypot(T.re[r * ld + c], T.im[r * ld + c]));
+  }
+  tmax_s[lane] = tm;
+  __syncthreads();
+  if (lane == 0) {
+    double v = 0;
+    for (int t = 0; t < nthr; ++t) v = fmax(v, tmax_s[t]);
+    tnorm_s = v;
+  }
+  __syncthreads();
+  const double small = fmax(tnorm_s * 2.220446049250313e-16, 1e-300);
+  // eigenvectors of T by back substitution: lane i owns column i of X
+  double* Xre = a.Xre;
+  double* Xim = a.Xim;
+  const double beta = st->beta;
+  for (int i = lane; i < mm; i += nthr) {
+    cd li = ev[i];
+    for (int j = 0; j < mm; ++j) {
+      Xre[(size_t)j * a.m + i] = j == i ? 1.0 : 0.0;
+      Xim[(size_t)j * a.m + i] = 0.0;
+    }
+    for (int j = i - 1; j >= 0; --j) {
+      cd s = {0, 0};
+      for (int l = j + 1; l <= i; ++l) {
+        cd x = {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]};
+        s = cadd(s, cmul(T.get(j, l), x));
+      }
+      cd d = csub(T.get(j, j), li);
+      if (hypot(d.re, d.im) < small) d = {small, 0.0};
+      double den = cabs2(d);
+      Xre[(size_t)j * a.m + i] = -(s.re * d.re + s.im * d.im) / den;
+      Xim[(size_t)j * a.m + i] = -(s.im * d.re - s.re * d.im) / den;
+    }
+    // y = Z x: ||y|| = ||x|| (Z unitary); residual estimate beta |y[mm-1]| / ||y||
+    double nx = 0;
+    cd ylast = {0, 0};
+    for (int l = 0; l <= i; ++l) {
+      cd x = {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]};
+      nx += cabs2(x);
+      ylast = cadd(ylast, cmul(Z.get(mm - 1, l), x));
+    }
+    est[i] = beta * hypot(ylast.re, ylast.im) / sqrt(nx);
+  }
+  __syncthreads();
+  if (lane == 0) st->clk[2] = clock64();
+  // stable rank sort by the wanted order
+  for (int i = lane; i < mm; i += nthr) {
+    int rank = 0;
+    for (int j = 0; j < mm; ++j) {
+      bool bj = ritz_before(ev[j], ev[i], a.which);
+      bool bi = ritz_before(ev[i], ev[j], a.which);
+      if (bj || (!bi && j < i)) ++rank;
+    }
+    order[rank] = i;
+  }
+  __syncthreads();
+  if (lane == 0) {
+    double maxabs = 0;
+    for (int i = 0; i < mm; ++i) maxabs = fmax(maxabs, sqrt(cabs2(ev[i])));
+    double tolabs = a.tol * (a.anorm > 0 ? a.anorm : fmax(maxabs, 1e-300));
+    int nconv = 0;
+    for (int t = 0; t < a.k && t < mm; ++t)
+      if (est[order[t]] <= tolabs) ++nconv;
+    // kept size k + min(nconv, (m - k) / 2) (ARPACK's nev adjustment):
+    // keeping only k lets an unconverged wanted eigenvalue be pushed out by
+    // the shifts, and the solve then converges to a neighbour outside the
+    // wanted set on clustered spectra
+    const int p2 = (mm - a.k) / 2;
+    int kk = a.k + (nconv < p2 ? nconv : p2);
+    if (kk == 1 && mm >= 6) kk = mm / 2;
+    else if (kk == 1 && mm > 2) kk = 2;
+    if (kk >= mm) kk = mm - 1;
+    st->nconv = nconv;
+    st->kk = kk;
+    kk_s = kk;
+    st->schur_fail = fail;
+    st->maxabs = maxabs;
+    st->tolabs = tolabs;
+    for (int t = 0; t < mm; ++t) {
+      st->ev_re[t] = ev[order[t]].re;
+      st->ev_im[t] = ev[order[t]].im;
+      st->est[t] = est[order[t]];
+      st->order[t] = order[t];
+    }
+  }
+  __syncthreads();
+  if (lane == 0) st->clk[3] = clock64();
+  // wanted Ritz vectors of H: Y[:, t] = Z x_{order[t]} / ||x||, t < k
+  for (int t = 0; t < a.k && t < mm; ++t) {
+    int i = order[t];
+    double nx = 0;
+    for (int l = 0; l <= i; ++l)
+      nx += Xre[(size_t)l * a.m + i] * Xre[(size_t)l * a.m + i] +
+            Xim[(size_t)l * a.m + i] * Xim[(size_t)l * a.m + i];
+    double inv = 1.0 / sqrt(nx);
+    for (int r = lane; r < mm; r += nthr) {
+      cd y = {0, 0};
+      for (int l = 0; l <= i; ++l)
+        y = cadd(y, cmul(Z.get(r, l), {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]}));
+      a.Yre[(size_t)r * a.k + t] = y.re * inv;
+      a.Yim[(size_t)r * a.k + t] = y.im * inv;
+    }
+  }
+  __syncthreads();
+  if (lane == 0) st->clk[4] = clock64();
+  if (!a.do_shifts) return;  // uniform: every thread returns
+  // exact shifts: T <- H, Q <- I (Q reuses Z), sweeps with the unwanted values
+  const int kk = kk_s;
+  for (int idx = lane; idx < mm * mm; idx += nthr) {
+    int r = idx / mm, c = idx - r * mm;
+    double hr = r <= c + 1 ? a.Hre[(size_t)r * a.m + c] : 0.0;
+    double hi = r <= c + 1 ? a.Him[(size_t)r * a.m + c] : 0.0;
+    T.set(r, c, {hr, hi});
+    Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
+  }
+  __syncthreads();
+  for (int t = kk; t < mm; ++t) shifted_sweep(T, Z, mm, ev[order[t]]);
+  __syncthreads();
+  if (lane == 0) st->clk[5] = clock64();
+  for (int idx = lane; idx < mm * mm; idx += nthr) {
+    int r = idx / mm, c = idx - r * mm;
+    a.Qre[(size_t)r * a.m + c] = Z.re[r * ld + c];
+    a.Qim[(size_t)r * a.m + c] = Z.im[r * ld + c];
+    bool keep = r < kk && c < kk;
+    a.Hre[(size_t)r * a.m + c] = keep ? T.re[r * ld + c] : 0.0;
+    a.Him[(size_t)r * a.m + c] = keep ? T.im[r * ld + c] : 0.0;
+  }
+  if (lane == 0) {
+    st->beta_k_re = T.re[kk * ld + kk - 1];
+    st->beta_k_im = T.im[kk * ld + kk - 1];
+    st->sigma_re = Z.re[(mm - 1) * ld + kk - 1];
+    st->sigma_im = Z.im[(mm - 1) * ld + kk - 1];
+  }
+  __syncthreads();
+}
+
+
+}  // namespace
+}  // namespace iram
+}  // namespace oaph

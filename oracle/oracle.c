@@ -631,8 +631,8 @@ void orc_gen_c1(int n, uint64_t seed, double eps, double decay, double* are, dou
  * complex scaling, correctly accumulated Q, Ritz values from the shifted
  * complex Hessenberg QR above, Ritz vectors V y instead of basis columns,
  * residual-estimate stop |beta_m e_m^T y| <= tol*anorm, seeded start vector,
- * the ARPACK rule k_eff = k + min(nconv, p/2) against stagnation, and a
- * second classical Gram-Schmidt pass on every step (DGKS). */
+ * shifts chased inside unreduced blocks, and a second classical Gram-Schmidt
+ * pass on every step (DGKS). */
 /* ------------------------------------------------------------------ */
 typedef struct { double re, im; } cplx_t;
 
@@ -863,6 +863,10 @@ int orc_iram(const orc_iram_opts* o, orc_matvec_fn mv, void* user, double* ev_re
     }
     if (nconv >= k || restarts >= o->max_restarts) break;
     /* exact-shift restart with p = m - kk unwanted Ritz values */
+    /* kept size k + min(nconv, (m - k) / 2), ARPACK's nev adjustment:
+     * keeping only k lets the shifts push an unconverged wanted eigenvalue
+     * out, and the solve then converges to a neighbour outside the wanted
+     * set on clustered spectra */
     int kk = k + (nconv < (m - k) / 2 ? nconv : (m - k) / 2);
     if (kk == 1 && m >= 6) kk = m / 2;
     else if (kk == 1 && m > 2) kk = 2;
@@ -942,4 +946,45 @@ int orc_iram(const orc_iram_opts* o, orc_matvec_fn mv, void* user, double* ev_re
   if (res) { res->nconv = nconv; res->restarts = restarts; res->matvecs = matvecs; }
   free(vre); free(vim); free(wre); free(sm); free(xre_h); free(vtmp_re); free(order);
   return status;
+}
+
+/* ------------------------------------------------------------------ */
+/* C3 generator: X_i^T X_i / s covariance problems (the operator of the
+ * reference's RecToSquareApi, oapMatrixCuda/RecToSquareApi.cpp:57-141, on
+ * synthetic 16-bit grey patches with three planted rank-1 components)    */
+/* ------------------------------------------------------------------ */
+uint64_t orc_c3_stream(uint64_t seed, int i) {
+  return orc_splitmix64(seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(i + 1));
+}
+
+int orc_gen_c3_problem(int i, uint64_t seed, int s, float* X) {
+  const uint64_t si = orc_c3_stream(seed, i);
+  const int n = 64 + (int)(961.0 * orc_uniform(si, 0));
+  if (!X) return n;
+  double* a = (double*)malloc(sizeof(double) * (size_t)3 * s);
+  double* b = (double*)malloc(sizeof(double) * (size_t)3 * n);
+  for (int c = 0; c < 3; ++c) {
+    double na = 0, nb = 0;
+    for (int r = 0; r < s; ++r) {
+      a[c * s + r] = orc_normal(si ^ 0x2ULL, (uint64_t)c * s + r);
+      na += a[c * s + r] * a[c * s + r];
+    }
+    for (int j = 0; j < n; ++j) {
+      b[c * n + j] = orc_normal(si ^ 0x3ULL, (uint64_t)c * n + j);
+      nb += b[c * n + j] * b[c * n + j];
+    }
+    na = 1.0 / sqrt(na);
+    nb = 1.0 / sqrt(nb);
+    for (int r = 0; r < s; ++r) a[c * s + r] *= na;
+    for (int j = 0; j < n; ++j) b[c * n + j] *= nb;
+  }
+  for (int r = 0; r < s; ++r)
+    for (int j = 0; j < n; ++j) {
+      double v = floor(65535.0 * orc_uniform(si ^ 0x1ULL, (uint64_t)r * n + j) + 0.5) / 65535.0;
+      v += 5.0 * (a[r] * b[j] + a[s + r] * b[n + j] + a[2 * s + r] * b[2 * n + j]);
+      X[(size_t)r * n + j] = (float)v;
+    }
+  free(a);
+  free(b);
+  return n;
 }

@@ -133,6 +133,14 @@ void orc_gen_c0(int n, uint64_t seed, double* are);
 /* C1: A = diag(d) + eps*(G_re + i G_im)/sqrt(n), d_k = exp(-k/decay) e^{2 pi i u_k}. */
 void orc_gen_c1(int n, uint64_t seed, double eps, double decay, double* are, double* aim);
 
+/* C3 (batched covariance problems): problem i of a batch with base seed
+ * `seed`: n_i = 64 + floor(961 u) in [64, 1024]; X_i (s x n_i, row-major,
+ * FP32) = round(65535 u)/65535 + 5 sum_{c<3} a_c b_c^T with unit-norm
+ * Gaussian a_c (length s) and b_c (length n_i); the operator is
+ * C_i = X_i^T X_i / s.  Returns n_i; writes X when X != NULL. */
+int orc_gen_c3_problem(int i, uint64_t seed, int s, float* X);
+uint64_t orc_c3_stream(uint64_t seed, int i);
+
 #ifdef __cplusplus
 }
 #endif

@@ -123,6 +123,8 @@ class Oracle:
         L.orc_splitmix64.restype = C.c_uint64
         L.orc_gen_c0.argtypes = [C.c_int, C.c_uint64, DP]
         L.orc_gen_c1.argtypes = [C.c_int, C.c_uint64, C.c_double, C.c_double, DP, DP]
+        L.orc_gen_c3_problem.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_float)]
+        L.orc_gen_c3_problem.restype = C.c_int
 
     # ---- procedures (return the output Planes)
     def dot_product(self, out: Planes, a: Planes, b: Planes, dims=None) -> int:
@@ -229,6 +231,13 @@ class Oracle:
         a = np.zeros((n, n))
         self.lib.orc_gen_c0(n, seed, _p(a))
         return a
+
+    def gen_c3(self, i, seed=0x5EED0003, s=256):
+        """Problem i of the seeded covariance batch: X (s x n float32)."""
+        n = self.lib.orc_gen_c3_problem(i, seed, s, None)
+        X = np.zeros((s, n), dtype=np.float32)
+        self.lib.orc_gen_c3_problem(i, seed, s, X.ctypes.data_as(C.POINTER(C.c_float)))
+        return X
 
     def gen_c1(self, n=8192, seed=0x5EED0001, eps=1e-2, decay=500.0):
         ar, ai = np.zeros((n, n)), np.zeros((n, n))
