@@ -2,22 +2,28 @@
 """Benchmark: converged eigenpairs/s of the implicitly restarted Arnoldi
 solver on one MI355X.
 
-Workload (one "step" = one full solve): dense complex 8192 x 8192 operator in
-split re/im FP64 planes, A = diag(d) + 1e-2 (G_re + i G_im)/sqrt(8192),
-d_k = exp(-k/500) e^{2 pi i u_k} (seed 0x5EED0001, generated in HBM), the 8
-largest-magnitude eigenpairs, Krylov dimension 48, accepted when the explicit
-FP64 residual ||A x - lambda x||_2 <= 1e-10 ||A||_F.  The operator is resident
-in HBM before the timed region.
+Workload (one "step" = one full solve from a fresh seeded random start
+vector): dense complex 8192 x 8192 operator in split re/im FP64 planes,
+A = diag(d) + 1e-2 (G_re + i G_im)/sqrt(8192), d_k = exp(-k/500) e^{2 pi i u_k}
+(seed 0x5EED0001, generated in HBM), the 8 largest-magnitude eigenpairs,
+Krylov dimension 48, a pair counted when its explicit FP64 residual
+||A x - lambda x||_2 <= 1e-10 ||A||_F.  The operator is resident in HBM before
+the timed region; step i starts from seed 1000 + i (warm-up steps use other
+seeds), so the steps sample the solver's start-vector dependence.
 
 Reported `value`: eigenpairs/s of the mixed-precision solver (FP32 matvec
-bulk, FP64 orthogonalisation, QR, shifts, polish and residual check); the
-plain-FP64 solver is reported beside it (`plain_fp64`) with the ratio.
+bulk, FP64 orthogonalisation, QR, shifts, polish and residual check) over the
+K timed steps; the plain-FP64 solver on the same start vectors is reported
+beside it (`plain_fp64`).  `batched_c3` is the 512-problem covariance batch
+(n_i in [64, 1024], s = 256, k = 4, m = 16) solved in one persistent launch.
 `roofline` is for the dominant kernel of the reported solve (the FP32-operand
-matvec), timed with HIP events on the solver's stream; `cpu_baseline` times
-the oracle's threaded FP64 matvec on the host cores and extrapolates with the
-FP64 GPU solve's matvec count.
+matvec), timed with HIP events on the solver's stream; `cpu_baseline` runs the
+oracle's FP64 IRAM (threaded dense matvec) for one Arnoldi cycle on the host
+cores and scales it by the FP64 GPU solve's matvec count.
 
-Usage: python bench.py [--gpus 1] [--steps 3] [--warmup 1]
+Usage: python bench.py [--gpus 1] [--steps 6] [--warmup 1]
+       python bench.py --traffic-probe   (a few matvec launches and a copy
+       kernel of known size, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
 """
 from __future__ import annotations
 
@@ -51,7 +57,6 @@ def make_solver(dA, precision):
     s.setSortType(ArnUtils.SortLargestValues)
     s.setCheckTolerance(TOL)
     s.setOutputType(ArnUtils.DEVICE)
-    s.setSeed(12345)
     s.setMaxRestarts(2000)
     s.setPrecision(precision)
     s.setMatrix(dA)
@@ -63,59 +68,146 @@ def make_solver(dA, precision):
 def solve_series(s, steps, warmup, fro):
     from oap_amd.api import MatrixInfo
     info = MatrixInfo(True, True, N, N)
-    for _ in range(warmup):
+    for w in range(warmup):
+        s.setSeed(900 + w)
         s.execute(M, K, info, raise_on_not_converged=False)
     good = 0
-    stats = []
+    stats, per_run = [], []
     t0 = time.perf_counter()
     for i in range(steps):
+        s.setSeed(1000 + i)
+        t1 = time.perf_counter()
         s.execute(M, K, info, raise_on_not_converged=False)
+        dt = time.perf_counter() - t1
         st = s.stats()
         ok = sum(1 for t in range(K) if s.testOutcome(t) <= TOL * fro)
         good += ok
         stats.append(st)
+        per_run.append(ok / dt)
         log(f"  solve {i}: {ok}/{K} pairs converged, {st['matvecs']} matvecs "
-            f"({st['matvecs_low']} FP32), {st['restarts']} restarts, {st['seconds']*1e3:.1f} ms, "
+            f"({st['matvecs_low']} FP32), {st['restarts']} restarts, {dt*1e3:.1f} ms, "
             f"max resid {st['max_resid']:.3e}")
     wall = time.perf_counter() - t0
-    return good, wall, stats
+    return good, wall, stats, float(np.median(per_run))
 
 
 def cpu_baseline(matvecs_fp64: int, threads: int) -> dict:
-    """The oracle's threaded FP64 matvec on this host, extrapolated to a solve."""
+    """The oracle's FP64 IRAM for one Arnoldi cycle (m matvecs with the
+    threaded dense matvec, CGS2, one restart) on this host, scaled to a solve."""
     from oracle.pyoracle import Oracle
     orc = Oracle()
     t0 = time.perf_counter()
     ar, ai = orc.gen_c1(N, SEED, EPS_OFF, DECAY)
     gen_s = time.perf_counter() - t0
-    x = np.random.default_rng(0).standard_normal(N) + 0j
-    reps = 3
-    orc.dense_matvec(ar, ai, x, threads)
+
+    def mv(x):
+        return orc.dense_matvec(ar, ai, x, threads)
+
     t0 = time.perf_counter()
-    for _ in range(reps):
-        orc.dense_matvec(ar, ai, x, threads)
-    t_mv = (time.perf_counter() - t0) / reps
+    _, _, _, _, info = orc.iram(mv, N, M, K, 0, TOL, 1.0, max_restarts=1, seed=1000)
+    t_cycle = time.perf_counter() - t0
     del ar, ai
-    est_solve = t_mv * matvecs_fp64
-    return {"value": K / est_solve, "unit": "eigenpairs/s", "cores": threads, "kind": "port",
-            "sample": (f"oracle orc_dense_matvec (FP64, {threads} threads) on the same 8192^2 "
-                       f"complex operator: {reps} timed matvecs, {t_mv*1e3:.1f} ms each, x "
-                       f"{matvecs_fp64} matvecs of the FP64 GPU solve (orthogonalisation not "
-                       f"counted, so an upper bound on CPU throughput); generation {gen_s:.1f} s")}
+    per_matvec = t_cycle / info["matvecs"]
+    est_solve = per_matvec * matvecs_fp64
+    return {"value": round(K / est_solve, 5), "unit": "eigenpairs/s", "cores": threads,
+            "kind": "port",
+            "sample": (f"oracle orc_iram (FP64, dense matvec on {threads} threads) on the same "
+                       f"8192^2 complex operator for one restart cycle: {info['matvecs']} matvecs, "
+                       f"{t_cycle:.1f} s incl. orthogonalisation and restart; scaled by the "
+                       f"{matvecs_fp64} matvecs (median) of the FP64 GPU solve; operator "
+                       f"generation {gen_s:.1f} s not counted")}
+
+
+def batched_c3(steps: int) -> dict:
+    """The 512-problem covariance batch: autotuned launch shape, then `steps`
+    timed solves (each one persistent launch)."""
+    from oap_amd.api import CovarianceBatch
+    seed, s, m, k = 0x5EED0003, 256, 16, 4
+    n = CovarianceBatch.c3_sizes(512, seed)
+    b = CovarianceBatch(n, s)
+    b.synth_c3(seed)
+    variant, times = b.autotune(m, k, reps=1)
+    shapes = CovarianceBatch.variants()
+    log(f"batched C3: autotune {[(shapes[i], round(t, 3)) for i, t in enumerate(times)]} "
+        f"-> {shapes[variant]}")
+    good, kms, wall = 0, [], 0.0
+    res = None
+    for i in range(steps):
+        t0 = time.perf_counter()
+        b.solve(m, k, 0, TOL, 1000, 0x5EED5EED + i)
+        wall += time.perf_counter() - t0
+        res = b.results()
+        lmax = res["evals"][:, :1]
+        good += int(np.sum(res["resid"] <= TOL * lmax))
+        kms.append(res["kernel_ms"])
+    ldx = (n.astype(np.int64) + 3) // 4 * 4
+    xbytes = float(np.sum(res["matvecs"].astype(np.int64) * s * ldx * 4))
+    kmed = float(np.median(kms))
+    return {"value": round(good / wall, 1), "unit": "eigenpairs/s", "pairs_per_solve": 512 * k,
+            "ms_per_solve": round(wall / steps * 1e3, 3), "kernel_ms": round(kmed, 3),
+            "variant": {"threads": shapes[variant][0], "rows_per_wave_step": shapes[variant][1]},
+            "matvecs_mean": round(float(np.mean(res["matvecs"])), 1),
+            "restarts_mean": round(float(np.mean(res["restarts"])), 2),
+            "residual_norm": "||C x - lambda x||_2 <= 1e-10 lambda_max (= ||C||_2)",
+            "roofline": {"bound": "hbm", "achieved": round(xbytes / (kmed * 1e-3) / 1e9, 1),
+                         "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": round(xbytes / (kmed * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                         "bytes": xbytes,
+                         "note": "X streamed once per implicit X^T X v / s; whole solve "
+                                 "(restarts included) in the kernel time"}}
+
+
+def profiled_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)."""
+    import re
+    f = ROOT / "profiles" / "traffic_matvec_c1.md"
+    if not f.exists():
+        return None
+    m = re.search(rf"^{kernel}_hbm_bytes_per_launch:\s*([0-9.eE+]+)", f.read_text(), re.M)
+    return float(m.group(1)) if m else None
+
+
+def traffic_probe() -> int:
+    """Launches for the rocprofv3 counter runs: 5 FP32-operand matvecs, 5 FP64
+    matvecs, and 5 multiply_re_constant copies of a 2^27-double plane (1 GiB
+    read + 1 GiB write) as the calibration kernel of known size."""
+    import ctypes as C
+    from oap_amd._lib import check, load
+    from oap_amd.api import CuProceduresApi, DeviceMatrix, Precision
+    lib = load()
+    dA = DeviceMatrix(N, N, True, True)
+    check(lib.oaph_synth_matrix(dA.ptr, 1, SEED, EPS_OFF, DECAY), "synth")
+    s, _, _ = make_solver(dA, Precision.MIXED_FP32)
+    ms = C.c_double()
+    check(lib.oaph_arnoldi_time_matvec(s._h, 1, 5, C.byref(ms)), "time_matvec")
+    check(lib.oaph_arnoldi_time_matvec(s._h, 0, 5, C.byref(ms)), "time_matvec")
+    api = CuProceduresApi()
+    a = DeviceMatrix(1 << 14, 1 << 13, True, False)
+    o = DeviceMatrix(1 << 14, 1 << 13, True, False)
+    for _ in range(5):
+        api.multiplyReConstant(o, a, 2.0)
+    api.synchronize()
+    print("traffic probe done", flush=True)
+    return 0
 
 
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=6)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the solver's multi-GPU sharding (row blocks of A, "
               "all-gather of v, all-reduce of V^H w) is a design, not built; run with --gpus 1",
               file=sys.stderr)
         return 2
+    if args.traffic_probe:
+        return traffic_probe()
 
     from oap_amd._lib import check, load
     from oap_amd.api import DeviceMatrix, Precision
@@ -129,7 +221,7 @@ def main() -> int:
     import ctypes as C
     s64, _, _ = make_solver(dA, Precision.FP64)
     log("plain FP64 solver:")
-    good64, wall64, st64 = solve_series(s64, args.steps, args.warmup, fro)
+    good64, wall64, st64, med64 = solve_series(s64, args.steps, args.warmup, fro)
     ms = C.c_double()
     check(lib.oaph_arnoldi_time_matvec(s64._h, 0, 20, C.byref(ms)), "time_matvec")
     t64 = ms.value
@@ -137,7 +229,7 @@ def main() -> int:
     # mixed precision
     smx, re, im = make_solver(dA, Precision.MIXED_FP32)
     log("mixed FP32/FP64 solver:")
-    goodmx, wallmx, stmx = solve_series(smx, args.steps, args.warmup, fro)
+    goodmx, wallmx, stmx, medmx = solve_series(smx, args.steps, args.warmup, fro)
     check(lib.oaph_arnoldi_time_matvec(smx._h, 1, 20, C.byref(ms)), "time_matvec")
     t32 = ms.value
 
@@ -148,6 +240,9 @@ def main() -> int:
     eps64 = good64 / wall64
     epsmx = goodmx / wallmx
     matvecs64 = int(np.median([s["matvecs"] for s in st64]))
+    batched = None
+    if not args.no_batched:
+        batched = batched_c3(args.steps)
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -170,11 +265,16 @@ def main() -> int:
         "config": {"workload": "dense complex 8192x8192 IRAM, k=8 largest |lambda|, m=48, "
                                "tol 1e-10 ||A||_F (C1 matrix, mixed precision)",
                    "n": N, "k": K, "m": M, "norm_a_fro": fro},
+        "median_per_run": round(medmx, 4),
+        "matvecs_median": {"fp32": int(np.median([s["matvecs_low"] for s in stmx])),
+                           "fp64": int(np.median([s["matvecs"] - s["matvecs_low"] for s in stmx]))},
         "plain_fp64": {"value": round(eps64, 4), "ms_per_step": round(wall64 / args.steps * 1e3, 3),
-                       "matvecs": matvecs64, "ratio_mixed_over_fp64": round(epsmx / eps64, 3)
-                       if eps64 > 0 else None},
+                       "median_per_run": round(med64, 4), "matvecs_median": matvecs64,
+                       "ratio_mixed_over_fp64": round(epsmx / eps64, 3) if eps64 > 0 else None},
+        "batched_c3": batched,
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4),
+                     "traffic": profiled_traffic("k_cgemv32"),
                      "kernel": "k_cgemv32 (FP32 operand, FP64 accumulate)",
                      "kernel_ms": round(t32, 5), "bytes_per_launch": bytes32},
         "roofline_fp64_matvec": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBS, 4),
