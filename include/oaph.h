@@ -301,6 +301,95 @@ oaph_status oaph_time_dot_product(oaph_api* api, oaph_matrix* out, const oaph_ma
                                   const oaph_matrix* b, uint32_t iters, double* avg_ms);
 
 /* ------------------------------------------------------------------ */
+/* Activation procedures (CuProceduresApi.h:239-290): every plane of the
+ * output on its own; OAPH_FN_MULTIPLY_* multiply the output in place by the
+ * derivative at the input.                                               */
+/* ------------------------------------------------------------------ */
+#define OAPH_FN_SIGMOID 0
+#define OAPH_FN_DSIGMOID 1
+#define OAPH_FN_MULTIPLY_DSIGMOID 2
+#define OAPH_FN_LINEAR 3
+#define OAPH_FN_DLINEAR 4
+#define OAPH_FN_TANH 5
+#define OAPH_FN_DTANH 6
+#define OAPH_FN_MULTIPLY_DTANH 7
+#define OAPH_FN_SIN 8
+#define OAPH_FN_DSIN 9 /* cos */
+#define OAPH_FN_MULTIPLY_DSIN 10
+
+/* sigmoid(out, in) ... dsin(out, in) over the whole matrix */
+oaph_status oaph_activation(oaph_api* api, int fn, oaph_matrix* out, const oaph_matrix* in);
+/* ... (out, in, dims[2]): elements with column < dims[0] and row < dims[1] */
+oaph_status oaph_activation_dims(oaph_api* api, int fn, oaph_matrix* out, const oaph_matrix* in,
+                                 const uint32_t dims[2]);
+/* ... (out, in, dims[2][2]): column < dims[0][0] and (row mod dims[1][1]) <
+ * dims[0][1] (the periodic form, CuFuncProcedures.h cuda_inRangePD) */
+oaph_status oaph_activation_dims_periodic(oaph_api* api, int fn, oaph_matrix* out,
+                                          const oaph_matrix* in, const uint32_t dims[2][2]);
+
+/* ------------------------------------------------------------------ */
+/* FP64 feed-forward network with oap's training semantics
+ * (oapNeuralDevice/oapNetwork.h:33-120, oapNeuralHost/oapGenericNeuralApi.h
+ * :195-364): per-sample forward / accumulateErrors (MEAN_SQUARE_ERROR) /
+ * backPropagation, updateWeights after a batch.  Activations: OAPH_FN_SIGMOID,
+ * _LINEAR, _TANH, _SIN.                                                   */
+/* ------------------------------------------------------------------ */
+typedef struct oaph_network oaph_network;
+
+/* api NULL: the network owns a stream of its own */
+oaph_status oaph_network_create(oaph_network** out, oaph_api* api);
+oaph_status oaph_network_destroy(oaph_network* n);
+/* Network::createLayer(neurons, addBias, activation); connects the previous layer */
+oaph_status oaph_network_add_layer(oaph_network* n, uint32_t neurons, int has_bias,
+                                   int activation);
+oaph_status oaph_network_set_learning_rate(oaph_network* n, double lr);
+/* weights layer -> layer+1: (neurons+bias of layer) columns x (neurons of
+ * layer+1) rows, element (c, r) at c + r * columns */
+oaph_status oaph_network_set_host_weights(oaph_network* n, uint32_t layer, const double* w);
+oaph_status oaph_network_get_host_weights(oaph_network* n, uint32_t layer, double* w);
+oaph_status oaph_network_set_inputs(oaph_network* n, const double* x);
+oaph_status oaph_network_set_expected(oaph_network* n, const double* t);
+oaph_status oaph_network_forward(oaph_network* n);
+oaph_status oaph_network_get_outputs(oaph_network* n, double* y);
+oaph_status oaph_network_accumulate_errors(oaph_network* n);
+oaph_status oaph_network_backward(oaph_network* n);
+/* W <- W - (lr / N) sum of nabla W (N accumulated errors), then post_step */
+oaph_status oaph_network_update_weights(oaph_network* n);
+oaph_status oaph_network_post_step(oaph_network* n);
+/* mean of 0.5 e^2 over the accumulated errors */
+oaph_status oaph_network_calculate_error(oaph_network* n, double* out);
+
+/* ------------------------------------------------------------------ */
+/* Learned-start MLP: tanh hidden layers, linear output, BF16 operands with
+ * FP32 accumulation on the matrix cores, FP32 master weights, Adam.  The
+ * batched form of oap's neural routines (oapNeuralHost/oapGenericNeuralApi.h
+ * :195-354).  Loss of a batch: 1/(2B) sum_b sum_j w_j (y_bj - t_bj)^2.
+ * Widths d_l <= 512.  Parameters flat: per layer W (d_{l+1} x d_l,
+ * row-major), then b.  Host arrays are row-major (n x d).                 */
+/* ------------------------------------------------------------------ */
+typedef struct oaph_mlp oaph_mlp;
+
+oaph_status oaph_mlp_create(oaph_mlp** out, const uint32_t* dims, uint32_t ndims,
+                            uint32_t max_batch);
+oaph_status oaph_mlp_destroy(oaph_mlp* m);
+oaph_status oaph_mlp_num_params(oaph_mlp* m, size_t* n);
+/* sets the parameters and resets the optimiser state */
+oaph_status oaph_mlp_set_params(oaph_mlp* m, const float* params);
+oaph_status oaph_mlp_get_params(oaph_mlp* m, float* params);
+/* per-output loss weights w_j (default 1) */
+oaph_status oaph_mlp_set_output_weights(oaph_mlp* m, const float* w);
+oaph_status oaph_mlp_predict(oaph_mlp* m, const float* X, uint32_t n, float* Y);
+/* loss and flat gradients of one batch (B <= max_batch) */
+oaph_status oaph_mlp_gradients(oaph_mlp* m, const float* X, const float* T, uint32_t B,
+                               float* grads, float* loss);
+/* training set kept in HBM for oaph_mlp_train */
+oaph_status oaph_mlp_set_dataset(oaph_mlp* m, const float* X, const float* T, uint32_t n);
+/* `epochs` shuffled passes of Adam (lr, betas 0.9 / 0.999) in batches of B;
+ * losses (may be NULL) receives the mean batch loss of each epoch */
+oaph_status oaph_mlp_train(oaph_mlp* m, uint32_t B, uint32_t epochs, float lr, uint64_t seed,
+                           float* losses);
+
+/* ------------------------------------------------------------------ */
 /* Batched covariance eigenproblems.  Problem i is C_i = X_i^T X_i / s with
  * X_i an s x n_i FP32 row-major sample matrix: the M M^T operator the
  * reference forms with RecToSquareApi / SquareMatrixApi

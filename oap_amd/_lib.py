@@ -131,6 +131,37 @@ _SIGS = {
     "oaph_arnoldi_restart_profile": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "oaph_synth_matrix": (C.c_int, [MP, C.c_int, C.c_uint64, C.c_double, C.c_double]),
     "oaph_time_dot_product": (C.c_int, [C.c_void_p, MP, MP, MP, U32, DP]),
+    "oaph_activation": (C.c_int, [C.c_void_p, C.c_int, MP, MP]),
+    "oaph_activation_dims": (C.c_int, [C.c_void_p, C.c_int, MP, MP, C.POINTER(U32)]),
+    "oaph_activation_dims_periodic": (C.c_int, [C.c_void_p, C.c_int, MP, MP, C.POINTER(U32)]),
+    "oaph_network_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p]),
+    "oaph_network_destroy": (C.c_int, [C.c_void_p]),
+    "oaph_network_add_layer": (C.c_int, [C.c_void_p, U32, C.c_int, C.c_int]),
+    "oaph_network_set_learning_rate": (C.c_int, [C.c_void_p, C.c_double]),
+    "oaph_network_set_host_weights": (C.c_int, [C.c_void_p, U32, DP]),
+    "oaph_network_get_host_weights": (C.c_int, [C.c_void_p, U32, DP]),
+    "oaph_network_set_inputs": (C.c_int, [C.c_void_p, DP]),
+    "oaph_network_set_expected": (C.c_int, [C.c_void_p, DP]),
+    "oaph_network_forward": (C.c_int, [C.c_void_p]),
+    "oaph_network_get_outputs": (C.c_int, [C.c_void_p, DP]),
+    "oaph_network_accumulate_errors": (C.c_int, [C.c_void_p]),
+    "oaph_network_backward": (C.c_int, [C.c_void_p]),
+    "oaph_network_update_weights": (C.c_int, [C.c_void_p]),
+    "oaph_network_post_step": (C.c_int, [C.c_void_p]),
+    "oaph_network_calculate_error": (C.c_int, [C.c_void_p, DP]),
+    "oaph_mlp_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(U32), U32, U32]),
+    "oaph_mlp_destroy": (C.c_int, [C.c_void_p]),
+    "oaph_mlp_num_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_size_t)]),
+    "oaph_mlp_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "oaph_mlp_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "oaph_mlp_set_output_weights": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "oaph_mlp_predict": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), U32, C.POINTER(C.c_float)]),
+    "oaph_mlp_gradients": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float), U32,
+                                     C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+    "oaph_mlp_set_dataset": (C.c_int, [C.c_void_p, C.POINTER(C.c_float), C.POINTER(C.c_float),
+                                       U32]),
+    "oaph_mlp_train": (C.c_int, [C.c_void_p, U32, U32, C.c_float, C.c_uint64,
+                                 C.POINTER(C.c_float)]),
     "oaph_batch_c3_sizes": (C.c_int, [U32, C.c_uint64, C.POINTER(U32)]),
     "oaph_batch_create": (C.c_int, [C.POINTER(C.c_void_p), U32, C.POINTER(U32), U32]),
     "oaph_batch_destroy": (C.c_int, [C.c_void_p]),

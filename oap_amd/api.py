@@ -254,6 +254,49 @@ class CuProceduresApi:
               "calcTriangularH")
 
 
+class Activation:
+    """OAPH_FN_* codes (CuProceduresApi.h:239-290); the layer activations of
+    oap's Network are SIGMOID, LINEAR, TANH and SIN."""
+    SIGMOID, DSIGMOID, MULTIPLY_DSIGMOID = 0, 1, 2
+    LINEAR, DLINEAR = 3, 4
+    TANH, DTANH, MULTIPLY_DTANH = 5, 6, 7
+    SIN, DSIN, MULTIPLY_DSIN = 8, 9, 10
+
+
+def _activation(api, fn, out, m, dims):
+    lib = api._lib
+    if dims is None:
+        check(lib.oaph_activation(api._h, fn, _m(out), _m(m)), "activation")
+        return
+    d = np.asarray(dims, dtype=np.uint32)
+    if d.shape == (2,):
+        arr = (C.c_uint32 * 2)(*d)
+        check(lib.oaph_activation_dims(api._h, fn, _m(out), _m(m), arr), "activation_dims")
+    elif d.shape == (2, 2):
+        arr = (C.c_uint32 * 4)(*d.ravel())
+        check(lib.oaph_activation_dims_periodic(api._h, fn, _m(out), _m(m), arr),
+              "activation_dims_periodic")
+    else:
+        raise ValueError("dims must be [2] or [2][2]")
+
+
+def _add_activation_methods():
+    names = {"sigmoid": Activation.SIGMOID, "dsigmoid": Activation.DSIGMOID,
+             "multiplyDSigmoid": Activation.MULTIPLY_DSIGMOID, "linear": Activation.LINEAR,
+             "dlinear": Activation.DLINEAR, "tanh": Activation.TANH, "dtanh": Activation.DTANH,
+             "multiplyDTanh": Activation.MULTIPLY_DTANH, "sin": Activation.SIN,
+             "dsin": Activation.DSIN, "cos": Activation.DSIN,
+             "multiplyDSin": Activation.MULTIPLY_DSIN}
+    for name, fn in names.items():
+        def method(self, out, m, dims=None, _fn=fn):
+            _activation(self, _fn, out, m, dims)
+        method.__name__ = name
+        setattr(CuProceduresApi, name, method)
+
+
+_add_activation_methods()
+
+
 def set_threads_blocks(w: int, h: int, limit: int = 1024):
     """utils::mapper::SetThreadsBlocks (oapUtils/ThreadsMapper.cpp:36-74) -> (blocks, threads)."""
     lib = load()
@@ -613,3 +656,182 @@ class CovarianceBatch:
 
 
 __all__ += ["CovarianceBatch"]
+
+
+class Network:
+    """oap::Network (oapNeuralDevice/oapNetwork.h:33-120) in FP64 on the GPU:
+    per-sample forwardPropagation / accumulateErrors(MEAN_SQUARE_ERROR) /
+    backPropagation and updateWeights after a batch."""
+
+    def __init__(self, api: "CuProceduresApi | None" = None):
+        self._lib = load()
+        h = C.c_void_p()
+        check(self._lib.oaph_network_create(C.byref(h), api._h if api else None), "network_create")
+        self._h = h
+        self._layers = []
+
+    def createLayer(self, neurons: int, addBias: bool = False, activation: int = Activation.SIGMOID):
+        check(self._lib.oaph_network_add_layer(self._h, neurons, int(addBias), activation),
+              "network_add_layer")
+        self._layers.append((neurons, bool(addBias), activation))
+
+    def getLayersCount(self) -> int:
+        return len(self._layers)
+
+    def weightsShape(self, layer: int):
+        n, b, _ = self._layers[layer]
+        return (self._layers[layer + 1][0], n + int(b))
+
+    def setLearningRate(self, lr: float):
+        check(self._lib.oaph_network_set_learning_rate(self._h, lr), "set_learning_rate")
+
+    def setHostWeights(self, w, layer: int):
+        w = np.ascontiguousarray(w, dtype=np.float64).ravel()
+        if w.size != int(np.prod(self.weightsShape(layer))):
+            raise ValueError("weights size")
+        check(self._lib.oaph_network_set_host_weights(self._h, layer, w.ctypes.data_as(DP)),
+              "set_host_weights")
+
+    def getHostWeights(self, layer: int) -> np.ndarray:
+        w = np.zeros(int(np.prod(self.weightsShape(layer))))
+        check(self._lib.oaph_network_get_host_weights(self._h, layer, w.ctypes.data_as(DP)),
+              "get_host_weights")
+        return w
+
+    def setInputs(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        check(self._lib.oaph_network_set_inputs(self._h, x.ctypes.data_as(DP)), "set_inputs")
+
+    def setExpected(self, t):
+        t = np.ascontiguousarray(np.atleast_1d(t), dtype=np.float64)
+        check(self._lib.oaph_network_set_expected(self._h, t.ctypes.data_as(DP)), "set_expected")
+
+    def forwardPropagation(self):
+        check(self._lib.oaph_network_forward(self._h), "forward")
+
+    def getOutputs(self) -> np.ndarray:
+        y = np.zeros(self._layers[-1][0])
+        check(self._lib.oaph_network_get_outputs(self._h, y.ctypes.data_as(DP)), "get_outputs")
+        return y
+
+    def accumulateErrors(self):
+        check(self._lib.oaph_network_accumulate_errors(self._h), "accumulate_errors")
+
+    def backPropagation(self):
+        check(self._lib.oaph_network_backward(self._h), "backward")
+
+    def updateWeights(self):
+        check(self._lib.oaph_network_update_weights(self._h), "update_weights")
+
+    def postStep(self):
+        check(self._lib.oaph_network_post_step(self._h), "post_step")
+
+    def calculateError(self) -> float:
+        v = C.c_double()
+        check(self._lib.oaph_network_calculate_error(self._h, C.byref(v)), "calculate_error")
+        return v.value
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            self._lib.oaph_network_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+__all__ += ["Activation", "Network"]
+
+
+class MLP:
+    """The learned-start network on the GPU (oaph_mlp_*): tanh hidden layers,
+    linear output, BF16 operands / FP32 accumulation, FP32 master weights,
+    Adam."""
+
+    def __init__(self, dims, max_batch: int = 512):
+        self._lib = load()
+        self.dims = [int(d) for d in dims]
+        arr = (C.c_uint32 * len(self.dims))(*self.dims)
+        h = C.c_void_p()
+        check(self._lib.oaph_mlp_create(C.byref(h), arr, len(self.dims), max_batch), "mlp_create")
+        self._h = h
+        self.max_batch = max_batch
+        n = C.c_size_t()
+        check(self._lib.oaph_mlp_num_params(self._h, C.byref(n)), "mlp_num_params")
+        self.num_params = n.value
+
+    @staticmethod
+    def _f(a):
+        return np.ascontiguousarray(a, dtype=np.float32)
+
+    @staticmethod
+    def _fp(a):
+        return a.ctypes.data_as(C.POINTER(C.c_float))
+
+    def init_params(self, seed: int = 0) -> np.ndarray:
+        """Glorot-uniform weights, zero biases."""
+        rng = np.random.default_rng(seed)
+        parts = []
+        for din, dout in zip(self.dims[:-1], self.dims[1:]):
+            lim = np.sqrt(6.0 / (din + dout))
+            parts += [rng.uniform(-lim, lim, dout * din), np.zeros(dout)]
+        p = self._f(np.concatenate(parts))
+        self.set_params(p)
+        return p
+
+    def set_params(self, p):
+        p = self._f(p)
+        if p.size != self.num_params:
+            raise ValueError("parameter count")
+        check(self._lib.oaph_mlp_set_params(self._h, self._fp(p)), "mlp_set_params")
+
+    def get_params(self) -> np.ndarray:
+        p = np.zeros(self.num_params, dtype=np.float32)
+        check(self._lib.oaph_mlp_get_params(self._h, self._fp(p)), "mlp_get_params")
+        return p
+
+    def set_output_weights(self, w):
+        w = self._f(w)
+        check(self._lib.oaph_mlp_set_output_weights(self._h, self._fp(w)), "mlp_set_output_weights")
+
+    def predict(self, X) -> np.ndarray:
+        X = self._f(X)
+        Y = np.zeros((X.shape[0], self.dims[-1]), dtype=np.float32)
+        check(self._lib.oaph_mlp_predict(self._h, self._fp(X), X.shape[0], self._fp(Y)), "mlp_predict")
+        return Y
+
+    def gradients(self, X, T):
+        X, T = self._f(X), self._f(T)
+        g = np.zeros(self.num_params, dtype=np.float32)
+        loss = C.c_float()
+        check(self._lib.oaph_mlp_gradients(self._h, self._fp(X), self._fp(T), X.shape[0],
+                                           self._fp(g), C.byref(loss)), "mlp_gradients")
+        return float(loss.value), g
+
+    def set_dataset(self, X, T):
+        X, T = self._f(X), self._f(T)
+        check(self._lib.oaph_mlp_set_dataset(self._h, self._fp(X), self._fp(T), X.shape[0]),
+              "mlp_set_dataset")
+
+    def train(self, batch: int, epochs: int, lr: float, seed: int = 1) -> np.ndarray:
+        losses = np.zeros(epochs, dtype=np.float32)
+        check(self._lib.oaph_mlp_train(self._h, batch, epochs, lr, seed, self._fp(losses)),
+              "mlp_train")
+        return losses
+
+    def destroy(self):
+        if getattr(self, "_h", None):
+            self._lib.oaph_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
+__all__ += ["MLP"]

@@ -988,3 +988,243 @@ int orc_gen_c3_problem(int i, uint64_t seed, int s, float* X) {
   free(b);
   return n;
 }
+
+/* ------------------------------------------------------------------ */
+/* Feed-forward network with oap's training semantics, FP64, one sample
+ * at a time: forwardPropagation (oapNeuralHost/oapGenericNeuralApi.h
+ * :195-216), accumulateErrors with MEAN_SQUARE_ERROR
+ * (oapNeuralDevice/oapNetwork.cpp:335-352), backPropagation
+ * (oapGenericNeuralApi.h:264-334), updateWeights (:336-354) and postStep
+ * (oapNetwork.cpp:553-567).  Activation codes as OAPH_FN_* in
+ * include/oaph.h: 0 sigmoid, 3 linear, 5 tanh, 8 sin.                  */
+/* ------------------------------------------------------------------ */
+#define ORC_MAX_LAYERS 16
+struct orc_net {
+  int L;
+  int n[ORC_MAX_LAYERS], b[ORC_MAX_LAYERS], act[ORC_MAX_LAYERS];
+  double* in[ORC_MAX_LAYERS];   /* total_l: activations + bias 1 */
+  double* sum[ORC_MAX_LAYERS];  /* n_l */
+  double* err[ORC_MAX_LAYERS];  /* n_l */
+  double* w[ORC_MAX_LAYERS];    /* layer l -> l+1: n_{l+1} rows x total_l cols */
+  double* gw[ORC_MAX_LAYERS];   /* accumulated nabla */
+  double* e_out;                /* y - t of the last accumulate */
+  double errsum;
+  int errcount;
+};
+
+static double orc_act(int fn, double v) {
+  switch (fn) {
+    case 0: return 1.0 / (1.0 + exp(-v));
+    case 3: return v;
+    case 8: return sin(v);
+    default: return tanh(v);
+  }
+}
+static double orc_dact(int fn, double v) {
+  switch (fn) {
+    case 0: { double s = 1.0 / (1.0 + exp(-v)); return s * (1.0 - s); }
+    case 3: return 1.0;
+    case 8: return cos(v);
+    default: { double t = tanh(v); return 1.0 - t * t; }
+  }
+}
+
+orc_net* orc_net_create(int nlayers, const int* neurons, const int* bias, const int* act) {
+  if (nlayers < 2 || nlayers > ORC_MAX_LAYERS) return NULL;
+  orc_net* N = (orc_net*)calloc(1, sizeof(orc_net));
+  N->L = nlayers;
+  for (int l = 0; l < nlayers; ++l) {
+    N->n[l] = neurons[l];
+    N->b[l] = bias[l] ? 1 : 0;
+    N->act[l] = act[l];
+    int t = N->n[l] + N->b[l];
+    N->in[l] = (double*)calloc((size_t)t, sizeof(double));
+    if (N->b[l]) N->in[l][t - 1] = 1.0;
+    N->sum[l] = (double*)calloc((size_t)t, sizeof(double));
+    N->err[l] = (double*)calloc((size_t)t, sizeof(double));
+  }
+  for (int l = 0; l + 1 < nlayers; ++l) {
+    size_t sz = (size_t)N->n[l + 1] * (N->n[l] + N->b[l]);
+    N->w[l] = (double*)calloc(sz, sizeof(double));
+    N->gw[l] = (double*)calloc(sz, sizeof(double));
+  }
+  N->e_out = (double*)calloc((size_t)N->n[nlayers - 1] + N->b[nlayers - 1], sizeof(double));
+  return N;
+}
+
+void orc_net_free(orc_net* N) {
+  if (!N) return;
+  for (int l = 0; l < N->L; ++l) { free(N->in[l]); free(N->sum[l]); free(N->err[l]); free(N->w[l]); free(N->gw[l]); }
+  free(N->e_out);
+  free(N);
+}
+
+static size_t orc_wsize(const orc_net* N, int l) { return (size_t)N->n[l + 1] * (N->n[l] + N->b[l]); }
+void orc_net_set_weights(orc_net* N, int l, const double* w) { memcpy(N->w[l], w, orc_wsize(N, l) * sizeof(double)); }
+void orc_net_get_weights(const orc_net* N, int l, double* w) { memcpy(w, N->w[l], orc_wsize(N, l) * sizeof(double)); }
+
+void orc_net_forward(orc_net* N, const double* x, double* y) {
+  memcpy(N->in[0], x, (size_t)N->n[0] * sizeof(double));
+  for (int l = 1; l < N->L; ++l) {
+    int tp = N->n[l - 1] + N->b[l - 1];
+    for (int r = 0; r < N->n[l]; ++r) {
+      double s = 0;
+      for (int c = 0; c < tp; ++c) s += N->w[l - 1][(size_t)c + (size_t)tp * r] * N->in[l - 1][c];
+      N->sum[l][r] = s;
+      N->in[l][r] = orc_act(N->act[l], s);
+    }
+  }
+  if (y) memcpy(y, N->in[N->L - 1], (size_t)N->n[N->L - 1] * sizeof(double));
+}
+
+void orc_net_accumulate(orc_net* N, const double* t) {
+  int L = N->L - 1, tot = N->n[L] + N->b[L];
+  for (int i = 0; i < tot; ++i) {
+    double e = N->in[L][i] - (i < N->n[L] ? t[i] : 0.0);
+    N->e_out[i] = e;
+    N->errsum += 0.5 * e * e;
+    N->errcount++;
+  }
+}
+
+void orc_net_backward(orc_net* N) {
+  int L = N->L - 1;
+  for (int i = 0; i < N->n[L]; ++i) N->err[L][i] = N->e_out[i] * orc_dact(N->act[L], N->sum[L][i]);
+  for (int l = L - 1; l >= 1; --l) {
+    int tl = N->n[l] + N->b[l];
+    for (int c = 0; c < N->n[l]; ++c) {
+      double s = 0;
+      for (int r = 0; r < N->n[l + 1]; ++r) s += N->w[l][(size_t)c + (size_t)tl * r] * N->err[l + 1][r];
+      N->err[l][c] = s * orc_dact(N->act[l], N->sum[l][c]);
+    }
+  }
+  for (int l = 0; l + 1 < N->L; ++l) {
+    int tl = N->n[l] + N->b[l];
+    for (int r = 0; r < N->n[l + 1]; ++r)
+      for (int c = 0; c < tl; ++c) N->gw[l][(size_t)c + (size_t)tl * r] += N->in[l][c] * N->err[l + 1][r];
+  }
+}
+
+void orc_net_post_step(orc_net* N) {
+  for (int l = 0; l + 1 < N->L; ++l) memset(N->gw[l], 0, orc_wsize(N, l) * sizeof(double));
+  N->errsum = 0;
+  N->errcount = 0;
+}
+
+void orc_net_update(orc_net* N, double lr) {
+  if (N->errcount == 0) return;
+  double f = lr / (double)N->errcount;
+  for (int l = 0; l + 1 < N->L; ++l)
+    for (size_t i = 0; i < orc_wsize(N, l); ++i) N->w[l][i] -= f * N->gw[l][i];
+  orc_net_post_step(N);
+}
+
+double orc_net_error(const orc_net* N) { return N->errcount ? N->errsum / N->errcount : 0.0; }
+
+void orc_net_gradient(const orc_net* N, int l, double* g) { memcpy(g, N->gw[l], orc_wsize(N, l) * sizeof(double)); }
+
+/* ------------------------------------------------------------------ */
+/* Learned-start MLP (tanh hidden layers, linear output): loss and
+ * gradients of one batch, the arithmetic of the device network in
+ * oap_amd/csrc/mlp.hip restated in double precision.  With round_bf16 the
+ * operands of every product are rounded to bfloat16 (nearest-even, from the
+ * float value) at the same points as on the device; without it the plain
+ * FP64 function, whose gradients the finite-difference test checks.
+ * Parameters flat: per layer W (d_{l+1} x d_l row-major) then b.          */
+/* ------------------------------------------------------------------ */
+double orc_bf16(double v) {
+  float f = (float)v;
+  uint32_t u;
+  memcpy(&u, &f, 4);
+  if ((u & 0x7f800000u) != 0x7f800000u) u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  memcpy(&f, &u, 4);
+  return (double)f;
+}
+
+double orc_mlp_grad(const uint32_t* dims, int ndims, const float* params, const float* X,
+                    const float* T, const float* w, int B, double* grads, int round_bf16) {
+  int L = ndims - 1;
+  size_t woff[16], boff[16], off = 0;
+  for (int l = 0; l < L; ++l) {
+    woff[l] = off; off += (size_t)dims[l + 1] * dims[l];
+    boff[l] = off; off += dims[l + 1];
+  }
+  double (*R)(double) = round_bf16 ? orc_bf16 : NULL;
+#define RND(v) (R ? R(v) : (v))
+  /* activations per layer: H[l] (B x d_l), values as stored (fp32 in the
+   * device network, rounded to float here when emulating) */
+  double* H[16];
+  double* Hr[16];  /* operand copies (bf16-rounded when emulating) */
+  for (int l = 0; l <= L; ++l) {
+    H[l] = (double*)calloc((size_t)B * dims[l], sizeof(double));
+    Hr[l] = (double*)calloc((size_t)B * dims[l], sizeof(double));
+  }
+  for (int b = 0; b < B; ++b)
+    for (uint32_t k = 0; k < dims[0]; ++k) {
+      H[0][(size_t)b * dims[0] + k] = X[(size_t)b * dims[0] + k];
+      Hr[0][(size_t)b * dims[0] + k] = RND((double)X[(size_t)b * dims[0] + k]);
+    }
+  for (int l = 0; l < L; ++l) {
+    int din = dims[l], dout = dims[l + 1];
+    for (int b = 0; b < B; ++b)
+      for (int o = 0; o < dout; ++o) {
+        double z = 0;
+        for (int k = 0; k < din; ++k)
+          z += Hr[l][(size_t)b * din + k] * RND((double)params[woff[l] + (size_t)o * din + k]);
+        z += params[boff[l] + o];
+        double h = (l + 1 < L) ? tanh(z) : z;
+        if (R) h = (double)(float)h;
+        H[l + 1][(size_t)b * dout + o] = h;
+        Hr[l + 1][(size_t)b * dout + o] = RND(h);
+      }
+  }
+  /* output delta */
+  int dL = dims[L];
+  double loss = 0;
+  double* dz = (double*)calloc((size_t)B * 512, sizeof(double));
+  double* dzr = (double*)calloc((size_t)B * 512, sizeof(double));
+  double* dz2 = (double*)calloc((size_t)B * 512, sizeof(double));
+  for (int b = 0; b < B; ++b)
+    for (int o = 0; o < dL; ++o) {
+      double d = H[L][(size_t)b * dL + o] - T[(size_t)b * dL + o];
+      double g = w[o] * d / B;
+      if (R) g = (double)(float)g;
+      dz[(size_t)b * dL + o] = g;
+      dzr[(size_t)b * dL + o] = RND(g);
+      loss += 0.5 * w[o] * d * d / B;
+    }
+  for (int l = L; l >= 1; --l) {
+    int dout = dims[l], din = dims[l - 1];
+    for (int o = 0; o < dout; ++o) {
+      for (int k = 0; k < din; ++k) {
+        double s = 0;
+        for (int b = 0; b < B; ++b) s += dzr[(size_t)b * dout + o] * Hr[l - 1][(size_t)b * din + k];
+        grads[woff[l - 1] + (size_t)o * din + k] = s;
+      }
+      double s = 0;
+      for (int b = 0; b < B; ++b) s += dz[(size_t)b * dout + o];
+      grads[boff[l - 1] + o] = s;
+    }
+    if (l == 1) break;
+    for (int b = 0; b < B; ++b)
+      for (int k = 0; k < din; ++k) {
+        double s = 0;
+        for (int o = 0; o < dout; ++o)
+          s += dzr[(size_t)b * dout + o] * RND((double)params[woff[l - 1] + (size_t)o * din + k]);
+        double h = H[l - 1][(size_t)b * din + k];
+        double g = s * (1.0 - h * h);
+        if (R) g = (double)(float)g;
+        dz2[(size_t)b * din + k] = g;
+      }
+    for (int b = 0; b < B; ++b)
+      for (int k = 0; k < din; ++k) {
+        dz[(size_t)b * din + k] = dz2[(size_t)b * din + k];
+        dzr[(size_t)b * din + k] = RND(dz2[(size_t)b * din + k]);
+      }
+  }
+#undef RND
+  for (int l = 0; l <= L; ++l) { free(H[l]); free(Hr[l]); }
+  free(dz); free(dzr); free(dz2);
+  return loss;
+}

@@ -141,6 +141,26 @@ void orc_gen_c1(int n, uint64_t seed, double eps, double decay, double* are, dou
 int orc_gen_c3_problem(int i, uint64_t seed, int s, float* X);
 uint64_t orc_c3_stream(uint64_t seed, int i);
 
+/* Feed-forward network, oap training semantics in FP64 (oracle.c) */
+typedef struct orc_net orc_net;
+orc_net* orc_net_create(int nlayers, const int* neurons, const int* bias, const int* act);
+void orc_net_free(orc_net* N);
+void orc_net_set_weights(orc_net* N, int layer, const double* w);
+void orc_net_get_weights(const orc_net* N, int layer, double* w);
+void orc_net_forward(orc_net* N, const double* x, double* y);
+void orc_net_accumulate(orc_net* N, const double* t);
+void orc_net_backward(orc_net* N);
+void orc_net_update(orc_net* N, double lr);
+void orc_net_post_step(orc_net* N);
+double orc_net_error(const orc_net* N);
+void orc_net_gradient(const orc_net* N, int layer, double* g);
+
+/* Learned-start MLP: batch loss and flat gradients (oracle.c); round_bf16
+ * emulates the device network's bfloat16 operand rounding */
+double orc_bf16(double v);
+double orc_mlp_grad(const uint32_t* dims, int ndims, const float* params, const float* X,
+                    const float* T, const float* w, int B, double* grads, int round_bf16);
+
 #ifdef __cplusplus
 }
 #endif

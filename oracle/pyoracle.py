@@ -125,6 +125,25 @@ class Oracle:
         L.orc_gen_c1.argtypes = [C.c_int, C.c_uint64, C.c_double, C.c_double, DP, DP]
         L.orc_gen_c3_problem.argtypes = [C.c_int, C.c_uint64, C.c_int, C.POINTER(C.c_float)]
         L.orc_gen_c3_problem.restype = C.c_int
+        IP = C.POINTER(C.c_int)
+        L.orc_net_create.argtypes = [C.c_int, IP, IP, IP]
+        L.orc_net_create.restype = C.c_void_p
+        L.orc_net_free.argtypes = [C.c_void_p]
+        for nm in ("orc_net_set_weights", "orc_net_get_weights", "orc_net_gradient"):
+            getattr(L, nm).argtypes = [C.c_void_p, C.c_int, DP]
+        L.orc_net_forward.argtypes = [C.c_void_p, DP, DP]
+        L.orc_net_accumulate.argtypes = [C.c_void_p, DP]
+        L.orc_net_backward.argtypes = [C.c_void_p]
+        L.orc_net_update.argtypes = [C.c_void_p, C.c_double]
+        L.orc_net_post_step.argtypes = [C.c_void_p]
+        L.orc_net_error.argtypes = [C.c_void_p]
+        L.orc_net_error.restype = C.c_double
+        FP = C.POINTER(C.c_float)
+        L.orc_bf16.argtypes = [C.c_double]
+        L.orc_bf16.restype = C.c_double
+        L.orc_mlp_grad.argtypes = [C.POINTER(C.c_uint32), C.c_int, FP, FP, FP, FP, C.c_int, DP,
+                                   C.c_int]
+        L.orc_mlp_grad.restype = C.c_double
 
     # ---- procedures (return the output Planes)
     def dot_product(self, out: Planes, a: Planes, b: Planes, dims=None) -> int:
@@ -324,3 +343,75 @@ class Ref:
 
     def is_upper_triangular(self, a) -> bool:
         return bool(self.lib.ref_is_upper_triangular(*self._args(a)))
+
+
+def mlp_grad(oracle, dims, params, X, T, w, round_bf16=True):
+    """(loss, flat gradients) of the learned-start MLP on one batch."""
+    f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    params, X, T, w = f(params), f(X), f(T), f(w)
+    d = (C.c_uint32 * len(dims))(*dims)
+    g = np.zeros(params.size)
+    FP = C.POINTER(C.c_float)
+    loss = oracle.lib.orc_mlp_grad(d, len(dims), params.ctypes.data_as(FP), X.ctypes.data_as(FP),
+                                   T.ctypes.data_as(FP), w.ctypes.data_as(FP), X.shape[0], _p(g),
+                                   int(round_bf16))
+    return loss, g
+
+
+class OracleNet:
+    """The oracle's FP64 network (orc_net_*): layers (neurons, bias, activation)."""
+
+    def __init__(self, oracle: "Oracle", layers):
+        self.lib = oracle.lib
+        self.shape = [(int(n), bool(b), int(a)) for n, b, a in layers]
+        arr = lambda v: (C.c_int * len(v))(*v)
+        self.h = self.lib.orc_net_create(len(layers), arr([x[0] for x in self.shape]),
+                                         arr([int(x[1]) for x in self.shape]),
+                                         arr([x[2] for x in self.shape]))
+
+    def wshape(self, l):
+        n, b, _ = self.shape[l]
+        return (self.shape[l + 1][0], n + int(b))
+
+    def set_weights(self, l, w):
+        w = np.ascontiguousarray(w, dtype=np.float64).ravel()
+        assert w.size == np.prod(self.wshape(l))
+        self.lib.orc_net_set_weights(self.h, l, _p(w))
+
+    def get_weights(self, l):
+        w = np.zeros(self.wshape(l))
+        self.lib.orc_net_get_weights(self.h, l, _p(w))
+        return w.ravel()
+
+    def gradient(self, l):
+        g = np.zeros(self.wshape(l))
+        self.lib.orc_net_gradient(self.h, l, _p(g))
+        return g.ravel()
+
+    def forward(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.shape[-1][0])
+        self.lib.orc_net_forward(self.h, _p(x), _p(y))
+        return y
+
+    def accumulate(self, t):
+        t = np.ascontiguousarray(np.atleast_1d(t), dtype=np.float64)
+        self.lib.orc_net_accumulate(self.h, _p(t))
+
+    def backward(self):
+        self.lib.orc_net_backward(self.h)
+
+    def update(self, lr):
+        self.lib.orc_net_update(self.h, lr)
+
+    def post_step(self):
+        self.lib.orc_net_post_step(self.h)
+
+    def error(self):
+        return self.lib.orc_net_error(self.h)
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.lib.orc_net_free(self.h)
+        except Exception:
+            pass
