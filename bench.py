@@ -16,6 +16,9 @@ bulk, FP64 orthogonalisation, QR, shifts, polish and residual check) over the
 K timed steps; the plain-FP64 solver on the same start vectors is reported
 beside it (`plain_fp64`).  `batched_c3` is the 512-problem covariance batch
 (n_i in [64, 1024], s = 256, k = 4, m = 16) solved in one persistent launch.
+`learned_c4` compares the random start with the trained network's start
+vector on held-out convection-diffusion-reaction operators (n = 4096, 6
+smallest-real eigenpairs).
 `roofline` is for the dominant kernel of the reported solve (the FP32-operand
 matvec), timed with HIP events on the solver's stream; `cpu_baseline` runs the
 oracle's FP64 IRAM (threaded dense matvec) for one Arnoldi cycle on the host
@@ -157,6 +160,34 @@ def batched_c3(steps: int) -> dict:
                                  "(restarts included) in the kernel time"}}
 
 
+def learned_c4(count: int) -> dict:
+    """C4: convection-diffusion-reaction operators (64 x 64 grid, n = 4096,
+    FP32 coefficients), 6 smallest-real eigenpairs, m = 24; each operator
+    solved from the random start and from the committed trained network's
+    start vector (random-start fallback if that solve does not converge)."""
+    from oap_amd import learned
+    model = learned.LearnedStart.load()
+    params = learned.sample_params(count, 0x5EED0004 + 1)
+    learned.compare(model, params[:1])  # warm-up
+    res = learned.compare(model, params)
+    out = {}
+    for key, idx in (("random_start", 0), ("learned_start", 1)):
+        sols = [r[idx] for r in res]
+        secs = sum(x.seconds for x in sols)
+        good = sum(x.converged for x in sols)
+        out[key] = {"value": round(good / secs, 2), "unit": "eigenpairs/s",
+                    "matvecs_mean": round(float(np.mean([x.matvecs for x in sols])), 1),
+                    "ms_per_solve": round(secs / len(sols) * 1e3, 2)}
+    out["fallbacks"] = int(sum(r[1].fallback for r in res))
+    out["operators"] = count
+    out["ratio_learned_over_random"] = round(out["learned_start"]["value"] /
+                                             out["random_start"]["value"], 3)
+    pred = np.array([r[1].predicted for r in res])
+    got = np.array([np.sort(r[1].evals.real) for r in res])
+    out["eigenvalue_prediction_rel_err_median"] = round(float(np.median(np.abs(pred - got) / got)), 4)
+    return out
+
+
 def profiled_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
     summary (FETCH_SIZE + WRITE_SIZE, calibrated against a copy kernel)."""
@@ -199,6 +230,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--traffic-probe", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -243,6 +275,10 @@ def main() -> int:
     batched = None
     if not args.no_batched:
         batched = batched_c3(args.steps)
+    c4 = None
+    if not args.no_learned:
+        c4 = learned_c4(max(8, args.steps * 2))
+        log(f"learned C4: {c4}")
     cpu = None
     if not args.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
@@ -272,6 +308,7 @@ def main() -> int:
                        "median_per_run": round(med64, 4), "matvecs_median": matvecs64,
                        "ratio_mixed_over_fp64": round(epsmx / eps64, 3) if eps64 > 0 else None},
         "batched_c3": batched,
+        "learned_c4": c4,
         "roofline": {"bound": "hbm", "achieved": round(gbs32, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs32 / HBM_PEAK_GBS, 4),
                      "traffic": profiled_traffic("k_cgemv32"),

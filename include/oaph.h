@@ -295,6 +295,12 @@ oaph_status oaph_arnoldi_restart_profile(oaph_arnoldi* a, long long* clk6);
  *   kind 0: A = diag(1/(1+i)) + p1 G / sqrt(n), real (im plane zeroed if present)
  *   kind 1: A = diag(exp(-k/p2) e^{2 pi i u_k}) + p1 (G_re + i G_im) / sqrt(n) */
 oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1, double p2);
+/* Convection-diffusion-reaction operator -(alpha u_xx + u_yy) + pe (u_x + u_y)
+ * + c u on the grid x grid interior of the unit square (h = 1/(grid+1),
+ * Dirichlet, central differences), unknown (ix, iy) at ix + grid * iy,
+ * written dense into the re plane (im zeroed if present) with every
+ * coefficient rounded to FP32.  A must be grid^2 x grid^2. */
+oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, double alpha);
 /* Average duration of `iters` back-to-back dotProduct calls (HIP events on
  * the api's stream), after one warm-up call. */
 oaph_status oaph_time_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,

@@ -51,6 +51,33 @@ __global__ void k_synth(int kind, uint32_t n, uint64_t seed, double p1, double p
   }
 }
 
+// C4 operator: -(alpha d_xx + d_yy) u + pe (d_x + d_y) u + c u on the N x N
+// interior grid of the unit square (h = 1/(N+1), Dirichlet), second-order
+// central differences, unknown (ix, iy) at ix + N iy, assembled dense with
+// every coefficient rounded to FP32 (the operator is the FP32 matrix).
+__global__ void k_cdr(uint32_t N, double pe, double c, double alpha, double* re, double* im) {
+#pragma clang fp contract(off)
+  const uint32_t n = N * N;
+  const double h = 1.0 / (double)(N + 1), ih2 = 1.0 / (h * h), adv = pe / (2.0 * h);
+  const double diag = (double)(float)(2.0 * alpha * ih2 + 2.0 * ih2 + c);
+  const double xm = (double)(float)(-alpha * ih2 - adv), xp = (double)(float)(-alpha * ih2 + adv);
+  const double ym = (double)(float)(-ih2 - adv), yp = (double)(float)(-ih2 + adv);
+  const size_t total = (size_t)n * n;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < total;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t r = (uint32_t)(i / n), col = (uint32_t)(i - (size_t)r * n);
+    const uint32_t rx = r % N;
+    double v = 0.0;
+    if (col == r) v = diag;
+    else if (col + 1 == r && rx > 0) v = xm;
+    else if (col == r + 1 && rx + 1 < N) v = xp;
+    else if (col + N == r) v = ym;
+    else if (col == r + N) v = yp;
+    re[i] = v;
+    if (im) im[i] = 0.0;
+  }
+}
+
 }  // namespace
 }  // namespace oaph
 
@@ -68,6 +95,20 @@ oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1
   if (blocks > 65536) blocks = 65536;
   hipLaunchKernelGGL(k_synth, dim3((unsigned)blocks), dim3(256), 0, 0, kind, A->columns, seed, p1,
                      p2, A->reValues, A->imValues);
+  OAPH_HIP(hipGetLastError());
+  OAPH_HIP(hipDeviceSynchronize());
+  return OAPH_OK;
+}
+
+oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, double alpha) {
+  OAPH_TRY(ensure_init());
+  OAPH_CHECK(A && A->reValues && grid >= 2 && alpha > 0, OAPH_ERR_ARG);
+  OAPH_CHECK(A->columns == grid * grid && A->rows == grid * grid, OAPH_ERR_SHAPE);
+  const size_t total = (size_t)A->columns * A->rows;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_cdr, dim3((unsigned)blocks), dim3(256), 0, 0, grid, pe, c, alpha,
+                     A->reValues, A->imValues);
   OAPH_HIP(hipGetLastError());
   OAPH_HIP(hipDeviceSynchronize());
   return OAPH_OK;

@@ -1228,3 +1228,23 @@ double orc_mlp_grad(const uint32_t* dims, int ndims, const float* params, const 
   free(dz); free(dzr); free(dz2);
   return loss;
 }
+
+/* C4 operator (the generator of oaph_synth_cdr): -(alpha u_xx + u_yy) +
+ * pe (u_x + u_y) + c u, central differences on the N x N interior grid,
+ * coefficients rounded to FP32, row-major dense n x n (n = N^2). */
+void orc_gen_cdr(int N, double pe, double c, double alpha, double* A) {
+  int n = N * N;
+  double h = 1.0 / (double)(N + 1), ih2 = 1.0 / (h * h), adv = pe / (2.0 * h);
+  double diag = (double)(float)(2.0 * alpha * ih2 + 2.0 * ih2 + c);
+  double xm = (double)(float)(-alpha * ih2 - adv), xp = (double)(float)(-alpha * ih2 + adv);
+  double ym = (double)(float)(-ih2 - adv), yp = (double)(float)(-ih2 + adv);
+  memset(A, 0, sizeof(double) * (size_t)n * n);
+  for (int r = 0; r < n; ++r) {
+    int rx = r % N;
+    A[(size_t)r * n + r] = diag;
+    if (rx > 0) A[(size_t)r * n + r - 1] = xm;
+    if (rx + 1 < N) A[(size_t)r * n + r + 1] = xp;
+    if (r >= N) A[(size_t)r * n + r - N] = ym;
+    if (r + N < n) A[(size_t)r * n + r + N] = yp;
+  }
+}

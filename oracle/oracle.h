@@ -155,6 +155,9 @@ void orc_net_post_step(orc_net* N);
 double orc_net_error(const orc_net* N);
 void orc_net_gradient(const orc_net* N, int layer, double* g);
 
+/* C4 convection-diffusion-reaction operator, dense row-major n x n (n = N^2) */
+void orc_gen_cdr(int N, double pe, double c, double alpha, double* A);
+
 /* Learned-start MLP: batch loss and flat gradients (oracle.c); round_bf16
  * emulates the device network's bfloat16 operand rounding */
 double orc_bf16(double v);

@@ -138,6 +138,7 @@ class Oracle:
         L.orc_net_post_step.argtypes = [C.c_void_p]
         L.orc_net_error.argtypes = [C.c_void_p]
         L.orc_net_error.restype = C.c_double
+        L.orc_gen_cdr.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, DP]
         FP = C.POINTER(C.c_float)
         L.orc_bf16.argtypes = [C.c_double]
         L.orc_bf16.restype = C.c_double
@@ -249,6 +250,12 @@ class Oracle:
     def gen_c0(self, n=64, seed=0x5EED0000):
         a = np.zeros((n, n))
         self.lib.orc_gen_c0(n, seed, _p(a))
+        return a
+
+    def gen_cdr(self, N, pe, c, alpha):
+        """C4 operator (N^2 x N^2 dense, row-major)."""
+        a = np.zeros((N * N, N * N))
+        self.lib.orc_gen_cdr(N, pe, c, alpha, _p(a))
         return a
 
     def gen_c3(self, i, seed=0x5EED0003, s=256):

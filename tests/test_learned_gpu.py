@@ -1,0 +1,83 @@
+"""Learned start on the C4 operator family (oap_amd.learned): the device
+generator against the oracle's, the solver against the closed-form spectrum
+of a symmetric member, and what the trained network gains over the random
+start (solves judged by their explicit FP64 residuals)."""
+import numpy as np
+import pytest
+
+from oap_amd import learned
+from oap_amd.api import DeviceMatrix
+
+pytestmark = pytest.mark.gpu
+
+
+def test_generator_matches_oracle(oracle):
+    from oap_amd._lib import check, load
+    N = 16
+    A = DeviceMatrix(N * N, N * N, True, True)
+    check(load().oaph_synth_cdr(A.ptr, N, 33.0, 4.0, 0.8), "synth_cdr")
+    re, im = A.download()
+    want = oracle.gen_cdr(N, 33.0, 4.0, 0.8)
+    assert np.array_equal(re, want)
+    assert not np.any(im)
+
+
+def test_symmetric_member_closed_form():
+    # pe = 0: symmetric; eigenvalues c + 2 alpha/h^2 (1 - cos j pi h) + 2/h^2 (1 - cos k pi h)
+    prob = learned.CdrProblem()
+    pe, c, a = 0.0, 3.0, 0.8  # alpha != 1: no double eigenvalues
+    prob.set_params(pe, c, a)
+    sol = prob.solve(None, seed=3)
+    h = 1.0 / (learned.GRID + 1)
+    j = np.arange(1, learned.GRID + 1)
+    e1 = 2 / h**2 * (1 - np.cos(j * np.pi * h))
+    want = np.sort((a * e1[:, None] + e1[None, :]).ravel() + c)[:learned.K]
+    assert sol.converged == learned.K
+    got = np.sort(sol.evals.real)
+    # FP32 coefficients: relative perturbation ~6e-8 of the operator
+    assert np.max(np.abs(got - want)) <= 1e-6 * want[-1]
+    assert np.all(sol.resid <= learned.TOL * prob.fro)
+
+
+def test_start_vector_helpers():
+    v = np.arange(learned.N, dtype=np.float64)
+    c = learned.block_average(v)
+    assert c.shape == (learned.NCOEF,)
+    e = learned.expand(c)
+    assert np.allclose(learned.block_average(e), c)
+    x = np.exp(1j * 0.7) * np.random.default_rng(0).standard_normal(learned.N)
+    t = learned.start_target(np.array([x]), 20.0, 1.0)
+    t0 = learned.start_target(np.array([x * np.exp(-1j * 0.7)]), 20.0, 1.0)
+    assert np.allclose(t, t0)
+    # the start vector of the target coefficients reproduces their block shape
+    v = learned.start_from(t, 20.0, 1.0)
+    back = learned.block_average(v * np.exp(-learned.log_profile(20.0, 1.0)))
+    assert np.allclose(back / np.linalg.norm(back), t)
+
+
+def test_pipeline_small_and_fallback():
+    params = learned.sample_params(24, 11)
+    P, T, mv = learned.generate(params)
+    assert len(P) >= 20 and T.shape[1] == learned.NCOEF + learned.K
+    model = learned.LearnedStart()
+    losses = model.fit(P, T, epochs=60, batch=8)
+    assert losses[-1] < losses[0]
+    prob = learned.CdrProblem()
+    pe, c, a = params[0]
+    prob.set_params(pe, c, a)
+    # a start vector orthogonal to nothing useful still ends converged (the
+    # fallback takes over when the first solve does not converge)
+    sol = prob.solve(np.ones(learned.N), seed=5)
+    assert sol.converged == learned.K
+
+
+def test_trained_model_gain():
+    if not learned.DEFAULT_MODEL.exists():
+        pytest.skip("trained model file not present")
+    model = learned.LearnedStart.load()
+    res = learned.compare(model, learned.sample_params(8, 0x5EED0004 + 1))
+    plain = np.array([p.matvecs for p, _ in res])
+    lrn = np.array([l.matvecs for _, l in res])
+    for p, l in res:
+        assert p.converged == learned.K and l.converged == learned.K
+    assert lrn.mean() <= 0.85 * plain.mean(), (plain, lrn)
