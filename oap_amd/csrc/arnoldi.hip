@@ -7,7 +7,7 @@
 // residuals; see the oracle's orc_iram for the CPU restatement):
 //   v0 -> [ w = A v_j (callback or dense operator);  CGS2 against V[:, 0..j];
 //           H[:, j] = h;  beta = ||f||;  v_{j+1} = f / beta ]  j = start..m-1
-//   -> one-wave restart kernel: Schur form of H, Ritz estimates
+//   -> restart kernel (one workgroup): Schur form of H, Ritz estimates
 //      beta |e_m^T y|, wanted order, exact shifts -> (H+, Q)
 //   -> stop when the k wanted estimates are <= tol * norm_a
 //   -> V <- V Q[:, 0..kk), f <- V Q[:, kk] H+[kk][kk-1] + f Q[m-1][kk-1]

@@ -91,32 +91,164 @@ __device__ void rot_cols(Mat M, int j, int r0, int r1, double c, cd s) {
   }
 }
 
-// one implicit single-shift QR step with shift mu on the leading kdim block of
-// T, accumulating Q on the right.  The block is split at negligible
-// subdiagonals and the bulge chased inside each unreduced block (a sweep
-// started at the top of a reducible T stays the identity).  Same steps as
-// shifted_sweep in oracle/oracle.c.
-__device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu) {
-  for (int kk = lo; kk < hi; ++kk) {
-    cd f, g;
-    if (kk == lo) {
-      f = csub(T.get(lo, lo), mu);
-      g = T.get(lo + 1, lo);
-    } else {
-      f = T.get(kk, kk - 1);
-      g = T.get(kk + 1, kk - 1);
+// Rotation log of a sweep (rotation kk acts on rows/columns kk, kk+1).
+struct RotLog {
+  double* c;
+  cd* s;
+};
+
+// Single-wave ordering point: LDS operations of one wave complete in issue
+// order; the fences keep the compiler from moving them across this point.
+__device__ __forceinline__ void wsync() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ double lane_read(double v, int l) {
+  const long long bits = __builtin_bit_cast(long long, v);
+  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffll), l);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __builtin_bit_cast(double, ((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ cd lane_read(cd v, int l) { return {lane_read(v.re, l), lane_read(v.im, l)}; }
+
+// row pair (a above b) <- G (a; b), G = [[c, s], [-conj s, c]]  (rot_rows)
+__device__ __forceinline__ cd grow_top(cd a, cd b, double c, cd s) {
+  return {c * a.re + (s.re * b.re - s.im * b.im), c * a.im + (s.re * b.im + s.im * b.re)};
+}
+__device__ __forceinline__ cd grow_bot(cd a, cd b, double c, cd s) {
+  return {-(s.re * a.re + s.im * a.im) + c * b.re, -(s.re * a.im - s.im * a.re) + c * b.im};
+}
+// column pair (a left of b) <- (a, b) G^H  (rot_cols)
+__device__ __forceinline__ cd gcol_left(cd a, cd b, double c, cd s) {
+  return {a.re * c + (b.re * s.re + b.im * s.im), a.im * c + (b.im * s.re - b.re * s.im)};
+}
+__device__ __forceinline__ cd gcol_right(cd a, cd b, double c, cd s) {
+  return {-(a.re * s.re - a.im * s.im) + b.re * c, -(a.re * s.im + a.im * s.re) + b.im * c};
+}
+
+constexpr int kDefer = 8;  // rotations per deferred batch
+
+// The column halves of rotations [k1, k2) that no later rotation of the
+// sweep reads: rows r < kk of T (row r receives rotations max(k1, r+1) ..
+// k2-1 in order) and every row of Q.  One row per worker, carried in
+// registers across the batch.
+__device__ void chase_deferred(Mat T, Mat Q, int kdim, int k1, int k2, const double* lc,
+                               const cd* ls, int worker, int nworkers) {
+  const int nt = k2 - 1 > 0 ? k2 - 1 : 0;  // T rows 0 .. k2-2
+  for (int task = worker; task < nt + kdim; task += nworkers) {
+    const bool isq = task >= nt;
+    Mat M = isq ? Q : T;
+    const int r = isq ? task - nt : task;
+    const int kb = isq ? k1 : (r + 1 > k1 ? r + 1 : k1);
+    if (kb >= k2) continue;
+    cd a = M.get(r, kb);
+    for (int kk = kb; kk < k2; ++kk) {
+      const cd b = M.get(r, kk + 1);
+      const double c = lc[kk];
+      const cd s = ls[kk];
+      M.set(r, kk, gcol_left(a, b, c, s));
+      a = gcol_right(a, b, c, s);
     }
-    double c;
-    cd s;
-    cgivens(f, g, c, s);
-    bsync();
-    rot_rows(T, kk, kk > lo ? kk - 1 : kk, kdim - 1, c, s);
-    bsync();
-    if (kk > lo && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
-    int r1 = kk + 2 < hi ? kk + 2 : hi;
-    rot_cols(T, kk, 0, r1, c, s);
-    rot_cols(Q, kk, 0, kdim - 1, c, s);
-    bsync();
+    M.set(r, k2, a);
+  }
+}
+
+// One implicit single-shift QR step with shift mu on rows/columns [lo, hi] of
+// T (kdim <= 64), accumulating Q on the right: rotation kk (lo <= kk < hi) is
+// computed from (T[kk][kk-1], T[kk+1][kk-1]) (from (T[lo][lo] - mu,
+// T[lo+1][lo]) for kk = lo), applied to rows kk, kk+1 over columns >= kk-1
+// (>= kk at lo), to columns kk, kk+1 over rows <= min(kk+2, hi) of T and all
+// rows of Q, and the bulge entry T[kk+1][kk-1] is zeroed.  Same steps as
+// shifted_sweep in oracle/oracle.c.
+//
+// Critical path in registers: wave 0 holds rows kk and kk+1 with lane j =
+// column j, prefetches row kk+2 from LDS, rotates the row pair in place,
+// applies the column pair to rows kk .. kk+2 with lane reads of the four or
+// six entries involved, hands the next (f, g) on in registers and writes row
+// kk back once it is final.  The rest of each column rotation (rows < kk of
+// T, all of Q) is applied by the other waves in batches of kDefer rotations,
+// one batch behind (disjoint rows); a one-wave block applies each batch
+// itself.  The block is split at negligible subdiagonals by the caller
+// (shifted_sweep): a sweep started at the top of a reducible T stays the
+// identity.
+__device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu, RotLog log) {
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const bool solo = blockDim.x <= 64;
+  const int j = lane;
+  cd a = {0.0, 0.0}, b = {0.0, 0.0}, f = {0.0, 0.0}, g = {0.0, 0.0};
+  if (wave == 0) {
+    if (j < kdim) {
+      a = T.get(lo, j);
+      b = T.get(lo + 1, j);
+    }
+    f = csub(lane_read(a, lo), mu);
+    g = lane_read(b, lo);
+  }
+  for (int kb = lo;; kb += kDefer) {
+    const bool crit = kb < hi;
+    const int ke = crit ? (kb + kDefer < hi ? kb + kDefer : hi) : hi;
+    if (wave == 0 && crit) {
+      int j0 = lo;
+      for (int kk = kb; kk < ke; ++kk) {
+        const bool have2 = kk + 2 <= hi;
+        cd nxt = {0.0, 0.0};
+        if (have2 && j < kdim && j >= kk) nxt = T.get(kk + 2, j);
+        double c;
+        cd s;
+        cgivens(f, g, c, s);
+        if (lane == 0) {
+          log.c[kk] = c;
+          log.s[kk] = s;
+        }
+        j0 = kk > lo ? kk - 1 : kk;
+        cd na = a, nb = b;
+        if (j >= j0 && j < kdim) {
+          na = grow_top(a, b, c, s);
+          nb = grow_bot(a, b, c, s);
+        }
+        if (kk > lo && j == kk - 1) nb = cd{0.0, 0.0};
+        const cd t00 = lane_read(na, kk), t01 = lane_read(na, kk + 1);
+        const cd t10 = lane_read(nb, kk), t11 = lane_read(nb, kk + 1);
+        cd u20 = {0.0, 0.0}, u21 = {0.0, 0.0};
+        if (have2) {
+          const cd t21 = lane_read(nxt, kk + 1);
+          u20 = gcol_left(cd{0.0, 0.0}, t21, c, s);
+          u21 = gcol_right(cd{0.0, 0.0}, t21, c, s);
+        }
+        if (j == kk) {
+          na = gcol_left(t00, t01, c, s);
+          nb = gcol_left(t10, t11, c, s);
+          nxt = u20;
+        } else if (j == kk + 1) {
+          na = gcol_right(t00, t01, c, s);
+          nb = gcol_right(t10, t11, c, s);
+          nxt = u21;
+        }
+        // row kk is final for the critical path
+        if (j >= j0 && j < kdim) T.set(kk, j, na);
+        if (kk > lo && j == kk - 1) T.set(kk + 1, kk - 1, cd{0.0, 0.0});
+        f = gcol_left(t10, t11, c, s);
+        g = u20;
+        a = nb;
+        b = nxt;
+      }
+      if (ke == hi && j >= j0 && j < kdim) T.set(hi, j, a);  // the last row of the block
+    }
+    if (solo) {
+      if (crit) {
+        wsync();
+        chase_deferred(T, Q, kdim, kb, ke, log.c, log.s, lane, 64);
+        wsync();
+      }
+    } else {
+      if (wave >= 1 && kb > lo)
+        chase_deferred(T, Q, kdim, kb - kDefer, kb < hi ? kb : hi, log.c, log.s, tid - 64,
+                       (int)blockDim.x - 64);
+      __syncthreads();
+    }
+    if (!crit) break;
   }
 }
 
@@ -136,7 +268,7 @@ __device__ __forceinline__ unsigned long long negligible_mask(Mat T, int kdim) {
   return __ballot(neg);
 }
 
-__device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu) {
+__device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu, RotLog log) {
   const unsigned long long mask = negligible_mask(T, kdim);
   bsync();
   {
@@ -149,12 +281,12 @@ __device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu) {
     // the block [lo, hi] ends before the next negligible subdiagonal above lo
     const unsigned long long above = lo >= 63 ? 0ull : mask & ~((2ull << lo) - 1ull);
     const int hi = above ? __builtin_ctzll(above) - 1 : kdim - 1;
-    if (hi > lo) chase(T, Q, kdim, lo, hi, mu);
+    if (hi > lo) chase(T, Q, kdim, lo, hi, mu, log);
     lo = hi + 1;
   }
 }
 
-__device__ int schur(Mat T, Mat Z, int mm) {
+__device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
   int ihi = mm - 1, iter = 0, total = 0;
   while (ihi >= 0) {
     // largest l <= ihi with a negligible subdiagonal (0 if none)
@@ -186,27 +318,7 @@ __device__ int schur(Mat T, Mat Z, int mm) {
       mu = cabs2(t1) < cabs2(t2) ? cadd(d, t1) : cadd(d, t2);
     }
     ++iter;
-    for (int kk = l; kk < ihi; ++kk) {
-      cd f, g;
-      if (kk == l) {
-        f = csub(T.get(l, l), mu);
-        g = T.get(l + 1, l);
-      } else {
-        f = T.get(kk, kk - 1);
-        g = T.get(kk + 1, kk - 1);
-      }
-      double c;
-      cd s;
-      cgivens(f, g, c, s);
-      bsync();
-      rot_rows(T, kk, kk > l ? kk - 1 : kk, mm - 1, c, s);
-      bsync();
-      if (kk > l && threadIdx.x == 0) T.set(kk + 1, kk - 1, {0, 0});
-      int r1 = kk + 2 < ihi ? kk + 2 : ihi;
-      rot_cols(T, kk, 0, r1, c, s);
-      rot_cols(Z, kk, 0, mm - 1, c, s);
-      bsync();
-    }
+    chase(T, Z, mm, l, ihi, mu, log);
   }
   return 0;
 }
@@ -225,7 +337,7 @@ __device__ bool ritz_before(cd a, cd b, int which) {
   }
 }
 
-// The restart of one problem by one workgroup (any block size; the one-wave
+// The restart of one problem by one workgroup (any block size; the
 // k_restart and the batched solver both call it).  smem: T and Z, mact x
 // mact complex each with the odd leading dimension lds_ld(mact), so that a
 // column walk (one row per lane) spreads over the LDS banks
@@ -255,7 +367,10 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
     Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
   }
   __syncthreads();
-  int fail = schur(T, Z, mm);
+  __shared__ double log_c[kMaxM];
+  __shared__ cd log_s[kMaxM];
+  const RotLog log{log_c, log_s};
+  int fail = schur(T, Z, mm, log);
   __syncthreads();
   if (lane == 0) st->clk[1] = clock64();
   for (int i = lane; i < mm; i += nthr) ev[i] = T.get(i, i);
@@ -379,7 +494,7 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
     Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
   }
   __syncthreads();
-  for (int t = kk; t < mm; ++t) shifted_sweep(T, Z, mm, ev[order[t]]);
+  for (int t = kk; t < mm; ++t) shifted_sweep(T, Z, mm, ev[order[t]], log);
   __syncthreads();
   if (lane == 0) st->clk[5] = clock64();
   for (int idx = lane; idx < mm * mm; idx += nthr) {

@@ -1,0 +1,78 @@
+"""Every launch shape of the streaming matvecs (rows per wave x loads in
+flight) computes the product to FP64 accuracy (the shapes differ only in how
+a lane's partial sums are grouped): dotProduct through each FP64 shape
+against the exact product within gamma_n sum |a||x|, and the mixed solver
+through each FP32-operand shape against the dense eigenvalues."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from oap_amd._lib import check, load
+from oap_amd.api import CuProceduresApi, DeviceMatrix
+
+pytestmark = pytest.mark.gpu
+
+
+def shapes(fp32):
+    lib, out, v = load(), [], 0
+    while True:
+        r, u = C.c_int(), C.c_int()
+        if lib.oaph_matvec_shape_info(fp32, v, C.byref(r), C.byref(u)) != 0:
+            return out
+        out.append((r.value, u.value))
+        v += 1
+
+
+@pytest.mark.parametrize("n,cplx", [(1000, True), (1024, False), (4098, True)])
+def test_fp64_shapes_bit_identical(n, cplx):
+    lib = load()
+    rng = np.random.default_rng(n)
+    A = DeviceMatrix.from_numpy(rng.standard_normal((n, n)), rng.standard_normal((n, n)) if cplx else None)
+    x = DeviceMatrix.from_numpy(rng.standard_normal((n, 1)), rng.standard_normal((n, 1)))
+    api = CuProceduresApi()
+    outs = []
+    try:
+        for v, sh in enumerate(shapes(0)):
+            check(lib.oaph_set_matvec_shape(0, v), "shape")
+            y = DeviceMatrix(1, n, True, True)
+            api.dotProduct(y, A, x)
+            outs.append(y.to_complex())
+    finally:
+        check(lib.oaph_set_matvec_shape(0, 0), "shape")
+    Ah = A.to_complex()
+    xh = x.to_complex()
+    want = Ah @ xh
+    bound = 4 * n * 1.1e-16 * (np.abs(Ah) @ np.abs(xh))
+    for o in outs:
+        assert np.all(np.abs(o - want) <= bound)
+
+
+def test_fp32_shapes_bit_identical():
+    lib = load()
+    from oap_amd.api import CuHArnoldiCallback, Precision
+    n = 2048
+    A = DeviceMatrix(n, n)
+    check(lib.oaph_synth_matrix(A.ptr, 1, 7, 1e-2, 500.0), "synth")
+    s = CuHArnoldiCallback()
+    s.setMatrix(A)
+    s.setPrecision(Precision.MIXED_FP32)
+    s.setSortType(0)
+    ev = []
+    try:
+        for v, sh in enumerate(shapes(1)):
+            check(lib.oaph_set_matvec_shape(1, v), "shape")
+            s.setSeed(3)
+            re, im = np.zeros(4), np.zeros(4)
+            s.setOutputsEigenvalues(re, im)
+            from oap_amd.api import MatrixInfo
+            s.execute(32, 4, MatrixInfo(True, True, n, n))
+            st = s.stats()
+            ev.append((re + 1j * im, st["matvecs"], st["matvecs_low"]))
+    finally:
+        check(lib.oaph_set_matvec_shape(1, 0), "shape")
+    Ad = A.to_complex()
+    want = sorted(np.linalg.eigvals(Ad), key=lambda z: -abs(z))[:4]
+    for e, _, _ in ev:
+        for w in want:
+            assert min(abs(g - w) for g in e) <= 1e-10 * abs(want[0])
