@@ -450,6 +450,10 @@ oaph_status oaph_batch_autotune(oaph_batch* b, uint32_t hdim, uint32_t wanted, i
  * the last solve.  Any pointer may be NULL. */
 oaph_status oaph_batch_results(oaph_batch* b, double* evals, double* resid, uint32_t* nconv,
                                uint32_t* restarts, uint32_t* matvecs, double* kernel_ms);
+/* Per problem (count x 5): shader-clock cycles of the last solve spent in the
+ * operator, the orthogonalisation, the restarts, the basis update and the
+ * final Ritz vectors (profiling). */
+oaph_status oaph_batch_phase_cycles(oaph_batch* b, long long* cycles);
 /* Ritz vectors of problem i: wanted x n_i doubles, vector t at x + t * n_i. */
 oaph_status oaph_batch_eigenvectors(oaph_batch* b, uint32_t i, double* x);
 

@@ -179,6 +179,7 @@ _SIGS = {
                                       U32, C.c_char_p, C.POINTER(C.c_int), DP]),
     "oaph_batch_results": (C.c_int, [C.c_void_p, DP, DP, C.POINTER(U32), C.POINTER(U32),
                                      C.POINTER(U32), DP]),
+    "oaph_batch_phase_cycles": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "oaph_batch_eigenvectors": (C.c_int, [C.c_void_p, U32, DP]),
 }
 

@@ -637,6 +637,14 @@ class CovarianceBatch:
         return {"evals": ev, "resid": rs, "nconv": nconv, "restarts": rst, "matvecs": mv,
                 "kernel_ms": ms.value}
 
+    def phase_cycles(self) -> np.ndarray:
+        """count x 5 shader-clock cycles: operator, orthogonalisation,
+        restarts, basis update, Ritz vectors."""
+        c = np.zeros((len(self.n), 5), dtype=np.int64)
+        check(self._lib.oaph_batch_phase_cycles(self._h, c.ctypes.data_as(C.POINTER(C.c_longlong))),
+              "batch_phase_cycles")
+        return c
+
     def eigenvectors(self, i: int) -> np.ndarray:
         x = np.zeros((self.k, int(self.n[i])))
         check(self._lib.oaph_batch_eigenvectors(self._h, i, x.ctypes.data_as(C.POINTER(C.c_double))),

@@ -10,10 +10,11 @@
 // accumulates u_r x_r into its share of w -- so a matvec reads X once
 // (s ldx 4 bytes) and never forms C.
 //
-// One workgroup per problem runs the whole implicitly restarted Arnoldi
-// solve (Lanczos with full CGS2 reorthogonalisation: C is symmetric, so H is
-// kept symmetric tridiagonal), restarts with the shared small-dense code
-// (small_dense.h), computes the Ritz vectors and the explicit FP64
+// One workgroup per problem runs the whole implicitly restarted Lanczos
+// solve (the symmetric case of the Arnoldi iteration: full CGS2
+// reorthogonalisation, a real symmetric tridiagonal projection T), restarts
+// with exact shifts by implicit QR sweeps on T's diagonal and off-diagonal
+// (sym_restart), and computes the Ritz vectors and the explicit FP64
 // residual ||C x - lambda x|| of every wanted pair.  Problems are dispatched
 // largest n first so that the long ones start early.
 #include <algorithm>
@@ -22,14 +23,11 @@
 #include <numeric>
 #include <vector>
 
-#include "iram.h"
-#include "small_dense.h"
+#include "internal.h"
 
 namespace oaph {
 namespace batched {
 
-using iram::RestartArgs;
-using iram::Status;
 
 constexpr int kMaxMB = 24;   // Krylov dimension limit of the batched solver
 constexpr int kMaxN = 1024;  // largest problem dimension
@@ -39,7 +37,6 @@ struct Problem {
   uint32_t n, ldx;
   uint64_t x_off;  // floats
   uint64_t v_off;  // doubles: (m+1) columns of n
-  uint64_t h_off;  // doubles: H (2 m^2) | Q (2 m^2) | Xw (2 m^2) | Y (2 m k)
   uint64_t out_off;  // doubles: Ritz vectors k x n
 };
 
@@ -50,6 +47,7 @@ struct Result {
   int restarts;
   int matvecs;
   int pad;
+  long long cycles[5];  // shader clock per phase: matvec, orthogonalisation, restart, V Q, Ritz vectors
 };
 
 namespace {
@@ -100,21 +98,25 @@ __device__ void block_sums(const double (&v)[NV], int nv, double* red, double* o
 }
 
 // w = X^T (X v) / s in one pass over X.  vl (LDS) holds v; the result goes to
-// wl (LDS).  Each wave takes ROWS consecutive rows per step, the waves
-// interleaved; a row's float4 chunks are
-// spread over the lanes, u_r = x_r . v is a cross-lane sum, and u_r x_r
-// accumulates in the lanes' registers.  The waves' partial sums are then
-// added into wl in wave order (deterministic, no LDS partial buffer).
-template <int NT, int ROWS>
-__device__ void covariance_matvec(const float* __restrict__ X, uint32_t n, uint32_t ldx,
-                                  uint32_t s, const double* vl, double* wl) {
-  constexpr int NW = NT / 64, Q4 = Ctx<NT>::Q4;
+// wl (LDS).  A row's float4 chunks are spread over SEG lanes (SEG = 16, 32 or
+// 64, the smallest that covers the row), so a wave loads 64/SEG rows per
+// instruction and ROWS such groups per step, the waves interleaved;
+// u_r = x_r . v is a segmented cross-lane sum and u_r x_r accumulates in the
+// lanes' registers.  Lanes holding the same columns are summed across the
+// segments, then the waves' partial sums are added into wl in wave order
+// (deterministic, no LDS partial buffer).
+template <int NT, int ROWS, int SEG>
+__device__ void covariance_matvec_seg(const float* __restrict__ X, uint32_t n, uint32_t ldx,
+                                      uint32_t s, const double* vl, double* wl) {
+  constexpr int NW = NT / 64, RPL = 64 / SEG;
+  constexpr int Q4 = SEG == 64 ? kMaxN / 4 / 64 : 1;  // float4 chunks per lane per row
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = lane / SEG, q0 = lane % SEG;
   const uint32_t nq = ldx / 4;  // float4 chunks per row (padding is zero)
   double vreg[Q4][4], acc[Q4][4];
 #pragma unroll
   for (int t = 0; t < Q4; ++t) {
-    const uint32_t q = lane + 64 * t;
+    const uint32_t q = q0 + SEG * t;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
       const uint32_t c = 4 * q + e;
@@ -123,18 +125,21 @@ __device__ void covariance_matvec(const float* __restrict__ X, uint32_t n, uint3
     }
   }
   const f4* X4 = reinterpret_cast<const f4*>(X);
-  for (uint32_t r0 = ROWS * wave; r0 < s; r0 += ROWS * NW) {
-    f4 x[ROWS][Q4];
+  // rows r0 + rr*RPL + sub of one step; the next step's rows are loaded
+  // before this step's arithmetic (two steps of loads in flight)
+  auto load = [&](f4 (&x)[ROWS][Q4], uint32_t r0) {
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
-      const bool ok = r0 + rr < s;
+      const uint32_t r = r0 + rr * RPL + sub;
 #pragma unroll
       for (int t = 0; t < Q4; ++t) {
-        const uint32_t q = lane + 64 * t;
+        const uint32_t q = q0 + SEG * t;
         const f4 z = {0.f, 0.f, 0.f, 0.f};
-        x[rr][t] = (q < nq && ok) ? X4[(size_t)(r0 + rr) * nq + q] : z;
+        x[rr][t] = (q < nq && r < s) ? X4[(size_t)r * nq + q] : z;
       }
     }
+  };
+  auto step = [&](const f4 (&x)[ROWS][Q4]) {
     double u[ROWS];
 #pragma unroll
     for (int rr = 0; rr < ROWS; ++rr) {
@@ -147,7 +152,10 @@ __device__ void covariance_matvec(const float* __restrict__ X, uint32_t n, uint3
       u[rr] = d;
     }
 #pragma unroll
-    for (int rr = 0; rr < ROWS; ++rr) u[rr] = wave_sum(u[rr]);
+    for (int rr = 0; rr < ROWS; ++rr) {
+#pragma unroll
+      for (int off = SEG / 2; off > 0; off >>= 1) u[rr] += __shfl_xor(u[rr], off, 64);
+    }
 #pragma unroll
     for (int t = 0; t < Q4; ++t) {
 #pragma unroll
@@ -156,13 +164,35 @@ __device__ void covariance_matvec(const float* __restrict__ X, uint32_t n, uint3
         for (int rr = 0; rr < ROWS; ++rr) acc[t][e] += u[rr] * (double)x[rr][t][e];
       }
     }
+  };
+  const uint32_t stride = NW * RPL * ROWS;
+  f4 xa[ROWS][Q4], xb[ROWS][Q4];
+  uint32_t r0 = wave * RPL * ROWS;
+  if (r0 < s) load(xa, r0);
+  while (r0 < s) {
+    if (r0 + stride < s) load(xb, r0 + stride);
+    step(xa);
+    r0 += stride;
+    if (r0 >= s) break;
+    if (r0 + stride < s) load(xa, r0 + stride);
+    step(xb);
+    r0 += stride;
+  }
+  // lanes with the same columns: sum across the row segments
+  if (SEG < 64) {
+#pragma unroll
+    for (int t = 0; t < Q4; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+#pragma unroll
+        for (int off = SEG; off < 64; off <<= 1) acc[t][e] += __shfl_xor(acc[t][e], off, 64);
   }
   const double inv_s = 1.0 / (double)s;
   for (int w = 0; w < NW; ++w) {
-    if (wave == w) {
+    if (wave == w && sub == 0) {
 #pragma unroll
       for (int t = 0; t < Q4; ++t) {
-        const uint32_t q = lane + 64 * t;
+        const uint32_t q = q0 + SEG * t;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const uint32_t c = 4 * q + e;
@@ -177,14 +207,21 @@ __device__ void covariance_matvec(const float* __restrict__ X, uint32_t n, uint3
   }
 }
 
+template <int NT, int ROWS>
+__device__ void covariance_matvec(const float* __restrict__ X, uint32_t n, uint32_t ldx,
+                                  uint32_t s, const double* vl, double* wl) {
+  const uint32_t nq = ldx / 4;
+  if (nq <= 16) covariance_matvec_seg<NT, ROWS, 16>(X, n, ldx, s, vl, wl);
+  else if (nq <= 32) covariance_matvec_seg<NT, ROWS, 32>(X, n, ldx, s, vl, wl);
+  else covariance_matvec_seg<NT, ROWS, 64>(X, n, ldx, s, vl, wl);
+}
+
 struct SolveArgs {
   const Problem* probs;
   const uint32_t* order;  // dispatch order (largest n first)
-  Status* status;
   Result* results;
   const float* X;
   double* V;
-  double* H;
   double* out;
   uint32_t s, m, k;
   int which;
@@ -193,30 +230,225 @@ struct SolveArgs {
   uint64_t seed;
 };
 
+// ---------------------------------------------------------------------------
+// Restart of the symmetric problem: the Lanczos matrix T is real symmetric
+// tridiagonal (diagonal d, off-diagonal e, e[j] = T[j+1][j]), so its Schur
+// form is an eigendecomposition computed by implicit QR sweeps with
+// Wilkinson shifts on (d, e) alone, and the exact shifts of the restart are
+// the same sweeps with the unwanted Ritz values (ARPACK's dseigt / dsapps
+// steps).  The scalar chain runs in wave 0 (every lane computes the same
+// values; lane 0 stores them); the eigenvector / shift accumulation is
+// row-parallel, lane r owning row r.
+// ---------------------------------------------------------------------------
+constexpr double kEps = 2.220446049250313e-16;
+
+__device__ __forceinline__ bool negligible(double e, double a, double b) {
+  return fabs(e) <= kEps * (fabs(a) + fabs(b)) || fabs(e) < 1e-300;
+}
+
+// One implicit symmetric QR sweep with shift mu on rows/columns [l, h] of
+// (d, e), accumulating M <- M R^T on rows 0 .. nrows-1 (ld ldm).  Rotation k
+// R = [[c, s], [-s, c]] on rows/columns k, k+1 zeroes the bulge (from
+// (d[l] - mu, e[l]) at k = l); T <- R T R^T.
+__device__ void sym_sweep(double* d, double* e, int l, int h, double mu, double* M, int ldm,
+                          int nrows) {
+  const int lane = threadIdx.x & 63;
+  double a = d[l], b = e[l], dd = d[l + 1];
+  double en = l + 1 < h ? e[l + 1] : 0.0;
+  double x = a - mu, z = b;
+  double mk = lane < nrows ? M[lane * ldm + l] : 0.0;
+  double mk1 = lane < nrows ? M[lane * ldm + l + 1] : 0.0;
+  for (int k = l; k < h; ++k) {
+    // the next step's operands, loaded ahead
+    const double d2 = k + 2 <= h ? d[k + 2] : 0.0;
+    const double e2 = k + 2 < h ? e[k + 2] : 0.0;
+    const double mk2 = (lane < nrows && k + 2 <= h) ? M[lane * ldm + k + 2] : 0.0;
+    double c = 1.0, s = 0.0, r = x;
+    const double q2 = x * x + z * z;
+    if (z != 0.0) {
+      const double inv = rsqrt(q2);
+      c = x * inv;
+      s = z * inv;
+      r = q2 * inv;
+    }
+    const double c2 = c * c, s2 = s * s, cs = c * s;
+    const double na = c2 * a + 2.0 * cs * b + s2 * dd;
+    const double ndd = s2 * a - 2.0 * cs * b + c2 * dd;
+    const double nb = cs * (dd - a) + (c2 - s2) * b;
+    const double bulge = s * en, ne = c * en;
+    if (lane == 0) {
+      if (k > l) e[k - 1] = r;
+      d[k] = na;
+    }
+    if (lane < nrows) M[lane * ldm + k] = c * mk + s * mk1;
+    mk = -s * mk + c * mk1;
+    mk1 = mk2;
+    if (k + 1 < h) {
+      a = ndd;
+      b = ne;
+      dd = d2;
+      en = e2;
+      x = nb;
+      z = bulge;
+    } else if (lane == 0) {
+      d[k + 1] = ndd;
+      e[k] = nb;
+    }
+  }
+  if (lane < nrows) M[lane * ldm + h] = mk;
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ void wave_fence() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_wave_barrier();
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+__device__ __forceinline__ bool sym_before(double a, double b, int which) {
+  switch (which) {
+    case 0: return fabs(a) > fabs(b);   // largest |lambda|
+    case 1: return a > b;               // largest real part
+    case 3: return fabs(a) < fabs(b);   // smallest |lambda|
+    case 4: return a < b;               // smallest real part
+    default: return false;              // imaginary parts are all zero
+  }
+}
+
+struct SymLds {
+  double alpha[kMaxMB], beta[kMaxMB];  // Lanczos T (beta[j] = T[j+1][j])
+  double d[kMaxMB], e[kMaxMB];         // work copies
+  double ev[kMaxMB], est[kMaxMB];
+  int order[kMaxMB];
+  double Z[kMaxMB * (kMaxMB + 1)];     // eigenvectors of T, then Q of the shifts
+  double Y[kMaxMB * kMaxMB];           // wanted Ritz vectors of T (m x k)
+  int ctl[4];                          // nconv, kk, fail
+  double ctld[4];                      // beta_k, sigma, maxabs
+};
+
+// wave 0 only.  T = (alpha, beta) of size m, bm = ||f||.  Leaves in L:
+// ev/est/order, Y, ctl = {nconv, kk}, and when `shift`: the kept T in
+// (alpha, beta)[0..kk), Z = Q of the shifts, ctld = {beta_k, sigma}.
+__device__ void sym_restart(SymLds& L, int m, int k, int which, double tol, double bm, bool shift) {
+  const int lane = threadIdx.x & 63;
+  const int ld = kMaxMB + 1;
+  // eigendecomposition of T
+  if (lane < m) {
+    for (int c = 0; c < m; ++c) L.Z[lane * ld + c] = lane == c ? 1.0 : 0.0;
+  }
+  if (lane == 0) {
+    for (int i = 0; i < m; ++i) {
+      L.d[i] = L.alpha[i];
+      L.e[i] = i + 1 < m ? L.beta[i] : 0.0;
+    }
+  }
+  wave_fence();
+  int h = m - 1, iters = 0, fail = 0;
+  while (h > 0) {
+    if (negligible(L.e[h - 1], L.d[h - 1], L.d[h])) {
+      if (lane == 0) L.e[h - 1] = 0.0;
+      wave_fence();
+      --h;
+      continue;
+    }
+    int l = h - 1;
+    while (l > 0 && !negligible(L.e[l - 1], L.d[l - 1], L.d[l])) --l;
+    if (l > 0 && lane == 0) L.e[l - 1] = 0.0;
+    if (++iters > 30 * m) {
+      fail = 1;
+      break;
+    }
+    // Wilkinson shift from the trailing 2 x 2 of the block
+    const double a = L.d[h - 1], b = L.e[h - 1], cc = L.d[h];
+    const double del = 0.5 * (a - cc);
+    const double den = del + copysign(sqrt(del * del + b * b), del);
+    const double mu = den != 0.0 ? cc - b * b / den : cc - fabs(b);
+    wave_fence();
+    sym_sweep(L.d, L.e, l, h, mu, L.Z, ld, m);
+  }
+  // Ritz values, estimates bm |Z[m-1][i]|, wanted order
+  double maxabs = 0.0;
+  for (int i = 0; i < m; ++i) maxabs = fmax(maxabs, fabs(L.d[i]));
+  if (lane < m) {
+    L.ev[lane] = L.d[lane];
+    L.est[lane] = bm * fabs(L.Z[(m - 1) * ld + lane]);
+    int rank = 0;
+    const double me = L.d[lane];
+    for (int j = 0; j < m; ++j) {
+      const double oj = L.d[j];
+      if (sym_before(oj, me, which) || (!sym_before(me, oj, which) && j < lane)) ++rank;
+    }
+    L.order[rank] = lane;
+  }
+  wave_fence();
+  int nconv = 0;
+  const double tolabs = tol * fmax(maxabs, 1e-300);
+  for (int t = 0; t < k && t < m; ++t)
+    if (L.est[L.order[t]] <= tolabs) ++nconv;
+  const int p2 = (m - k) / 2;
+  int kk = k + (nconv < p2 ? nconv : p2);
+  if (kk == 1 && m >= 6) kk = m / 2;
+  else if (kk == 1 && m > 2) kk = 2;
+  if (kk >= m) kk = m - 1;
+  // wanted Ritz vectors of T
+  if (lane < m)
+    for (int t = 0; t < k; ++t) L.Y[lane * k + t] = L.Z[lane * ld + L.order[t]];
+  if (lane == 0) {
+    L.ctl[0] = nconv;
+    L.ctl[1] = kk;
+    L.ctl[2] = fail;
+    L.ctld[2] = maxabs;
+  }
+  wave_fence();
+  if (!shift) return;
+  // exact shifts: sweeps with the unwanted Ritz values on the original T
+  if (lane < m)
+    for (int c = 0; c < m; ++c) L.Z[lane * ld + c] = lane == c ? 1.0 : 0.0;
+  if (lane == 0)
+    for (int i = 0; i < m; ++i) {
+      L.d[i] = L.alpha[i];
+      L.e[i] = i + 1 < m ? L.beta[i] : 0.0;
+    }
+  wave_fence();
+  for (int t = kk; t < m; ++t) {
+    const double mu = L.ev[L.order[t]];
+    int l = 0;
+    while (l < m - 1) {
+      int hh = l;
+      while (hh < m - 1 && !negligible(L.e[hh], L.d[hh], L.d[hh + 1])) ++hh;
+      if (hh < m - 1 && lane == 0) L.e[hh] = 0.0;
+      wave_fence();
+      if (hh > l) sym_sweep(L.d, L.e, l, hh, mu, L.Z, ld, m);
+      l = hh + 1;
+    }
+  }
+  if (lane == 0) {
+    for (int i = 0; i < kk; ++i) {
+      L.alpha[i] = L.d[i];
+      L.beta[i] = L.e[i];  // beta[kk-1] is replaced by the new ||f||
+    }
+    L.ctld[0] = L.e[kk - 1];                // T+[kk][kk-1]
+    L.ctld[1] = L.Z[(m - 1) * ld + kk - 1];  // Q[m-1][kk-1]
+  }
+  wave_fence();
+}
+
 template <int NT, int ROWS>
 __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
   constexpr int CPT = Ctx<NT>::CPT;
   __shared__ double vl[kMaxN], wl[kMaxN];
   __shared__ double red[(NT / 64) * (kMaxMB + 2)];
   __shared__ double coef[kMaxMB + 2];
-  __shared__ int ctl_i[2];
-  __shared__ double ctl_d[2];
-  __shared__ double small_lds[4 * kMaxMB * iram::lds_ld(kMaxMB)];
+  __shared__ SymLds L;
   const uint32_t pid = a.order[blockIdx.x];
   const Problem P = a.probs[pid];
   const uint32_t n = P.n, m = a.m, k = a.k;
   const float* X = a.X + P.x_off;
-  double* V = a.V + P.v_off;  // column c at V + c*n
-  double* Hre = a.H + P.h_off;
-  double* Him = Hre + m * m;
-  double* Qre = Him + m * m;
-  double* Qim = Qre + m * m;
-  double* Xwre = Qim + m * m;
-  double* Xwim = Xwre + m * m;
-  double* Yre = Xwim + m * m;
-  double* Yim = Yre + m * k;
-  Status* st = a.status + pid;
+  double* V = a.V + P.v_off;  // column c at V + c*n; column m holds f
   const int tid = threadIdx.x;
+  constexpr int ld = kMaxMB + 1;
 
   // start vector: uniform in [-1, 1) from the problem's SplitMix64 stream
   {
@@ -230,27 +462,21 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
     const double inv = 1.0 / sqrt(coef[0]);
     for (uint32_t c = tid; c < n; c += NT) V[c] *= inv;
   }
-  for (uint32_t i = tid; i < 2 * m * m; i += NT) Hre[i] = 0.0;  // H and its zero im plane
-  if (tid == 0) {
-    st->breakdown = 0;
-    st->beta = 0;
-  }
   __syncthreads();
 
-  double* F = V + (size_t)m * n;  // residual f (column m of V)
+  double* F = V + (size_t)m * n;
   uint32_t start = 0, restarts = 0, matvecs = 0;
-  RestartArgs ra;
-  ra.Hre = Hre; ra.Him = Him; ra.Qre = Qre; ra.Qim = Qim;
-  ra.Yre = Yre; ra.Yim = Yim; ra.Xre = Xwre; ra.Xim = Xwim;
-  ra.st = st; ra.m = (int)m; ra.mact = (int)m; ra.k = (int)k; ra.which = a.which;
-  ra.tol = a.tol; ra.anorm = 0.0;  // relative to the largest Ritz value (= ||C||_2)
-  ra.do_shifts = 1;
+  long long cyc[5] = {0, 0, 0, 0, 0};
+  double bm = 0.0;  // ||f|| after the last Lanczos step
   for (;;) {
     for (uint32_t j = start; j < m; ++j) {
+      const long long t0 = clock64();
       for (uint32_t c = tid; c < n; c += NT) vl[c] = V[(size_t)j * n + c];
       __syncthreads();
       covariance_matvec<NT, ROWS>(X, n, P.ldx, a.s, vl, wl);
       ++matvecs;
+      const long long t1 = clock64();
+      cyc[0] += t1 - t0;
       // two classical Gram-Schmidt passes against V[:, 0..j]
       double w[CPT];
 #pragma unroll
@@ -258,52 +484,101 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
         const uint32_t c = tid + NT * q;
         w[q] = c < n ? wl[c] : 0.0;
       }
-      double alpha = 0;
-      for (int pass = 0; pass < 2; ++pass) {
-        for (uint32_t l = 0; l <= j; ++l) {
-          const double* Vl = V + (size_t)l * n;
+      // classical Gram-Schmidt against V[:, 0..j] with the DGKS correction
+      // (ARPACK dsaitr): h = V^T w (||w||^2 in the same reduction),
+      // w -= V h; when ||w|| < 0.717 ||w_before||, once more h2 = V^T w,
+      // w -= V h2, alpha_j += h2[j]
+      // columns are taken four at a time so that their loads are in flight
+      // together
+      auto project = [&](bool with_norm) {
+        for (uint32_t l0 = 0; l0 <= j; l0 += 4) {
+          double vv[4][CPT];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t l = l0 + u;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+              const uint32_t c = tid + NT * q;
+              vv[u][q] = (l <= j && c < n) ? V[(size_t)l * n + c] : 0.0;
+            }
+          }
+          double p[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            p[u] = 0;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) p[u] += vv[u][q] * w[q];
+          }
+#pragma unroll
+          for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+            for (int u = 0; u < 4; ++u) p[u] += __shfl_xor(p[u], off, 64);
+          if ((tid & 63) == 0) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+              if (l0 + u <= j) red[(tid >> 6) * (kMaxMB + 2) + l0 + u] = p[u];
+          }
+        }
+        if (with_norm) {
           double p = 0;
 #pragma unroll
-          for (int q = 0; q < CPT; ++q) {
-            const uint32_t c = tid + NT * q;
-            if (c < n) p += Vl[c] * w[q];
-          }
+          for (int q = 0; q < CPT; ++q) p += w[q] * w[q];
           p = wave_sum(p);
-          if ((tid & 63) == 0) red[(tid >> 6) * (kMaxMB + 2) + l] = p;
+          if ((tid & 63) == 0) red[(tid >> 6) * (kMaxMB + 2) + j + 1] = p;
         }
         __syncthreads();
-        if ((uint32_t)tid <= j) {
+        if ((uint32_t)tid <= j + (with_norm ? 1u : 0u)) {
           double t = 0;
           for (int wv = 0; wv < NT / 64; ++wv) t += red[wv * (kMaxMB + 2) + tid];
           coef[tid] = t;
         }
         __syncthreads();
-        for (uint32_t l = 0; l <= j; ++l) {
-          const double* Vl = V + (size_t)l * n;
-          const double cl = coef[l];
+      };
+      auto subtract = [&]() {
+        for (uint32_t l0 = 0; l0 <= j; l0 += 4) {
+          double vv[4][CPT];
 #pragma unroll
-          for (int q = 0; q < CPT; ++q) {
-            const uint32_t c = tid + NT * q;
-            if (c < n) w[q] -= Vl[c] * cl;
+          for (int u = 0; u < 4; ++u) {
+            const uint32_t l = l0 + u;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) {
+              const uint32_t c = tid + NT * q;
+              vv[u][q] = (l <= j && c < n) ? V[(size_t)l * n + c] : 0.0;
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const double cl = l0 + u <= j ? coef[l0 + u] : 0.0;
+#pragma unroll
+            for (int q = 0; q < CPT; ++q) w[q] -= vv[u][q] * cl;
           }
         }
-        alpha += coef[j];
-        __syncthreads();
-      }
+      };
+      project(true);
+      double alpha = coef[j];
+      const double wnorm = sqrt(coef[j + 1]);
+      subtract();
       double ss[1] = {0};
 #pragma unroll
       for (int q = 0; q < CPT; ++q) ss[0] += w[q] * w[q];
       block_sums<NT, 1>(ss, 1, red, coef);
-      const double beta = sqrt(coef[0]);
-      // symmetric tridiagonal H: alpha_j on the diagonal, beta_j mirrored
-      if (tid == 0) {
-        Hre[(size_t)j * m + j] = alpha;
-        if (j + 1 < m) {
-          Hre[(size_t)(j + 1) * m + j] = beta;
-          Hre[(size_t)j * m + j + 1] = beta;
-        }
-        st->beta = beta;
+      double beta = sqrt(coef[0]);
+      if (beta < 0.717 * wnorm) {
+        project(false);
+        alpha += coef[j];
+        subtract();
+        ss[0] = 0;
+#pragma unroll
+        for (int q = 0; q < CPT; ++q) ss[0] += w[q] * w[q];
+        block_sums<NT, 1>(ss, 1, red, coef);
+        beta = sqrt(coef[0]);
       }
+      if (tid == 0) {
+        L.alpha[j] = alpha;
+        if (j + 1 < m) L.beta[j] = beta;
+      }
+      bm = beta;
+      cyc[1] += clock64() - t1;
       const double inv = beta > 0 ? 1.0 / beta : 0.0;
       double* dst = j + 1 < m ? V + (size_t)(j + 1) * n : F;
       const double sc = j + 1 < m ? inv : 1.0;
@@ -314,25 +589,18 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
       }
       __syncthreads();
     }
-    // restart: Schur form of H, estimates, wanted order, exact shifts
-    iram::restart_body(ra, small_lds);
     ++restarts;
-    // the status written by one thread, broadcast through LDS so that every
-    // thread takes the same branch and loop bounds
-    if (tid == 0) {
-      ctl_i[0] = st->nconv;
-      ctl_i[1] = st->kk;
-      ctl_d[0] = st->beta_k_re;
-      ctl_d[1] = st->sigma_re;
-    }
+    const bool last = restarts >= a.max_restarts;
+    const long long t2 = clock64();
+    if (tid < 64) sym_restart(L, (int)m, (int)k, a.which, a.tol, bm, !last);
     __syncthreads();
-    if (ctl_i[0] >= (int)k || restarts >= a.max_restarts) break;
-    // V <- V Q[:, 0..kk), f <- (V Q)[:, kk] beta_k + f sigma, v_kk = f / ||f||.
-    // Q (real: H is real) is still in the LDS copy restart_body left in Z.
-    const uint32_t qld = (uint32_t)iram::lds_ld((int)m);
-    const double* Qs = small_lds + 2 * m * qld;
-    const uint32_t kk = (uint32_t)ctl_i[1];
-    const double bk = ctl_d[0], sg = ctl_d[1];
+    const long long t3 = clock64();
+    cyc[2] += t3 - t2;
+    if (L.ctl[0] >= (int)k || last) break;
+    // V <- V Q[:, 0..kk), f <- (V Q)[:, kk] beta_k + f sigma, v_kk = f / ||f||
+    const uint32_t kk = (uint32_t)L.ctl[1];
+    const double bk = L.ctld[0], sg = L.ctld[1];
+    if (kk == 0 || kk >= m) break;
     double ss[1] = {0};
     for (uint32_t c = tid; c < n; c += NT) {
       double vrow[kMaxMB];
@@ -343,7 +611,7 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
         double acc = 0;
 #pragma unroll
         for (int l = 0; l < kMaxMB; ++l)
-          if ((uint32_t)l < m) acc += vrow[l] * Qs[l * qld + t];
+          if ((uint32_t)l < m) acc += vrow[l] * L.Z[l * ld + t];
         if (t < kk) V[(size_t)t * n + c] = acc;
         else fk = acc;
       }
@@ -355,13 +623,12 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
     const double beta = sqrt(coef[0]);
     const double inv = beta > 0 ? 1.0 / beta : 0.0;
     for (uint32_t c = tid; c < n; c += NT) V[(size_t)kk * n + c] = F[c] * inv;
-    if (tid == 0) {
-      Hre[(size_t)kk * m + kk - 1] = beta;
-      Hre[(size_t)(kk - 1) * m + kk] = beta;
-    }
+    if (tid == 0) L.beta[kk - 1] = beta;
     __syncthreads();
+    cyc[3] += clock64() - t3;
     start = kk;
   }
+  const long long t4 = clock64();
   // Ritz vectors x_t = V Y[:, t] (normalised), Rayleigh quotients and
   // explicit residuals ||C x - lambda x||_2 in FP64
   double* OUT = a.out + P.out_off;
@@ -370,7 +637,7 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
     double ss[1] = {0};
     for (uint32_t c = tid; c < n; c += NT) {
       double acc = 0;
-      for (uint32_t l = 0; l < m; ++l) acc += V[(size_t)l * n + c] * Yre[(size_t)l * k + t];
+      for (uint32_t l = 0; l < m; ++l) acc += V[(size_t)l * n + c] * L.Y[l * k + t];
       vl[c] = acc;
       ss[0] += acc * acc;
     }
@@ -397,10 +664,12 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
     }
     __syncthreads();
   }
+  cyc[4] += clock64() - t4;
   if (tid == 0) {
-    R->nconv = ctl_i[0];
+    R->nconv = L.ctl[0];
     R->restarts = (int)restarts;
     R->matvecs = (int)matvecs;
+    for (int i = 0; i < 5; ++i) R->cycles[i] = cyc[i];
   }
 }
 
@@ -473,7 +742,7 @@ struct Variant {
 };
 const Variant kVariants[] = {
     {256, 1, k_batched_solve<256, 1>},
-    {256, 2, k_batched_solve<256, 2>},
+    {128, 1, k_batched_solve<128, 1>},
     {512, 1, k_batched_solve<512, 1>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
@@ -485,7 +754,7 @@ struct oaph_batch {
   std::vector<uint32_t> n;
   std::vector<Problem> probs;
   std::vector<uint32_t> order;
-  DevBuf dprobs, dorder, X, V, H, out, status, results;
+  DevBuf dprobs, dorder, X, V, out, results;
   uint32_t m = 0, k = 0;
   int variant = 1;
   hipStream_t stream = nullptr;
@@ -508,8 +777,8 @@ static void drop_graph(oaph_batch* b) {
 }
 
 static bool same_args(const SolveArgs& x, const SolveArgs& y) {
-  return x.probs == y.probs && x.order == y.order && x.status == y.status &&
-         x.results == y.results && x.X == y.X && x.V == y.V && x.H == y.H && x.out == y.out &&
+  return x.probs == y.probs && x.order == y.order && x.results == y.results && x.X == y.X &&
+         x.V == y.V && x.out == y.out &&
          x.s == y.s && x.m == y.m && x.k == y.k && x.which == y.which && x.tol == y.tol &&
          x.max_restarts == y.max_restarts && x.seed == y.seed;
 }
@@ -530,33 +799,27 @@ static oaph_status prepare(oaph_batch* b, uint32_t hdim, uint32_t wanted, int so
   for (uint32_t i = 0; i < b->count; ++i) OAPH_CHECK(b->n[i] >= hdim, OAPH_ERR_SHAPE);
   const uint32_t m = hdim, k = wanted;
   if (m != b->m || k != b->k) {
-    uint64_t vo = 0, ho = 0, oo = 0;
+    uint64_t vo = 0, oo = 0;
     for (uint32_t i = 0; i < b->count; ++i) {
       Problem& p = b->probs[i];
       p.v_off = vo;
       vo += (uint64_t)(m + 1) * p.n;
-      p.h_off = ho;
-      ho += 6ull * m * m + 2ull * m * k;
       p.out_off = oo;
       oo += (uint64_t)k * p.n;
     }
     drop_graph(b);
     OAPH_TRY(upload_problems(b));
     OAPH_TRY(b->V.reserve(vo * sizeof(double)));
-    OAPH_TRY(b->H.reserve(ho * sizeof(double)));
     OAPH_TRY(b->out.reserve(oo * sizeof(double)));
-    OAPH_TRY(b->status.reserve(b->count * sizeof(Status)));
     OAPH_TRY(b->results.reserve(b->count * sizeof(Result)));
     b->m = m;
     b->k = k;
   }
   a->probs = b->dprobs.as<Problem>();
   a->order = b->dorder.as<uint32_t>();
-  a->status = b->status.as<Status>();
   a->results = b->results.as<Result>();
   a->X = b->X.as<float>();
   a->V = b->V.as<double>();
-  a->H = b->H.as<double>();
   a->out = b->out.as<double>();
   a->s = b->s;
   a->m = m;
@@ -671,7 +934,7 @@ oaph_status oaph_batch_destroy(oaph_batch* b) {
   if (!b) return OAPH_OK;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   drop_graph(b);
-  for (DevBuf* d : {&b->dprobs, &b->dorder, &b->X, &b->V, &b->H, &b->out, &b->status, &b->results})
+  for (DevBuf* d : {&b->dprobs, &b->dorder, &b->X, &b->V, &b->out, &b->results})
     d->release();
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
@@ -812,6 +1075,14 @@ oaph_status oaph_batch_results(oaph_batch* b, double* evals, double* resid, uint
     if (matvecs) matvecs[i] = (uint32_t)r.matvecs;
   }
   if (kernel_ms) *kernel_ms = b->last_ms;
+  return OAPH_OK;
+}
+
+oaph_status oaph_batch_phase_cycles(oaph_batch* b, long long* cycles) {
+  OAPH_CHECK(b && cycles, OAPH_ERR_ARG);
+  OAPH_CHECK(b->host_results.size() == b->count, OAPH_ERR_STATE);
+  for (uint32_t i = 0; i < b->count; ++i)
+    for (int p = 0; p < 5; ++p) cycles[(size_t)i * 5 + p] = b->host_results[i].cycles[p];
   return OAPH_OK;
 }
 
