@@ -16,6 +16,7 @@
 // throws oap::Error carrying the oaph_status.
 #pragma once
 
+#include <algorithm>
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -196,6 +197,78 @@ class CuProceduresApi {
  private:
   oaph_api* m_api = nullptr;
   bool m_owned = true;
+};
+
+// oap::RecToSquareApi (oapMatrixCuda/RecToSquareApi.h:33-58) over a DEVICE
+// matrix M (the reference takes a host matrix and copies blocks of it to the
+// device on demand; here M stays resident in HBM).  With deallocate = true
+// the object frees M when it is destroyed, as the reference frees its host
+// matrix.  S = M M^T, or M itself when M is square.
+class RecToSquareApi final {
+ public:
+  RecToSquareApi(CuProceduresApi& api, const math::Matrix* rec, bool deallocate)
+      : m_rec(rec), m_deallocate(deallocate) {
+    check(oaph_rts_create(&m_rts, api.handle(), c(rec)), "RecToSquareApi");
+  }
+  ~RecToSquareApi() {
+    oaph_rts_destroy(m_rts);
+    if (m_deallocate) oaph_matrix_free(const_cast<oaph_matrix*>(c(m_rec)));
+  }
+  RecToSquareApi(const RecToSquareApi&) = delete;
+  RecToSquareApi(RecToSquareApi&&) = delete;
+  RecToSquareApi& operator=(const RecToSquareApi&) = delete;
+  RecToSquareApi& operator=(RecToSquareApi&&) = delete;
+
+  math::MatrixInfo getMatrixInfo() const {
+    oaph_matrix_info i{};
+    check(oaph_rts_info(m_rts, &i), "RecToSquareApi::getMatrixInfo");
+    return math::MatrixInfo(i.isRe != 0, i.isIm != 0, i.columns, i.rows);
+  }
+  math::Matrix* createDeviceMatrix() {
+    const math::MatrixInfo i = getMatrixInfo();
+    math::Matrix* m = cuda::NewDeviceReMatrix(i.columns(), i.rows());
+    return getDeviceMatrix(m);
+  }
+  math::Matrix* getDeviceMatrix(math::Matrix* dmatrix) {
+    check(oaph_rts_square(m_rts, c(dmatrix)), "RecToSquareApi::getDeviceMatrix");
+    return dmatrix;
+  }
+  math::Matrix* createDeviceSubMatrix(uintt rindex, uintt rlength) {
+    const math::MatrixInfo i = getMatrixInfo();
+    const uintt rows = rindex < i.rows() ? std::min(rlength, i.rows() - rindex) : 1;
+    math::Matrix* m = cuda::NewDeviceMatrix(i.isRe, i.isIm, i.columns(), rows ? rows : 1);
+    return getDeviceSubMatrix(rindex, rlength, m);
+  }
+  math::Matrix* getDeviceSubMatrix(uintt rindex, uintt rlength, math::Matrix* dmatrix) {
+    if (dmatrix == nullptr) return createDeviceSubMatrix(rindex, rlength);
+    check(oaph_rts_sub_matrix(m_rts, rindex, rlength, c(dmatrix)),
+          "RecToSquareApi::getDeviceSubMatrix");
+    return dmatrix;
+  }
+  math::Matrix* createDeviceRowVector(uintt index) {
+    const math::MatrixInfo i = getMatrixInfo();
+    return getDeviceRowVector(index, cuda::NewDeviceReMatrix(i.columns(), 1));
+  }
+  math::Matrix* getDeviceRowVector(uintt index, math::Matrix* dmatrix) {
+    if (dmatrix == nullptr) return createDeviceRowVector(index);
+    check(oaph_rts_row_vector(m_rts, index, c(dmatrix)), "RecToSquareApi::getDeviceRowVector");
+    return dmatrix;
+  }
+  // w = S v without forming S (the product of the reference's multiply
+  // callbacks, oap2dt3dDevice/MainAPExecutor.cpp:42-127), enqueued on the
+  // stream of `api` -- in a multiply callback, the api the callback receives
+  void multiply(math::Matrix* w, const math::Matrix* v, CuProceduresApi& api) {
+    check(oaph_rts_multiply(m_rts, api.handle(), c(w), c(v)), "RecToSquareApi::multiply");
+  }
+  void multiply(math::Matrix* w, const math::Matrix* v) {
+    check(oaph_rts_multiply(m_rts, nullptr, c(w), c(v)), "RecToSquareApi::multiply");
+  }
+  oaph_rts* handle() const { return m_rts; }
+
+ private:
+  oaph_rts* m_rts = nullptr;
+  const math::Matrix* m_rec;
+  bool m_deallocate;
 };
 
 enum class QRType { NONE = -1, QRGR, QRHT };

@@ -402,6 +402,43 @@ oaph_status oaph_mlp_train(oaph_mlp* m, uint32_t B, uint32_t epochs, float lr, u
                            float* losses);
 
 /* ------------------------------------------------------------------ */
+/* Square operator of a rectangular matrix: oap::RecToSquareApi           */
+/* (oapMatrixCuda/RecToSquareApi.h:33-58, RecToSquareApi.cpp:57-141;      */
+/* oapMatrixCuda/SquareMatrixApi.cpp:143-181).  For a device matrix M of  */
+/* R rows and C columns the operator is S = M M^T (R x R, plain transpose */
+/* as in the reference); when C == R, S is M itself (RecToSquareApi.cpp   */
+/* :57-63,94-97).  M stays owned by the caller and must outlive the       */
+/* handle; work is enqueued on the api's stream.                          */
+/* ------------------------------------------------------------------ */
+typedef struct oaph_rts oaph_rts;
+
+/* RecToSquareApi(api, recMatrix, deallocate = false) over a device matrix */
+oaph_status oaph_rts_create(oaph_rts** out, oaph_api* api, const oaph_matrix* rec);
+oaph_status oaph_rts_destroy(oaph_rts* t);
+/* getMatrixInfo: R x R with M's planes (RecToSquareApi.cpp:31-43) */
+oaph_status oaph_rts_info(const oaph_rts* t, oaph_matrix_info* info);
+/* getDeviceMatrix / createDeviceMatrix: out (R x R) = S; out's planes
+ * select the product's planes as in oaph_dot_product */
+oaph_status oaph_rts_square(oaph_rts* t, oaph_matrix* out);
+/* getDeviceSubMatrix(rindex, rlength, dmatrix): rows [rindex, rindex + n)
+ * of S with n = min(rlength, R - rindex); out must have R columns and is
+ * given exactly n rows (its planes are reallocated when its row count
+ * differs, as the reference reallocates, SquareMatrixApi.cpp:162-168).
+ * rindex >= R or rlength == 0 -> OAPH_ERR_ARG (the reference throws,
+ * RecToSquareApi.cpp:143-165). */
+oaph_status oaph_rts_sub_matrix(oaph_rts* t, uint32_t rindex, uint32_t rlength, oaph_matrix* out);
+/* getDeviceRowVector(index, dmatrix): row `index` of S into out (R x 1 row) */
+oaph_status oaph_rts_row_vector(oaph_rts* t, uint32_t index, oaph_matrix* out);
+/* w = S v without forming S: u = M^T v, then w = M u (the product the
+ * reference's multiply callbacks form block by block,
+ * oap2dt3dDevice/MainAPExecutor.cpp:42-127).  v, w are R x 1 columns; w
+ * needs an imaginary plane when M or v has one.  Enqueued on api's stream
+ * (NULL: the handle's api); inside a solver multiply callback pass the api
+ * the callback receives, so the product is ordered with the solver's work.
+ * Calls on one handle must not overlap (shared workspace). */
+oaph_status oaph_rts_multiply(oaph_rts* t, oaph_api* api, oaph_matrix* w, const oaph_matrix* v);
+
+/* ------------------------------------------------------------------ */
 /* Batched covariance eigenproblems.  Problem i is C_i = X_i^T X_i / s with
  * X_i an s x n_i FP32 row-major sample matrix: the M M^T operator the
  * reference forms with RecToSquareApi / SquareMatrixApi

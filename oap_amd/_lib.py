@@ -165,6 +165,13 @@ _SIGS = {
                                        U32]),
     "oaph_mlp_train": (C.c_int, [C.c_void_p, U32, U32, C.c_float, C.c_uint64,
                                  C.POINTER(C.c_float)]),
+    "oaph_rts_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, MP]),
+    "oaph_rts_destroy": (C.c_int, [C.c_void_p]),
+    "oaph_rts_info": (C.c_int, [C.c_void_p, C.POINTER(oaph_matrix_info)]),
+    "oaph_rts_square": (C.c_int, [C.c_void_p, MP]),
+    "oaph_rts_sub_matrix": (C.c_int, [C.c_void_p, U32, U32, MP]),
+    "oaph_rts_row_vector": (C.c_int, [C.c_void_p, U32, MP]),
+    "oaph_rts_multiply": (C.c_int, [C.c_void_p, C.c_void_p, MP, MP]),
     "oaph_batch_c3_sizes": (C.c_int, [U32, C.c_uint64, C.POINTER(U32)]),
     "oaph_batch_create": (C.c_int, [C.POINTER(C.c_void_p), U32, C.POINTER(U32), U32]),
     "oaph_batch_destroy": (C.c_int, [C.c_void_p]),

@@ -541,6 +541,79 @@ __all__ += ["CuHArnoldiCallback", "MatrixInfo", "QRType", "VecMultiplicationType
             "InitVVectorType", "ArnUtils", "Precision", "DeviceMatrixView"]
 
 
+class RecToSquareApi:
+    """The square operator S = M M^T of a rectangular device matrix M.
+
+    Mirrors oap::RecToSquareApi (oapMatrixCuda/RecToSquareApi.h:33-58): S is
+    R x R for M with R rows; a matrix that is already square is used as it
+    is (RecToSquareApi.cpp:57-63).  Blocks of S come from a tiled product
+    that reads M transposed in place (SquareMatrixApi.cpp:143-181 forms a
+    transposed copy instead); multiply(w, v) applies S without forming it.
+    M must outlive this object.
+    """
+
+    def __init__(self, api: "CuProceduresApi", rec: DeviceMatrix):
+        self._lib = load()
+        self._rec = rec
+        h = C.c_void_p()
+        check(self._lib.oaph_rts_create(C.byref(h), api.handle, rec.ptr), "oaph_rts_create")
+        self._h = h
+
+    def getMatrixInfo(self) -> oaph_matrix_info:
+        inf = oaph_matrix_info()
+        check(self._lib.oaph_rts_info(self._h, C.byref(inf)), "oaph_rts_info")
+        return inf
+
+    def createDeviceMatrix(self) -> DeviceMatrix:
+        inf = self.getMatrixInfo()
+        out = DeviceMatrix(inf.columns, inf.rows, True, False)
+        return self.getDeviceMatrix(out)
+
+    def getDeviceMatrix(self, out: DeviceMatrix) -> DeviceMatrix:
+        check(self._lib.oaph_rts_square(self._h, out.ptr), "oaph_rts_square")
+        return out
+
+    def createDeviceSubMatrix(self, rindex: int, rlength: int) -> DeviceMatrix:
+        inf = self.getMatrixInfo()
+        rows = max(1, min(rlength, inf.rows - rindex)) if rindex < inf.rows else 1
+        out = DeviceMatrix(inf.columns, rows, bool(inf.isRe), bool(inf.isIm))
+        return self.getDeviceSubMatrix(rindex, rlength, out)
+
+    def getDeviceSubMatrix(self, rindex: int, rlength: int, out: DeviceMatrix | None) -> DeviceMatrix:
+        if out is None:
+            return self.createDeviceSubMatrix(rindex, rlength)
+        check(self._lib.oaph_rts_sub_matrix(self._h, rindex, rlength, out.ptr),
+              "oaph_rts_sub_matrix")
+        return out
+
+    def createDeviceRowVector(self, index: int) -> DeviceMatrix:
+        inf = self.getMatrixInfo()
+        return self.getDeviceRowVector(index, DeviceMatrix(inf.columns, 1, True, False))
+
+    def getDeviceRowVector(self, index: int, out: DeviceMatrix | None) -> DeviceMatrix:
+        if out is None:
+            return self.createDeviceRowVector(index)
+        check(self._lib.oaph_rts_row_vector(self._h, index, out.ptr), "oaph_rts_row_vector")
+        return out
+
+    def multiply(self, w: DeviceMatrix, v: DeviceMatrix, api: "CuProceduresApi | None" = None):
+        """w = S v without forming S; `api` selects the stream (inside a
+        solver callback, the api the callback receives)."""
+        check(self._lib.oaph_rts_multiply(self._h, None if api is None else api.handle, w.ptr,
+                                          v.ptr), "oaph_rts_multiply")
+
+    def destroy(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.oaph_rts_destroy(self._h)
+            self._h = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+
 class CovarianceBatch:
     """Many real symmetric eigenproblems C_i = X_i^T X_i / s solved together
     on the GPU (one persistent launch, one workgroup per problem).

@@ -252,8 +252,10 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
 // CuDotProductDimProcedures.h:30-48).  64x64 output tile per 256 threads
 // (4x4 outputs per thread), K staged through LDS in slices of 16.
 // MODE 0: complex, 1: re only, 2: im only (dims semantics: im = -sum a_im b_im).
+// TB: the right operand is read transposed, B[k][x] = b[x * ldb + k] (A * B^T
+// without forming B^T).
 constexpr int GT = 64, GK = 16;
-template <int MODE>
+template <int MODE, bool TB = false>
 __global__ __launch_bounds__(256) void k_gemm(uint32_t W, uint32_t H, uint32_t K,
                                               const double* __restrict__ are,
                                               const double* __restrict__ aim, uint32_t lda,
@@ -280,10 +282,11 @@ __global__ __launch_bounds__(256) void k_gemm(uint32_t W, uint32_t H, uint32_t K
       size_t ia = (size_t)gy * lda + gk;
       sAr[kk][rr] = (HASRE || MODE == 0) && ok && are ? are[ia] : 0.0;
       sAi[kk][rr] = HASIM && ok && aim ? aim[ia] : 0.0;
-      int cc = e & (GT - 1), kb = e / GT;  // B: consecutive threads walk columns
+      // B: consecutive threads walk columns (or k, when B is read transposed)
+      int cc = TB ? rr : (e & (GT - 1)), kb = TB ? kk : e / GT;
       uint32_t gx = x0 + cc, gk2 = k0 + kb;
       bool ok2 = gx < W && gk2 < K;
-      size_t ib = (size_t)gk2 * ldb + gx;
+      size_t ib = TB ? (size_t)gx * ldb + gk2 : (size_t)gk2 * ldb + gx;
       sBr[kb][cc] = (HASRE || MODE == 0) && ok2 && bre ? bre[ib] : 0.0;
       sBi[kb][cc] = HASIM && ok2 && bim ? bim[ib] : 0.0;
     }
@@ -430,6 +433,24 @@ oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
     hipLaunchKernelGGL(k_gemm<2>, grid, dim3(256), 0, s, W, H, K, nullptr, a->imValues,
                        a->columns, nullptr, b->imValues, b->columns, nullptr, out->imValues,
                        out->columns);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, const double* are,
+                           const double* aim, uint32_t lda, const double* bre, const double* bim,
+                           uint32_t ldb, double* ore, double* oim, uint32_t ldo) {
+  dim3 grid((W + GT - 1) / GT, (H + GT - 1) / GT);
+  OAPH_CHECK(grid.y <= 65535, OAPH_ERR_UNSUPPORTED);
+  if (ore && oim)
+    hipLaunchKernelGGL((k_gemm<0, true>), grid, dim3(256), 0, s, W, H, K, are, aim, lda, bre, bim,
+                       ldb, ore, oim, ldo);
+  else if (ore)
+    hipLaunchKernelGGL((k_gemm<1, true>), grid, dim3(256), 0, s, W, H, K, are, nullptr, lda, bre,
+                       nullptr, ldb, ore, nullptr, ldo);
+  else
+    hipLaunchKernelGGL((k_gemm<2, true>), grid, dim3(256), 0, s, W, H, K, nullptr, aim, lda,
+                       nullptr, bim, ldb, nullptr, oim, ldo);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

@@ -92,6 +92,10 @@ oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
 oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                          const double* are, const double* aim, const double* xre,
                          const double* xim, double* yre, double* yim);
+// out(x, y) = sum_k A[y][k] B[x][k] (A * B^T, B read in place), W x H, K terms
+oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, const double* are,
+                           const double* aim, uint32_t lda, const double* bre, const double* bim,
+                           uint32_t ldb, double* ore, double* oim, uint32_t ldo);
 oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                            const float* are, const float* aim, const double* xre,
                            const double* xim, double* yre, double* yim);
