@@ -252,6 +252,70 @@ __device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu, RotLog log)
   }
 }
 
+// A train of ns (<= waves of the block, <= 4) implicit single-shift QR
+// sweeps with shifts mu0..mu3 over rows/columns [lo, hi] of T (kdim <= 64),
+// accumulating Q: bulge w is chased by wave w, three positions behind bulge
+// w - 1.  Every step has a row phase (each active wave computes its
+// rotation from the bulge column and rotates its row pair, lanes over the
+// columns) and a column phase (its column pair of T over rows <=
+// min(k + 2, hi) and of Q, lanes over the rows), separated by block
+// barriers.  Row pairs of different bulges are disjoint, and so are column
+// pairs; against ns consecutive chase() sweeps the only reordering is a row
+// rotation of bulge w before a column rotation of bulge w - 1 on the same
+// entries, and left and right products commute.  Same result as the ns
+// sweeps up to rounding, with ns rotations per step instead of one.
+__device__ void chase_train(Mat T, Mat Q, int kdim, int lo, int hi, int ns, cd mu0, cd mu1,
+                            cd mu2, cd mu3) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int L = hi - lo;
+  const int steps = L + 3 * (ns - 1);
+  const cd mu = wave == 0 ? mu0 : wave == 1 ? mu1 : wave == 2 ? mu2 : mu3;
+  for (int t = 0; t < steps; ++t) {
+    const int p = t - 3 * wave;
+    const bool act = wave < ns && p >= 0 && p < L;
+    const int k = lo + p;
+    double c = 1.0;
+    cd s = {0.0, 0.0};
+    if (act) {
+      const int j = lane, j0 = k > lo ? k - 1 : k;
+      const bool mine = j >= j0 && j < kdim;
+      cd a = {0.0, 0.0}, b = {0.0, 0.0};
+      if (mine) {
+        a = T.get(k, j);
+        b = T.get(k + 1, j);
+      }
+      cd f, g;
+      if (k == lo) {
+        f = csub(T.get(lo, lo), mu);
+        g = T.get(lo + 1, lo);
+      } else {
+        f = T.get(k, k - 1);
+        g = T.get(k + 1, k - 1);
+      }
+      cgivens(f, g, c, s);
+      if (mine) {
+        T.set(k, j, grow_top(a, b, c, s));
+        T.set(k + 1, j, (k > lo && j == k - 1) ? cd{0.0, 0.0} : grow_bot(a, b, c, s));
+      }
+    }
+    __syncthreads();
+    if (act) {
+      const int r = lane, rmax = k + 2 < hi ? k + 2 : hi;
+      if (r <= rmax) {
+        const cd x = T.get(r, k), y = T.get(r, k + 1);
+        T.set(r, k, gcol_left(x, y, c, s));
+        T.set(r, k + 1, gcol_right(x, y, c, s));
+      }
+      if (r < kdim) {
+        const cd x = Q.get(r, k), y = Q.get(r, k + 1);
+        Q.set(r, k, gcol_left(x, y, c, s));
+        Q.set(r, k + 1, gcol_right(x, y, c, s));
+      }
+    }
+    __syncthreads();
+  }
+}
+
 // Bit i (1 <= i < kdim) set when the subdiagonal entry (i, i-1) is
 // negligible: |t_{i,i-1}| <= eps (|t_{i-1,i-1}| + |t_{i,i}|).  Lane i tests
 // entry i; every wave computes the same mask.
@@ -286,6 +350,28 @@ __device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu, RotLog log) {
   }
 }
 
+// ns (<= 4) exact-shift sweeps as one train per unreduced block (blocks
+// split at the negligible subdiagonals found before the train).
+__device__ void shifted_train(Mat T, Mat Q, int kdim, int ns, const cd* mus) {
+  const unsigned long long mask = negligible_mask(T, kdim);
+  __syncthreads();
+  {
+    const int i = threadIdx.x;
+    if (i < 64 && ((mask >> i) & 1ull)) T.set(i, i - 1, {0, 0});
+  }
+  __syncthreads();
+  const cd z = {0.0, 0.0};
+  const cd m0 = mus[0], m1 = ns > 1 ? mus[1] : z, m2 = ns > 2 ? mus[2] : z,
+           m3 = ns > 3 ? mus[3] : z;
+  int lo = 0;
+  while (lo < kdim) {
+    const unsigned long long above = lo >= 63 ? 0ull : mask & ~((2ull << lo) - 1ull);
+    const int hi = above ? __builtin_ctzll(above) - 1 : kdim - 1;
+    if (hi > lo) chase_train(T, Q, kdim, lo, hi, ns, m0, m1, m2, m3);
+    lo = hi + 1;
+  }
+}
+
 __device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
   int ihi = mm - 1, iter = 0, total = 0;
   while (ihi >= 0) {
@@ -302,11 +388,14 @@ __device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
       continue;
     }
     if (++total > 100 * mm) return -1;
-    cd mu;
+    cd mu, mu2 = {0.0, 0.0};
+    bool two = false;
     if (iter > 0 && iter % 10 == 0) {
       cd d = T.get(ihi, ihi), sd = T.get(ihi, ihi - 1);
       mu = {d.re + 0.75 * fabs(sd.re), d.im};
     } else {
+      // both eigenvalues of the trailing 2 x 2: the one nearer t[ihi][ihi]
+      // (Wilkinson) first; a window of 4 or more takes both as a train of two
       cd a = T.get(ihi - 1, ihi - 1), b = T.get(ihi - 1, ihi), c = T.get(ihi, ihi - 1),
          d = T.get(ihi, ihi);
       cd h = {0.5 * (a.re - d.re), 0.5 * (a.im - d.im)};
@@ -315,10 +404,16 @@ __device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
       double sr = sqrt(0.5 * (mod + q.re)), si = sqrt(0.5 * (mod - q.re));
       if (q.im < 0) si = -si;
       cd t1 = {h.re + sr, h.im + si}, t2 = {h.re - sr, h.im - si};
-      mu = cabs2(t1) < cabs2(t2) ? cadd(d, t1) : cadd(d, t2);
+      const bool near1 = cabs2(t1) < cabs2(t2);
+      mu = near1 ? cadd(d, t1) : cadd(d, t2);
+      mu2 = near1 ? cadd(d, t2) : cadd(d, t1);
+      two = ihi - l >= 3 && blockDim.x >= 128;
     }
     ++iter;
-    chase(T, Z, mm, l, ihi, mu, log);
+    if (two)
+      chase_train(T, Z, mm, l, ihi, 2, mu, mu2, mu2, mu2);
+    else
+      chase(T, Z, mm, l, ihi, mu, log);
   }
   return 0;
 }
@@ -494,7 +589,19 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
     Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
   }
   __syncthreads();
-  for (int t = kk; t < mm; ++t) shifted_sweep(T, Z, mm, ev[order[t]], log);
+  // the p = mm - kk unwanted values, in trains of up to four (one per wave)
+  const int nw = nthr / 64 < 4 ? nthr / 64 : 4;
+  __shared__ cd mus_s[kMaxM];
+  for (int t = kk + lane; t < mm; t += nthr) mus_s[t - kk] = ev[order[t]];
+  __syncthreads();
+  if (nw >= 2) {
+    for (int t0 = 0; t0 < mm - kk; t0 += nw) {
+      const int ns = mm - kk - t0 < nw ? mm - kk - t0 : nw;
+      shifted_train(T, Z, mm, ns, mus_s + t0);
+    }
+  } else {
+    for (int t = kk; t < mm; ++t) shifted_sweep(T, Z, mm, ev[order[t]], log);
+  }
   __syncthreads();
   if (lane == 0) st->clk[5] = clock64();
   for (int idx = lane; idx < mm * mm; idx += nthr) {
