@@ -100,6 +100,52 @@ oaph_status oaph_matrix_set_value(oaph_matrix* m, uint32_t index, double re, dou
 oaph_status oaph_matrix_set_zero_row(oaph_matrix* m, uint32_t row, int re, int im);
 
 /* ------------------------------------------------------------------ */
+/* matrix formats (host code; device matrices are copied once)         */
+/* ------------------------------------------------------------------ */
+/* Text, as oap::host::ToString prints it (oapMatrixCpu/oapHostMatrixUtils.cpp
+ * :576-588 -> oapMatrix/MatrixPrinter.h:28-112 with the default PrintArgs,
+ * oapMatrix/MatrixUtils.h:104-176): "[v, v, ...|\n v ...]" per plane, the im
+ * plane after " + i * ", fixed notation with 9 digits, runs of equal values
+ * as "v <repeats n times>".  *len receives the length without the NUL; buf
+ * may be NULL to query it, else cap must exceed it.  A NULL matrix prints
+ * "nullptr". */
+oaph_status oaph_matrix_to_string(const oaph_matrix* m, char* buf, size_t cap, size_t* len);
+oaph_status oaph_host_matrix_to_string(uint32_t columns, uint32_t rows, const double* re,
+                                       const double* im, char* buf, size_t cap, size_t* len);
+/* oap::host::NewMatrix(text) (oapMatrixCpu/oapHostMatrixUtils.cpp:223-274,
+ * oapMatrix/MatrixParser.cpp): "columns=<c>" and "rows=<r>" anywhere in the
+ * text (else a square of side floor(sqrt(n)) of the first array's length
+ * n), the first "[...]" is the re plane and the second the im plane,
+ * elements split at ',' or '|', "v <repeats n times>" expands; values past
+ * c x r are ignored.  A text without an array -> OAPH_ERR_ARG; an array
+ * shorter than c x r -> OAPH_ERR_SHAPE (the reference reads past it). */
+oaph_status oaph_matrix_parse(const char* text, oaph_matrix** out);
+/* the same into host arrays (re/im may be NULL; cap = elements available) */
+oaph_status oaph_host_parse_matrix(const char* text, oaph_matrix_info* info, double* re,
+                                   double* im, size_t cap);
+/* matrixUtils::CreateArray(text, which) (MatrixParser.cpp:199-210) */
+oaph_status oaph_host_parse_array(const char* text, uint32_t which, double* out, size_t cap,
+                                  size_t* len);
+/* Binary record of oap::cuda::SaveMatrix / oap::host::SaveMatrix
+ * (oapMatrixCuda/oapCudaMatrixUtils.cpp:562-589, oapMatrixCpu/oapHostMatrixUtils
+ * .cpp:1043-1070): u8 isMatrix, u8 isRe, u8 isIm, u32 columns, u32 rows, the
+ * re doubles, the im doubles (little endian, unaligned); a NULL matrix is
+ * the single byte 0.  *len receives the record size (buf NULL: query). */
+oaph_status oaph_matrix_save(const oaph_matrix* m, uint8_t* buf, size_t cap, size_t* len);
+oaph_status oaph_host_matrix_save(int present, uint32_t columns, uint32_t rows, const double* re,
+                                  const double* im, uint8_t* buf, size_t cap, size_t* len);
+/* LoadMatrix (oapMatrixCuda/oapCudaMatrixUtils.cpp:591-603): one record into
+ * a new device matrix (*out = NULL for a NULL record); *consumed = bytes used */
+oaph_status oaph_matrix_load(oaph_matrix** out, const uint8_t* buf, size_t len, size_t* consumed);
+/* the same into host arrays (re/im may be NULL: info only) */
+oaph_status oaph_host_matrix_load(const uint8_t* buf, size_t len, size_t* consumed, int* present,
+                                  oaph_matrix_info* info, double* re, double* im, size_t cap);
+/* utils::ByteBuffer::fwrite / fread (oapUtils/Buffer.h): size_t 1, size_t
+ * length, the bytes.  read: buf NULL queries *len. */
+oaph_status oaph_buffer_write_file(const char* path, const uint8_t* buf, size_t len);
+oaph_status oaph_buffer_read_file(const char* path, uint8_t* buf, size_t cap, size_t* len);
+
+/* ------------------------------------------------------------------ */
 /* procedures API: oap::CuProceduresApi (oapMatrixCuda/CuProceduresApi.h:45-309) */
 /* ------------------------------------------------------------------ */
 typedef struct oaph_api oaph_api;

@@ -165,6 +165,24 @@ _SIGS = {
                                        U32]),
     "oaph_mlp_train": (C.c_int, [C.c_void_p, U32, U32, C.c_float, C.c_uint64,
                                  C.POINTER(C.c_float)]),
+    "oaph_matrix_to_string": (C.c_int, [MP, C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "oaph_host_matrix_to_string": (C.c_int, [U32, U32, DP, DP, C.c_char_p, C.c_size_t,
+                                             C.POINTER(C.c_size_t)]),
+    "oaph_matrix_parse": (C.c_int, [C.c_char_p, C.POINTER(MP)]),
+    "oaph_host_parse_matrix": (C.c_int, [C.c_char_p, C.POINTER(oaph_matrix_info), DP, DP,
+                                         C.c_size_t]),
+    "oaph_host_parse_array": (C.c_int, [C.c_char_p, U32, DP, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "oaph_matrix_save": (C.c_int, [MP, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "oaph_host_matrix_save": (C.c_int, [C.c_int, U32, U32, DP, DP, C.c_void_p, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]),
+    "oaph_matrix_load": (C.c_int, [C.POINTER(MP), C.c_void_p, C.c_size_t,
+                                   C.POINTER(C.c_size_t)]),
+    "oaph_host_matrix_load": (C.c_int, [C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t),
+                                        C.POINTER(C.c_int), C.POINTER(oaph_matrix_info), DP, DP,
+                                        C.c_size_t]),
+    "oaph_buffer_write_file": (C.c_int, [C.c_char_p, C.c_void_p, C.c_size_t]),
+    "oaph_buffer_read_file": (C.c_int, [C.c_char_p, C.c_void_p, C.c_size_t,
+                                        C.POINTER(C.c_size_t)]),
     "oaph_rts_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, MP]),
     "oaph_rts_destroy": (C.c_int, [C.c_void_p]),
     "oaph_rts_info": (C.c_int, [C.c_void_p, C.POINTER(oaph_matrix_info)]),

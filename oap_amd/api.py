@@ -36,6 +36,13 @@ class DeviceMatrix:
 
     # ---- construction helpers
     @classmethod
+    def _adopt(cls, p) -> "DeviceMatrix":
+        """Takes ownership of a library-allocated matrix (MP)."""
+        m = cls.__new__(cls)
+        m._p = p
+        return m
+
+    @classmethod
     def from_numpy(cls, re: np.ndarray | None, im: np.ndarray | None = None) -> "DeviceMatrix":
         """re/im are (rows, columns) arrays; either may be None (absent plane)."""
         ref = re if re is not None else im

@@ -181,7 +181,7 @@ __global__ void k_normalize(uint32_t n, const double* fre, const double* fim, do
 
 // The small dense machinery (Schur form, eigenvectors, exact shifts) lives in
 // small_dense.h, shared with the batched solver.
-__global__ __launch_bounds__(256) void k_restart(RestartArgs a) {
+__global__ __launch_bounds__(512) void k_restart(RestartArgs a) {
   extern __shared__ double smem[];
   restart_body(a, smem);
 }
@@ -382,7 +382,7 @@ oaph_status restart(hipStream_t s, const RestartArgs& a) {
                                  4 * kMaxM * lds_ld(kMaxM) * (int)sizeof(double)));
     attr_set = true;
   }
-  hipLaunchKernelGGL(k_restart, dim3(1), dim3(256), lds, s, a);
+  hipLaunchKernelGGL(k_restart, dim3(1), dim3(512), lds, s, a);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

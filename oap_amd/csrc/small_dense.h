@@ -252,30 +252,28 @@ __device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu, RotLog log)
   }
 }
 
-// A train of ns (<= waves of the block, <= 4) implicit single-shift QR
-// sweeps with shifts mu0..mu3 over rows/columns [lo, hi] of T (kdim <= 64),
-// accumulating Q: bulge w is chased by wave w, three positions behind bulge
-// w - 1.  Every step has a row phase (each active wave computes its
-// rotation from the bulge column and rotates its row pair, lanes over the
-// columns) and a column phase (its column pair of T over rows <=
-// min(k + 2, hi) and of Q, lanes over the rows), separated by block
+// A train of ns (<= waves of the block) implicit single-shift QR sweeps over
+// rows/columns [lo, hi] of T (kdim <= 64), accumulating Q: bulge w, with
+// shift mu (each thread passes the shift of its own wave), is chased by wave
+// w, three positions behind bulge w - 1.  Every step has a row phase (each
+// active wave computes its rotation from the bulge column and rotates its row
+// pair, lanes over the columns) and a column phase (its column pair of T over
+// rows <= min(k + 2, hi) and of Q, lanes over the rows), separated by block
 // barriers.  Row pairs of different bulges are disjoint, and so are column
 // pairs; against ns consecutive chase() sweeps the only reordering is a row
 // rotation of bulge w before a column rotation of bulge w - 1 on the same
 // entries, and left and right products commute.  Same result as the ns
 // sweeps up to rounding, with ns rotations per step instead of one.
-__device__ void chase_train(Mat T, Mat Q, int kdim, int lo, int hi, int ns, cd mu0, cd mu1,
-                            cd mu2, cd mu3) {
+__device__ void chase_train(Mat T, Mat Q, int kdim, int lo, int hi, int ns, cd mu) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int L = hi - lo;
   const int steps = L + 3 * (ns - 1);
-  const cd mu = wave == 0 ? mu0 : wave == 1 ? mu1 : wave == 2 ? mu2 : mu3;
   for (int t = 0; t < steps; ++t) {
     const int p = t - 3 * wave;
     const bool act = wave < ns && p >= 0 && p < L;
     const int k = lo + p;
     double c = 1.0;
-    cd s = {0.0, 0.0};
+    cd s = {0.0, 0.0}, qx = {0.0, 0.0}, qy = {0.0, 0.0};
     if (act) {
       const int j = lane, j0 = k > lo ? k - 1 : k;
       const bool mine = j >= j0 && j < kdim;
@@ -284,14 +282,15 @@ __device__ void chase_train(Mat T, Mat Q, int kdim, int lo, int hi, int ns, cd m
         a = T.get(k, j);
         b = T.get(k + 1, j);
       }
-      cd f, g;
-      if (k == lo) {
-        f = csub(T.get(lo, lo), mu);
-        g = T.get(lo + 1, lo);
-      } else {
-        f = T.get(k, k - 1);
-        g = T.get(k + 1, k - 1);
+      // Q's column pair is written only by this wave's column phases
+      if (j < kdim) {
+        qx = Q.get(j, k);
+        qy = Q.get(j, k + 1);
       }
+      // (f, g) = column j0 of the row pair: the bulge column, or the shifted
+      // leading column at the top of the block
+      cd f = lane_read(a, j0), g = lane_read(b, j0);
+      if (k == lo) f = csub(f, mu);
       cgivens(f, g, c, s);
       if (mine) {
         T.set(k, j, grow_top(a, b, c, s));
@@ -307,9 +306,8 @@ __device__ void chase_train(Mat T, Mat Q, int kdim, int lo, int hi, int ns, cd m
         T.set(r, k + 1, gcol_right(x, y, c, s));
       }
       if (r < kdim) {
-        const cd x = Q.get(r, k), y = Q.get(r, k + 1);
-        Q.set(r, k, gcol_left(x, y, c, s));
-        Q.set(r, k + 1, gcol_right(x, y, c, s));
+        Q.set(r, k, gcol_left(qx, qy, c, s));
+        Q.set(r, k + 1, gcol_right(qx, qy, c, s));
       }
     }
     __syncthreads();
@@ -350,7 +348,7 @@ __device__ void shifted_sweep(Mat T, Mat Q, int kdim, cd mu, RotLog log) {
   }
 }
 
-// ns (<= 4) exact-shift sweeps as one train per unreduced block (blocks
+// ns (<= waves) exact-shift sweeps as one train per unreduced block (blocks
 // split at the negligible subdiagonals found before the train).
 __device__ void shifted_train(Mat T, Mat Q, int kdim, int ns, const cd* mus) {
   const unsigned long long mask = negligible_mask(T, kdim);
@@ -360,14 +358,13 @@ __device__ void shifted_train(Mat T, Mat Q, int kdim, int ns, const cd* mus) {
     if (i < 64 && ((mask >> i) & 1ull)) T.set(i, i - 1, {0, 0});
   }
   __syncthreads();
-  const cd z = {0.0, 0.0};
-  const cd m0 = mus[0], m1 = ns > 1 ? mus[1] : z, m2 = ns > 2 ? mus[2] : z,
-           m3 = ns > 3 ? mus[3] : z;
+  const int wave = threadIdx.x >> 6;
+  const cd mu = mus[wave < ns ? wave : 0];
   int lo = 0;
   while (lo < kdim) {
     const unsigned long long above = lo >= 63 ? 0ull : mask & ~((2ull << lo) - 1ull);
     const int hi = above ? __builtin_ctzll(above) - 1 : kdim - 1;
-    if (hi > lo) chase_train(T, Q, kdim, lo, hi, ns, m0, m1, m2, m3);
+    if (hi > lo) chase_train(T, Q, kdim, lo, hi, ns, mu);
     lo = hi + 1;
   }
 }
@@ -411,7 +408,7 @@ __device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
     }
     ++iter;
     if (two)
-      chase_train(T, Z, mm, l, ihi, 2, mu, mu2, mu2, mu2);
+      chase_train(T, Z, mm, l, ihi, 2, (threadIdx.x >> 6) == 0 ? mu : mu2);
     else
       chase(T, Z, mm, l, ihi, mu, log);
   }
@@ -589,8 +586,8 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
     Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
   }
   __syncthreads();
-  // the p = mm - kk unwanted values, in trains of up to four (one per wave)
-  const int nw = nthr / 64 < 4 ? nthr / 64 : 4;
+  // the p = mm - kk unwanted values, in trains of up to one per wave
+  const int nw = nthr / 64;
   __shared__ cd mus_s[kMaxM];
   for (int t = kk + lane; t < mm; t += nthr) mus_s[t - kk] = ev[order[t]];
   __syncthreads();

@@ -296,6 +296,62 @@ class Ref:
     def _args(p: Planes):
         return [int(p.re is not None), int(p.im is not None), p.cols, p.rows, _p(p.re), _p(p.im)]
 
+    # ---- matrix formats (the reference's own ToString / NewMatrix / SaveMatrix)
+    def to_string(self, a: Planes) -> str:
+        f = self.lib.ref_to_string
+        f.restype = C.c_size_t
+        n = f(*self._args(a), None, C.c_size_t(0))
+        buf = C.create_string_buffer(n + 1)
+        f(*self._args(a), buf, C.c_size_t(n + 1))
+        return buf.value.decode()
+
+    def parse_matrix(self, text: str, cap: int = 1 << 20):
+        """NewMatrix(text): (columns, rows, re | None, im | None) or None."""
+        isre, isim, c, r = C.c_int(), C.c_int(), U32(), U32()
+        re, im = np.zeros(cap), np.zeros(cap)
+        ok = self.lib.ref_parse_matrix(text.encode(), C.byref(isre), C.byref(isim), C.byref(c),
+                                       C.byref(r), _p(re), _p(im), C.c_size_t(cap))
+        if not ok:
+            return None
+        n = c.value * r.value
+        assert n <= cap
+        shape = (r.value, c.value)
+        return (c.value, r.value, re[:n].reshape(shape) if isre.value else None,
+                im[:n].reshape(shape) if isim.value else None)
+
+    def create_array(self, text: str, which: int, cap: int = 1 << 20):
+        f = self.lib.ref_create_array
+        f.restype = C.c_long
+        out = np.zeros(cap)
+        n = f(text.encode(), C.c_uint(which), _p(out), C.c_size_t(cap))
+        return None if n < 0 else out[:n].copy()
+
+    def save_matrix(self, a: Planes | None) -> bytes:
+        f = self.lib.ref_save_matrix
+        f.restype = C.c_size_t
+        args = [1, 0, 0, 0, 0, None, None] if a is None else [0] + self._args(a)
+        n = f(*args, None, C.c_size_t(0))
+        buf = (C.c_uint8 * n)()
+        f(*args, buf, C.c_size_t(n))
+        return bytes(buf)
+
+    def load_matrix(self, data: bytes, cap: int = 1 << 20):
+        isre, isim, c, r = C.c_int(), C.c_int(), U32(), U32()
+        re, im = np.zeros(cap), np.zeros(cap)
+        buf = (C.c_uint8 * len(data)).from_buffer_copy(data)
+        ok = self.lib.ref_load_matrix(buf, C.c_size_t(len(data)), C.byref(isre), C.byref(isim),
+                                      C.byref(c), C.byref(r), _p(re), _p(im), C.c_size_t(cap))
+        if not ok:
+            return None
+        n = c.value * r.value
+        shape = (r.value, c.value)
+        return (c.value, r.value, re[:n].reshape(shape) if isre.value else None,
+                im[:n].reshape(shape) if isim.value else None)
+
+    def buffer_fwrite(self, path: str, data: bytes) -> None:
+        buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data + b"\0")
+        self.lib.ref_buffer_fwrite(str(path).encode(), buf, C.c_size_t(len(data)))
+
     def set_threads_blocks(self, w, h, limit=1024):
         b, t = (U32 * 2)(), (U32 * 2)()
         self.lib.ref_set_threads_blocks(b, t, U32(w), U32(h), U32(limit))

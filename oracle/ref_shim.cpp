@@ -15,8 +15,12 @@
 #include <cstring>
 #include <functional>
 
+#include <string>
+
+#include "ByteBuffer.h"
 #include "CuMatrixProcedures.h"
 #include "HostKernel.h"
+#include "MatrixParser.h"
 #include "HostProcedures.h"
 #include "ThreadsMapper.h"
 #include "oapHostMatrixUtils.h"
@@ -203,6 +207,79 @@ int ref_is_upper_triangular(M_ARGS(a)) {
   int res = CUDA_isUpperTriangular(A);
   take(A, nullptr, nullptr);
   return res;
+}
+
+// ---- matrix formats: oap::host::ToString, NewMatrix(text), CreateArray,
+// SaveMatrix / LoadMatrix into utils::ByteBuffer and its file framing ----
+size_t ref_to_string(M_ARGS(a), char* buf, size_t cap) {
+  math::Matrix* A = M_MAKE(a);
+  std::string s;
+  oap::host::ToString(s, A);
+  take(A, nullptr, nullptr);
+  if (buf && cap > s.size()) {
+    std::memcpy(buf, s.data(), s.size());
+    buf[s.size()] = 0;
+  }
+  return s.size();
+}
+
+// returns 0 when the text gives no matrix; else writes info and planes (cap elements)
+int ref_parse_matrix(const char* text, int* isre, int* isim, uint32_t* c, uint32_t* r, double* re,
+                     double* im, size_t cap) {
+  math::Matrix* m = oap::host::NewMatrix(std::string(text));
+  if (!m) return 0;
+  *isre = m->reValues != nullptr;
+  *isim = m->imValues != nullptr;
+  *c = m->columns;
+  *r = m->rows;
+  size_t n = (size_t)m->columns * m->rows;
+  if (n <= cap) take(m, re, im);
+  else oap::host::DeleteMatrix(m);
+  return 1;
+}
+
+// length of CreateArray(text, which), -1 when it fails; values into out (cap)
+long ref_create_array(const char* text, unsigned which, double* out, size_t cap) {
+  std::pair<floatt*, size_t> p = matrixUtils::CreateArray(std::string(text), which);
+  if (!p.first) return -1;
+  if (p.second <= cap) std::memcpy(out, p.first, p.second * sizeof(double));
+  delete[] p.first;
+  return (long)p.second;
+}
+
+// SaveMatrix of a host matrix (is_null: of nullptr) into a ByteBuffer
+size_t ref_save_matrix(int is_null, M_ARGS(a), uint8_t* buf, size_t cap) {
+  utils::ByteBuffer b;
+  math::Matrix* A = is_null ? nullptr : M_MAKE(a);
+  oap::host::SaveMatrix(A, b);
+  if (A) take(A, nullptr, nullptr);
+  size_t n = b.getUsedLength();
+  if (buf && n <= cap) std::memcpy(buf, b.m_buffer, n);
+  return n;
+}
+
+// LoadMatrix from bytes; returns 0 for a null record
+int ref_load_matrix(const uint8_t* bytes, size_t len, int* isre, int* isim, uint32_t* c,
+                    uint32_t* r, double* re, double* im, size_t cap) {
+  utils::ByteBuffer b;
+  b.push_back(reinterpret_cast<const char*>(bytes), len);
+  math::Matrix* m = oap::host::LoadMatrix(b);
+  if (!m) return 0;
+  *isre = m->reValues != nullptr;
+  *isim = m->imValues != nullptr;
+  *c = m->columns;
+  *r = m->rows;
+  size_t n = (size_t)m->columns * m->rows;
+  if (n <= cap) take(m, re, im);
+  else oap::host::DeleteMatrix(m);
+  return 1;
+}
+
+// ByteBuffer::fwrite of the given bytes
+void ref_buffer_fwrite(const char* path, const uint8_t* bytes, size_t len) {
+  utils::ByteBuffer b;
+  b.push_back(reinterpret_cast<const char*>(bytes), len);
+  b.fwrite(std::string(path));
 }
 
 }  // extern "C"
