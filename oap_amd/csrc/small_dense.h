@@ -39,17 +39,21 @@ __device__ __forceinline__ void bsync() {
 // convention).  Reciprocal square roots on the common path; hypot-based
 // scaling when a modulus is near the under/overflow thresholds.
 __device__ void cgivens_safe(cd f, cd g, double& c, cd& s);
+// One reciprocal square root on the common path: with t = 1 / sqrt(|f|^2 r^2),
+// r^2 = |f|^2 + |g|^2, c = |f|^2 t = |f| / r and s = f conj(g) t, which keeps
+// the dependent chain short (FP64 results feed the next instruction only
+// after tens of cycles on one wave).
 __device__ __forceinline__ void cgivens(cd f, cd g, double& c, cd& s) {
   const double af2 = f.re * f.re + f.im * f.im, ag2 = g.re * g.re + g.im * g.im;
+  const double pr = f.re * g.re + f.im * g.im, pi = f.im * g.re - f.re * g.im;  // f conj(g)
   if (ag2 == 0.0) { c = 1; s = {0, 0}; return; }
-  if (!(af2 > 1e-280 && ag2 > 1e-280 && af2 < 1e280 && ag2 < 1e280)) {
+  if (!(af2 > 1e-150 && ag2 > 1e-150 && af2 < 1e150 && ag2 < 1e150)) {
     cgivens_safe(f, g, c, s);
     return;
   }
-  const double inv_r = rsqrt(af2 + ag2), inv_af = rsqrt(af2);
-  c = af2 * inv_af * inv_r;
-  const double ur = f.re * inv_af, ui = f.im * inv_af;
-  s = {(ur * g.re + ui * g.im) * inv_r, (ui * g.re - ur * g.im) * inv_r};
+  const double t = rsqrt(af2 * (af2 + ag2));
+  c = af2 * t;
+  s = {pr * t, pi * t};
 }
 __device__ void cgivens_safe(cd f, cd g, double& c, cd& s) {
   double af = hypot(f.re, f.im), ag = hypot(g.re, g.im);
