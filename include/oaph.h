@@ -307,6 +307,12 @@ oaph_status oaph_arnoldi_test_outcome(oaph_arnoldi* a, uint32_t index, double* o
 oaph_status oaph_arnoldi_set_seed(oaph_arnoldi* a, uint64_t seed);
 oaph_status oaph_arnoldi_set_norm_a(oaph_arnoldi* a, double norm_a);
 oaph_status oaph_arnoldi_set_max_restarts(oaph_arnoldi* a, uint32_t max_restarts);
+/* OAPH_PREC_MIXED_FP32 (dense operator only): the first execute after
+ * oaph_arnoldi_set_matrix makes an FP32 copy of the operator; call set_matrix
+ * again after changing the operator's values.  When every entry is an FP32
+ * value the copy IS the operator and all cycles use it (no FP64 restart);
+ * otherwise FP32 cycles to the mixed tolerance, then an FP64 restart from the
+ * sum of the wanted Ritz vectors.  Residuals are explicit FP64 either way. */
 oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* a, int precision);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
  * the largest Ritz value (default 1e-6) */

@@ -61,6 +61,7 @@ struct oaph_arnoldi {
   DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32;
   size_t a32_elems = 0;
   const oaph_matrix* a32_of = nullptr;
+  bool a32_exact = false;  // every operator entry is an FP32 value
   Status host_status{};
   std::vector<double> ev_re, ev_im, resid;
   oaph_arnoldi_stats stats{};
@@ -122,8 +123,8 @@ oaph_status apply_operator(oaph_arnoldi* s, double* xre, double* xim, double* yr
     const oaph_matrix* A = s->A;
     if (low) {
       const float* a32 = s->A32.as<float>();
-      OAPH_TRY(launch_cgemv32(st, s->n, s->n, A->columns, a32, a32 + s->a32_elems, xre, xim, yre,
-                              yim));
+      OAPH_TRY(launch_cgemv32(st, s->n, s->n, A->columns, a32,
+                              A->imValues ? a32 + s->a32_elems : nullptr, xre, xim, yre, yim));
       s->stats.matvecs_low++;
     } else {
       OAPH_TRY(launch_zgemv(st, s->n, s->n, A->columns, A->reValues, A->imValues, xre, xim, yre,
@@ -148,6 +149,25 @@ oaph_status read_status(oaph_arnoldi* s) {
 }
 
 // One Arnoldi step: column j of H and v_{j+1} (CGS2; ArnoldiProcedures.cpp:416-459)
+// The FP32 copy of the dense operator (both planes) and whether it is exact.
+oaph_status make_fp32_copy(oaph_arnoldi* s) {
+  hipStream_t st = s->api->stream;
+  const size_t elems = (size_t)s->A->rows * s->A->columns;
+  OAPH_TRY(s->A32.reserve((s->A->imValues ? 2 : 1) * elems * sizeof(float)));
+  s->a32_elems = elems;
+  float* a32 = s->A32.as<float>();
+  unsigned* flag = s->counter.as<unsigned>() + 48;
+  OAPH_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned), st));
+  OAPH_TRY(to_f32(st, elems, s->A->reValues, a32, flag));
+  if (s->A->imValues) OAPH_TRY(to_f32(st, elems, s->A->imValues, a32 + elems, flag));
+  unsigned inexact = 1;
+  OAPH_HIP(hipMemcpyAsync(&inexact, flag, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  OAPH_HIP(hipStreamSynchronize(st));
+  s->a32_exact = inexact == 0;
+  s->a32_of = s->A;
+  return OAPH_OK;
+}
+
 oaph_status arnoldi_step(oaph_arnoldi* s, uint32_t j, bool low) {
   hipStream_t st = s->api->stream;
   Work w = work_of(s);
@@ -547,21 +567,16 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     s->norm_a = norm;
   }
   const bool mixed = s->precision == OAPH_PREC_MIXED_FP32 && s->A != nullptr;
-  if (mixed && s->a32_of != s->A) {
-    size_t elems = (size_t)s->n * s->A->columns;
-    OAPH_TRY(s->A32.reserve(2 * elems * sizeof(float)));
-    s->a32_elems = elems;
-    float* a32 = s->A32.as<float>();
-    OAPH_TRY(to_f32(st, elems, s->A->reValues, a32));
-    if (s->A->imValues) OAPH_TRY(to_f32(st, elems, s->A->imValues, a32 + elems));
-    else OAPH_HIP(hipMemsetAsync(a32 + elems, 0, elems * sizeof(float), st));
-    s->a32_of = s->A;
-  }
+  if (mixed && s->a32_of != s->A) OAPH_TRY(make_fp32_copy(s));
   OAPH_TRY(start_vector(s));
   OAPH_TRY(normalize_v0(s));
   uint32_t cycles = 0;
   bool converged = false;
-  if (mixed) {
+  // an operator whose entries are all FP32 values is the FP32 copy itself:
+  // the FP32-operand matvec (FP64 accumulation) is then as accurate as the
+  // FP64 one and every cycle runs on it, with no FP64 restart
+  const bool all_low = mixed && s->a32_exact;
+  if (mixed && !all_low) {
     // FP32 phase (tolerance relative to max |Ritz value|)
     OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
     OAPH_TRY(ritz_vectors(s, s->m));
@@ -573,7 +588,7 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     OAPH_HIP(hipMemsetAsync(s->H.ptr, 0, 2 * (size_t)s->m * s->m * sizeof(double), st));
   }
   uint32_t c2 = 0;
-  OAPH_TRY(run_cycles(s, false, s->tol, norm, &c2, &converged));
+  OAPH_TRY(run_cycles(s, all_low, s->tol, norm, &c2, &converged));
   int bd = s->host_status.breakdown;
   uint32_t mact = bd > 0 ? (uint32_t)bd : s->m;
   OAPH_TRY(finish(s, mact));
@@ -605,16 +620,7 @@ oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* s, int low, uint32_t iters, d
     s->k = s->k ? s->k : 1;
     OAPH_TRY(alloc(s));
   }
-  if (low && s->a32_of != s->A) {
-    size_t elems = (size_t)n * s->A->columns;
-    OAPH_TRY(s->A32.reserve(2 * elems * sizeof(float)));
-    s->a32_elems = elems;
-    float* a32 = s->A32.as<float>();
-    OAPH_TRY(to_f32(st, elems, s->A->reValues, a32));
-    if (s->A->imValues) OAPH_TRY(to_f32(st, elems, s->A->imValues, a32 + elems));
-    else OAPH_HIP(hipMemsetAsync(a32 + elems, 0, elems * sizeof(float), st));
-    s->a32_of = s->A;
-  }
+  if (low && s->a32_of != s->A) OAPH_TRY(make_fp32_copy(s));
   double* xre = Vre(s, s->V, 0);
   double* xim = Vim(s, s->V, 0);
   double* yre = s->W.as<double>();

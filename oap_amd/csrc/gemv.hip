@@ -166,7 +166,8 @@ __global__ __launch_bounds__(256) void k_zgemv_scalar(uint32_t rows, uint32_t co
 // FP64, products and sums in FP64: the only rounding beyond FP64 is the
 // storage of A.  4 elements (16 B) per lane per load.
 typedef float v4f __attribute__((ext_vector_type(4)));
-template <int R, int U>
+// CPLX = false: real A (aim unused), complex x -- half the operator bytes
+template <int R, int U, bool CPLX = true>
 __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, uint32_t lda,
                                                  const float* __restrict__ are,
                                                  const float* __restrict__ aim,
@@ -190,7 +191,7 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
   for (int q = 0; q < R; ++q) {
     uint32_t r = r0 + q < rows ? r0 + q : rows - 1;
     ar4[q] = reinterpret_cast<const v4f*>(are + (size_t)r * lda);
-    ai4[q] = reinterpret_cast<const v4f*>(aim + (size_t)r * lda);
+    ai4[q] = CPLX ? reinterpret_cast<const v4f*>(aim + (size_t)r * lda) : nullptr;
   }
   for (uint32_t p = lane; p < nquad; p += 64 * U) {
     v2d xa[U][2], xb[U][2];
@@ -210,7 +211,7 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
         uint32_t pp = p + 64 * u;
         if (pp < nquad) {
           a[q][u] = __builtin_nontemporal_load(ar4[q] + pp);
-          b[q][u] = __builtin_nontemporal_load(ai4[q] + pp);
+          if (CPLX) b[q][u] = __builtin_nontemporal_load(ai4[q] + pp);
         }
       }
 #pragma unroll
@@ -219,15 +220,20 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
       for (int u = 0; u < U; ++u) {
         if (p + 64 * u < nquad) {
           double a0 = a[q][u].x, a1 = a[q][u].y, a2 = a[q][u].z, a3 = a[q][u].w;
-          double b0 = b[q][u].x, b1 = b[q][u].y, b2 = b[q][u].z, b3 = b[q][u].w;
           sr[q] = fma(a0, xa[u][0].x, sr[q]); sr[q] = fma(a1, xa[u][0].y, sr[q]);
           sr[q] = fma(a2, xa[u][1].x, sr[q]); sr[q] = fma(a3, xa[u][1].y, sr[q]);
-          sr[q] = fma(-b0, xb[u][0].x, sr[q]); sr[q] = fma(-b1, xb[u][0].y, sr[q]);
-          sr[q] = fma(-b2, xb[u][1].x, sr[q]); sr[q] = fma(-b3, xb[u][1].y, sr[q]);
+          if (CPLX) {
+            double b0 = b[q][u].x, b1 = b[q][u].y, b2 = b[q][u].z, b3 = b[q][u].w;
+            sr[q] = fma(-b0, xb[u][0].x, sr[q]); sr[q] = fma(-b1, xb[u][0].y, sr[q]);
+            sr[q] = fma(-b2, xb[u][1].x, sr[q]); sr[q] = fma(-b3, xb[u][1].y, sr[q]);
+          }
           si[q] = fma(a0, xb[u][0].x, si[q]); si[q] = fma(a1, xb[u][0].y, si[q]);
           si[q] = fma(a2, xb[u][1].x, si[q]); si[q] = fma(a3, xb[u][1].y, si[q]);
-          si[q] = fma(b0, xa[u][0].x, si[q]); si[q] = fma(b1, xa[u][0].y, si[q]);
-          si[q] = fma(b2, xa[u][1].x, si[q]); si[q] = fma(b3, xa[u][1].y, si[q]);
+          if (CPLX) {
+            double b0 = b[q][u].x, b1 = b[q][u].y, b2 = b[q][u].z, b3 = b[q][u].w;
+            si[q] = fma(b0, xa[u][0].x, si[q]); si[q] = fma(b1, xa[u][0].y, si[q]);
+            si[q] = fma(b2, xa[u][1].x, si[q]); si[q] = fma(b3, xa[u][1].y, si[q]);
+          }
         }
       }
   }
@@ -362,6 +368,9 @@ constexpr ZFn kZ[] = {k_zgemv<4, 2, MODE>, k_zgemv<2, 4, MODE>, k_zgemv<4, 4, MO
                       k_zgemv<8, 2, MODE>, k_zgemv<6, 2, MODE>};
 constexpr CFn kC[] = {k_cgemv32<4, 3>, k_cgemv32<2, 4>, k_cgemv32<4, 2>, k_cgemv32<4, 4>,
                       k_cgemv32<8, 1>, k_cgemv32<8, 2>, k_cgemv32<6, 2>, k_cgemv32<16, 1>};
+constexpr CFn kCR[] = {k_cgemv32<4, 3, false>, k_cgemv32<2, 4, false>, k_cgemv32<4, 2, false>,
+                       k_cgemv32<4, 4, false>, k_cgemv32<8, 1, false>, k_cgemv32<8, 2, false>,
+                       k_cgemv32<6, 2, false>, k_cgemv32<16, 1, false>};
 
 }  // namespace
 
@@ -398,11 +407,12 @@ oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   OAPH_CHECK(cols % 4 == 0 && lda % 4 == 0, OAPH_ERR_UNSUPPORTED);
   OAPH_CHECK(((((uintptr_t)are) | ((uintptr_t)aim) | ((uintptr_t)xre) | ((uintptr_t)xim)) & 15) == 0,
              OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(xim != nullptr && yim != nullptr, OAPH_ERR_PLANE);
   const int R = kCShapes[g_cshape].R;
   const unsigned waves = (rows + R - 1) / R;
   const dim3 grid((waves + 3) / 4);
-  hipLaunchKernelGGL(kC[g_cshape], grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre,
-                     yim);
+  hipLaunchKernelGGL(aim ? kC[g_cshape] : kCR[g_cshape], grid, dim3(256), 0, s, rows, cols, lda,
+                     are, aim, xre, xim, yre, yim);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

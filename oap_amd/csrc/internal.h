@@ -96,6 +96,7 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
 oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, const double* are,
                            const double* aim, uint32_t lda, const double* bre, const double* bim,
                            uint32_t ldb, double* ore, double* oim, uint32_t ldo);
+// FP32 operand (aim NULL: real operator), complex FP64 x and y
 oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                            const float* are, const float* aim, const double* xre,
                            const double* xim, double* yre, double* yim);

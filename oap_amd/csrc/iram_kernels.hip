@@ -296,10 +296,19 @@ __global__ __launch_bounds__(256) void k_pairs(uint32_t n, uint32_t k, const dou
 }
 
 // FP64 -> FP32 copy of the operator planes (mixed-precision matvec)
-__global__ void k_to_f32(size_t n, const double* __restrict__ src, float* __restrict__ dst) {
+// dst = (float)src; *inexact is set to 1 (plain stores, all alike) when some
+// element is not representable in FP32
+__global__ void k_to_f32(size_t n, const double* __restrict__ src, float* __restrict__ dst,
+                         unsigned* inexact) {
+  bool bad = false;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
-       i += (size_t)gridDim.x * blockDim.x)
-    dst[i] = (float)src[i];
+       i += (size_t)gridDim.x * blockDim.x) {
+    const double v = src[i];
+    const float f = (float)v;
+    dst[i] = f;
+    bad |= (double)f != v;
+  }
+  if (bad && inexact) *inexact = 1u;
 }
 
 // x = sum_t X[:, t] (start vector of the polishing phase), written to V[:, 0]
@@ -416,10 +425,10 @@ size_t pairs_partials_bytes(uint32_t n, uint32_t k) {
   return (size_t)((n + 255) / 256) * k * 3 * sizeof(double);
 }
 
-oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst) {
+oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsigned* inexact) {
   size_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, n, src, dst);
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, n, src, dst, inexact);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

@@ -39,7 +39,7 @@ from pathlib import Path
 import numpy as np
 
 from ._lib import check, load
-from .api import (MLP, ArnUtils, CuHArnoldiCallback, DeviceMatrix, MatrixInfo, QRType)
+from .api import (MLP, ArnUtils, CuHArnoldiCallback, DeviceMatrix, MatrixInfo, Precision, QRType)
 
 GRID = 64
 N = GRID * GRID
@@ -156,6 +156,8 @@ class CdrProblem:
 
     def set_params(self, pe: float, c: float, alpha: float) -> None:
         check(self.lib.oaph_synth_cdr(self.A.ptr, GRID, pe, c, alpha), "synth_cdr")
+        # new values: the solver's FP32 copy of the operator is remade
+        self.solver.setMatrix(self.A)
         from .api import CuProceduresApi
         self.fro = CuProceduresApi().magnitude(self.A)
 
@@ -246,18 +248,28 @@ def generate(params: np.ndarray, seed: int = 1, log=None):
     return np.array(keep), np.array(rows, dtype=np.float32), np.array(mvs)
 
 
-def compare(model: LearnedStart, params: np.ndarray, seed: int = 7):
-    """Each operator solved from the random start and from the learned start
-    (same random seed for the fallback)."""
+def compare(model: LearnedStart, params: np.ndarray, seed: int = 7, mixed: bool = False):
+    """Each operator solved by the plain FP64 solver from the random start and
+    from the learned start (same random seed for the fallback); with `mixed`
+    also from the learned start with the FP32 operand (the CDR operator's
+    entries are FP32 values, so the FP32 copy is the operator itself)."""
     prob = CdrProblem()
     out = []
     for i, (pe, c, a) in enumerate(params):
         prob.set_params(pe, c, a)
+        prob.solver.setPrecision(Precision.FP64)
         plain = prob.solve(None, seed=seed + i)
         v0, ev_hat = model.start_vector(pe, c, a)
         learned = prob.solve(v0, seed=seed + i)
         learned.predicted = ev_hat
-        out.append((plain, learned))
+        row = [plain, learned]
+        if mixed:
+            prob.solver.setPrecision(Precision.MIXED_FP32)
+            lm = prob.solve(v0, seed=seed + i)
+            lm.predicted = ev_hat
+            row.append(lm)
+            prob.solver.setPrecision(Precision.FP64)
+        out.append(tuple(row))
     return out
 
 

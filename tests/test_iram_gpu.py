@@ -108,6 +108,27 @@ def test_mixed_precision_reaches_fp64_residual(oracle):
     assert s.stats()["matvecs_low"] > 0
 
 
+def test_mixed_with_fp32_valued_operator_runs_all_cycles_in_fp32(oracle, rng):
+    # entries that are FP32 values: the FP32 copy is the operator, so every
+    # cycle uses the FP32-operand matvec and no FP64 restart follows
+    n = 768
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = (ar.astype(np.float32).astype(np.float64) +
+         1j * ai.astype(np.float32).astype(np.float64))
+    s, ev, vecs = run_dense(A, 8, 40, precision=Precision.MIXED_FP32)
+    st = s.stats()
+    assert st["matvecs_low"] == st["matvecs"] - 8  # the 8 residual products are FP64
+    want = top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues)
+    match(ev, want, 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+    # one entry that is not an FP32 value: FP32 phase, then the FP64 restart
+    A[3, 5] += 1e-12
+    s, ev, vecs = run_dense(A, 8, 40, precision=Precision.MIXED_FP32)
+    st = s.stats()
+    assert 0 < st["matvecs_low"] < st["matvecs"] - 8
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+
+
 def test_callback_mode_matches_dense(oracle):
     n = 512
     ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
