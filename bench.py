@@ -146,7 +146,11 @@ def batched_c3(steps: int) -> dict:
     ldx = (n.astype(np.int64) + 3) // 4 * 4
     xbytes = float(np.sum(res["matvecs"].astype(np.int64) * s * ldx * 4))
     kmed = float(np.median(kms))
-    return {"value": round(good / wall, 1), "unit": "eigenpairs/s", "pairs_per_solve": 512 * k,
+    plain = plain_c3(b, n, s, m, k)
+    batched_value = good / wall
+    return {"value": round(batched_value, 1), "unit": "eigenpairs/s", "pairs_per_solve": 512 * k,
+            "plain_fp64_one_problem_at_a_time": plain,
+            "ratio_batched_over_plain": round(batched_value / plain["value"], 1),
             "ms_per_solve": round(wall / steps * 1e3, 3), "kernel_ms": round(kmed, 3),
             "variant": {"threads": shapes[variant][0], "rows_per_wave_step": shapes[variant][1]},
             "matvecs_mean": round(float(np.mean(res["matvecs"])), 1),
@@ -158,6 +162,48 @@ def batched_c3(steps: int) -> dict:
                          "bytes": xbytes,
                          "note": "X streamed once per implicit X^T X v / s; whole solve "
                                  "(restarts included) in the kernel time"}}
+
+
+def plain_c3(b, n, s: int, m: int, k: int, sample: int = 16) -> dict:
+    """The same problems solved one at a time by the plain FP64 solver
+    (CuHArnoldiCallback, multiply callback = the implicit X^T X v / s of
+    RecToSquareApi), on `sample` problems spread over the size range;
+    eigenpairs/s = converged pairs / summed solve time."""
+    from oap_amd.api import (ArnUtils, CuHArnoldiCallback, CuProceduresApi, DeviceMatrix,
+                             MatrixInfo, QRType, RecToSquareApi)
+    order = np.argsort(n)
+    pick = order[np.linspace(0, len(n) - 1, sample).astype(int)]
+    good, secs, mv = 0, 0.0, []
+    for t, i in enumerate(pick):
+        X = b.download(int(i)).astype(np.float64)       # s x n_i
+        M = DeviceMatrix.from_numpy(np.ascontiguousarray(X.T) / np.sqrt(s), None)  # n_i x s
+        sol = CuHArnoldiCallback()
+        sol.setQRType(QRType.QRGR)
+        sol.setSortType(ArnUtils.SortLargestValues)
+        sol.setCheckTolerance(TOL)
+        sol.setOutputType(ArnUtils.DEVICE)
+        sol.setSeed(0x5EED5EED + t)
+        rts = RecToSquareApi(sol.api, M)
+
+        def multiply(w, v, api, user, mt):
+            user.multiply(w, v, api)
+
+        sol.setCallback(multiply, rts)
+        re_, im_ = np.zeros(k), np.zeros(k)
+        sol.setOutputsEigenvalues(re_, im_)
+        ni = int(n[i])
+        if t == 0:  # warm-up of this path
+            sol.execute(m, k, MatrixInfo(True, False, ni, ni))
+        t0 = time.perf_counter()
+        sol.execute(m, k, MatrixInfo(True, False, ni, ni))
+        secs += time.perf_counter() - t0
+        lmax = float(np.max(np.abs(re_ + 1j * im_)))
+        good += sum(sol.testOutcome(q) <= TOL * lmax for q in range(k))
+        mv.append(sol.stats()["matvecs"])
+        del rts, sol, M
+    return {"value": round(good / secs, 1), "unit": "eigenpairs/s", "problems": int(sample),
+            "ms_per_problem": round(secs / sample * 1e3, 3), "matvecs_mean": round(float(np.mean(mv)), 1),
+            "sizes": [int(n[i]) for i in pick]}
 
 
 def learned_c4(count: int) -> dict:
