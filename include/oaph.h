@@ -491,6 +491,46 @@ oaph_status oaph_rts_row_vector(oaph_rts* t, uint32_t index, oaph_matrix* out);
 oaph_status oaph_rts_multiply(oaph_rts* t, oaph_api* api, oaph_matrix* w, const oaph_matrix* v);
 
 /* ------------------------------------------------------------------ */
+/* PCA data path of the oap2dt3d application (host code): PNG images ->  */
+/* cropped pixel columns -> the data matrix that RecToSquareApi squares. */
+/* ------------------------------------------------------------------ */
+/* 8-bit non-interlaced PNG, colour types grey / RGB / palette (raw index
+ * bytes) / grey+alpha / RGBA: raw rows (height x width*colors bytes), as
+ * PngFile reads them (oapAppUtils/PngFile.cpp:107-140).  pixels NULL: size
+ * query.  Decoder of this library's own (zlib). */
+oaph_status oaph_png_read(const char* path, uint32_t* width, uint32_t* height, uint32_t* colors,
+                          uint8_t* pixels, size_t cap);
+/* writes such rows as a PNG (colors 1, 2, 3, 4), for synthetic image stacks */
+oaph_status oaph_png_write(const char* path, uint32_t width, uint32_t height, uint32_t colors,
+                           const uint8_t* pixels);
+/* oap::GetOptWidth / GetOptHeight (oapAppUtils/GraphicsUtilsImpl.h):
+ * out = {width, begin column, height, begin row} of the non-constant window */
+oaph_status oaph_image_opt_size(const uint8_t* bitmap, uint32_t width, uint32_t height,
+                                uint32_t colors, uint32_t out[4]);
+/* DataLoader's load process (oapAppUtils/DataLoader.cpp:162-230) on given
+ * windows sizes[4i..4i+3] = {w, wbegin, h, hbegin}: out = the windows used,
+ * loads (may be NULL) = bitmap loads the reference performs per image */
+oaph_status oaph_dataloader_plan(const uint32_t* sizes, uint32_t count, uint32_t* out,
+                                 uint32_t* loads);
+typedef struct oaph_dataloader oaph_dataloader;
+/* DataLoader(images) over PNG files (truncate: crop to the optimal window,
+ * PngFile's default); every image gets one column of pixel / 0xFFFFFF
+ * values (Image.cpp:134-178).  Windows that do not fit an image or differ
+ * in area -> OAPH_ERR_SHAPE (the reference reads past the image, or throws
+ * NotIdenticalLengths). */
+oaph_status oaph_dataloader_create(oaph_dataloader** out, const char* const* paths,
+                                   uint32_t count, int truncate);
+oaph_status oaph_dataloader_destroy(oaph_dataloader* d);
+/* getMatrixInfo: columns = images, rows = window pixels, re plane only */
+oaph_status oaph_dataloader_info(const oaph_dataloader* d, oaph_matrix_info* info);
+oaph_status oaph_dataloader_window(const oaph_dataloader* d, uint32_t image, uint32_t out[4],
+                                   uint32_t* loads);
+/* createColumnVector(index) into host memory (rows values) */
+oaph_status oaph_dataloader_column(const oaph_dataloader* d, uint32_t image, double* out);
+/* createMatrix() into a device matrix of images columns x pixels rows */
+oaph_status oaph_dataloader_matrix(const oaph_dataloader* d, oaph_matrix* m);
+
+/* ------------------------------------------------------------------ */
 /* Batched covariance eigenproblems.  Problem i is C_i = X_i^T X_i / s with
  * X_i an s x n_i FP32 row-major sample matrix: the M M^T operator the
  * reference forms with RecToSquareApi / SquareMatrixApi

@@ -183,6 +183,18 @@ _SIGS = {
     "oaph_buffer_write_file": (C.c_int, [C.c_char_p, C.c_void_p, C.c_size_t]),
     "oaph_buffer_read_file": (C.c_int, [C.c_char_p, C.c_void_p, C.c_size_t,
                                         C.POINTER(C.c_size_t)]),
+    "oaph_png_read": (C.c_int, [C.c_char_p, C.POINTER(U32), C.POINTER(U32), C.POINTER(U32),
+                                C.c_void_p, C.c_size_t]),
+    "oaph_png_write": (C.c_int, [C.c_char_p, U32, U32, U32, C.c_void_p]),
+    "oaph_image_opt_size": (C.c_int, [C.c_void_p, U32, U32, U32, C.POINTER(U32)]),
+    "oaph_dataloader_plan": (C.c_int, [C.POINTER(U32), U32, C.POINTER(U32), C.POINTER(U32)]),
+    "oaph_dataloader_create": (C.c_int, [C.POINTER(C.c_void_p), C.POINTER(C.c_char_p), U32,
+                                         C.c_int]),
+    "oaph_dataloader_destroy": (C.c_int, [C.c_void_p]),
+    "oaph_dataloader_info": (C.c_int, [C.c_void_p, C.POINTER(oaph_matrix_info)]),
+    "oaph_dataloader_window": (C.c_int, [C.c_void_p, U32, C.POINTER(U32), C.POINTER(U32)]),
+    "oaph_dataloader_column": (C.c_int, [C.c_void_p, U32, DP]),
+    "oaph_dataloader_matrix": (C.c_int, [C.c_void_p, MP]),
     "oaph_rts_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, MP]),
     "oaph_rts_destroy": (C.c_int, [C.c_void_p]),
     "oaph_rts_info": (C.c_int, [C.c_void_p, C.POINTER(oaph_matrix_info)]),

@@ -58,7 +58,7 @@ def build(verbose: bool = True) -> Path:
         objs = list(ex.map(lambda s: _compile(s, hipcc), sources))
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs),
-           "-lpthread"]
+           "-lpthread", "-lz"]
     res = subprocess.run(cmd, capture_output=True, text=True)
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")

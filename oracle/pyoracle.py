@@ -348,6 +348,14 @@ class Ref:
         return (c.value, r.value, re[:n].reshape(shape) if isre.value else None,
                 im[:n].reshape(shape) if isim.value else None)
 
+    def opt_size(self, bitmap: np.ndarray):
+        a = np.ascontiguousarray(bitmap, dtype=np.uint8)
+        h, w = a.shape[:2]
+        c = 1 if a.ndim == 2 else a.shape[2]
+        out = (U32 * 4)()
+        self.lib.ref_opt_size(a.ctypes.data_as(C.c_void_p), U32(w), U32(h), U32(c), out)
+        return tuple(out)
+
     def buffer_fwrite(self, path: str, data: bytes) -> None:
         buf = (C.c_uint8 * max(1, len(data))).from_buffer_copy(data + b"\0")
         self.lib.ref_buffer_fwrite(str(path).encode(), buf, C.c_size_t(len(data)))

@@ -18,7 +18,10 @@
 #include <string>
 
 #include "ByteBuffer.h"
+#include <vector>
+
 #include "CuMatrixProcedures.h"
+#include "GraphicUtils.h"
 #include "HostKernel.h"
 #include "MatrixParser.h"
 #include "HostProcedures.h"
@@ -273,6 +276,18 @@ int ref_load_matrix(const uint8_t* bytes, size_t len, int* isre, int* isim, uint
   if (n <= cap) take(m, re, im);
   else oap::host::DeleteMatrix(m);
   return 1;
+}
+
+// oap::GetOptWidth / GetOptHeight over a row-major byte bitmap
+void ref_opt_size(const uint8_t* bm, uint32_t w, uint32_t h, uint32_t colors, uint32_t* out) {
+  std::vector<unsigned char*> rows(h);
+  for (uint32_t y = 0; y < h; ++y) rows[y] = const_cast<unsigned char*>(bm + (size_t)y * w * colors);
+  oap::OptSize ow = oap::GetOptWidth<unsigned char**, unsigned char>(rows.data(), w, h, colors);
+  oap::OptSize oh = oap::GetOptHeight<unsigned char**, unsigned char>(rows.data(), w, h, colors);
+  out[0] = (uint32_t)ow.optSize;
+  out[1] = (uint32_t)ow.begin;
+  out[2] = (uint32_t)oh.optSize;
+  out[3] = (uint32_t)oh.begin;
 }
 
 // ByteBuffer::fwrite of the given bytes
