@@ -16,8 +16,8 @@ from dataclasses import dataclass
 import numpy as np
 
 from ._lib import DP, check, load, oaph_matrix_info
-from .api import (ArnUtils, CuHArnoldiCallback, CuProceduresApi, DeviceMatrix, MatrixInfo,
-                  QRType, RecToSquareApi)
+from .api import (ArnUtils, CuHArnoldiCallback, DeviceMatrix, MatrixInfo, QRType,
+                  RecToSquareApi)
 
 U32 = C.c_uint32
 
@@ -165,6 +165,11 @@ class EigenCalculator:
         s.setOutputsEigenvectors(vecs)
         s.execute(self.eigens, self.wanted, MatrixInfo(True, False, n, n))
         errors = np.array([s.testOutcome(t) for t in range(self.wanted)])
-        vectors = [v.to_complex()[:, 0].real for v in vecs]
+        # S is real symmetric: remove the Ritz vector's arbitrary complex phase
+        vectors = []
+        for v in vecs:
+            z = v.to_complex()[:, 0]
+            p = z[np.argmax(np.abs(z))]
+            vectors.append((z * (np.conj(p) / abs(p))).real if p != 0 else z.real)
         self._keep = (M, rts, s)
         return Outcome(re.copy(), errors, vectors)
