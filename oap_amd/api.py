@@ -435,6 +435,12 @@ class CuHArnoldiCallback:
     def setPrecision(self, p):
         check(self._lib.oaph_arnoldi_set_precision(self._h, p), "setPrecision")
 
+    def setRestartStages(self, tols):
+        """restart from the sum of the wanted Ritz vectors at each tolerance"""
+        t = np.ascontiguousarray(tols, dtype=np.float64)
+        check(self._lib.oaph_arnoldi_set_restart_stages(self._h, _dptr(t) if t.size else None,
+                                                         t.size), "setRestartStages")
+
     def setMixedTolerance(self, tol_low):
         check(self._lib.oaph_arnoldi_set_mixed_tolerance(self._h, tol_low), "set_mixed_tolerance")
 

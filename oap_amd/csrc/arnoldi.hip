@@ -46,6 +46,10 @@ struct oaph_arnoldi {
   uint32_t max_restarts = 1000;
   int precision = OAPH_PREC_FP64;
   double tol_low = 1e-6;  // FP32 phase tolerance, relative to max |Ritz value|
+  // tolerances (relative to max |Ritz value|, descending) at which the solve
+  // restarts from the sum of the wanted Ritz vectors before going on
+  std::vector<double> stage_tols;
+  bool stages_set = false;  // else the mixed flow uses {1e-2, 1e-4}
   oaph_multiply_fn mfn = nullptr;
   void* muser = nullptr;
   oaph_check_fn cfn = nullptr;
@@ -473,6 +477,17 @@ oaph_status oaph_arnoldi_set_norm_a(oaph_arnoldi* s, double v) {
   s->norm_a_user = v > 0;
   return OAPH_OK;
 }
+oaph_status oaph_arnoldi_set_restart_stages(oaph_arnoldi* s, const double* tols, uint32_t n) {
+  OAPH_CHECK(s && (n == 0 || tols), OAPH_ERR_ARG);
+  for (uint32_t i = 0; i < n; ++i) {
+    OAPH_CHECK(tols[i] > 0 && tols[i] < 1, OAPH_ERR_ARG);
+    if (i) OAPH_CHECK(tols[i] < tols[i - 1], OAPH_ERR_ARG);
+  }
+  s->stage_tols.assign(tols, tols + n);
+  s->stages_set = true;
+  return OAPH_OK;
+}
+
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* s, double tol_low) {
   OAPH_CHECK(s && tol_low > 0 && tol_low < 1, OAPH_ERR_ARG);
   s->tol_low = tol_low;
@@ -576,9 +591,8 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
   // the FP32-operand matvec (FP64 accumulation) is then as accurate as the
   // FP64 one and every cycle runs on it, with no FP64 restart
   const bool all_low = mixed && s->a32_exact;
-  if (mixed && !all_low) {
-    // FP32 phase (tolerance relative to max |Ritz value|)
-    OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
+  // restart from the sum of the wanted Ritz vectors (a fresh Arnoldi start)
+  auto restart_from_ritz_sum = [&]() -> oaph_status {
     OAPH_TRY(ritz_vectors(s, s->m));
     double* Xre = s->X.as<double>();
     OAPH_TRY(colsum(st, s->n, s->k, Xre, Xre + (size_t)s->k * s->n, Vre(s, s->V, 0),
@@ -586,6 +600,29 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     OAPH_TRY(normalize_v0(s));
     OAPH_HIP(hipMemsetAsync(s->status.ptr, 0, sizeof(Status), st));
     OAPH_HIP(hipMemsetAsync(s->H.ptr, 0, 2 * (size_t)s->m * s->m * sizeof(double), st));
+    return OAPH_OK;
+  };
+  // stages of the low-precision (or, in FP64 mode, the FP64) cycles.  The
+  // mixed flow restarts from the Ritz-vector sum at 1e-2 and 1e-4 unless the
+  // caller chose stages: measured on the dense complex 8192 problem, 72.5 vs
+  // 48.5 eigenpairs/s over 12 start vectors (60.0 vs 54.1 on 12 others, 98.4
+  // vs 93.5 on another matrix), never slower; plain FP64 keeps the
+  // reference's single cycle sequence unless stages are set.
+  static const double kMixedStages[] = {1e-2, 1e-4};
+  const bool stage_low = mixed;
+  std::vector<double> stages = s->stage_tols;
+  if (!s->stages_set && mixed && !all_low) stages.assign(kMixedStages, kMixedStages + 2);
+  for (double t : stages) {
+    if (mixed && !all_low && t <= s->tol_low) break;
+    uint32_t c = 0;
+    OAPH_TRY(run_cycles(s, stage_low, t, 0.0, &c, &converged));
+    cycles += c;
+    OAPH_TRY(restart_from_ritz_sum());
+  }
+  if (mixed && !all_low) {
+    // FP32 phase (tolerance relative to max |Ritz value|)
+    OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
+    OAPH_TRY(restart_from_ritz_sum());
   }
   uint32_t c2 = 0;
   OAPH_TRY(run_cycles(s, all_low, s->tol, norm, &c2, &converged));

@@ -129,6 +129,30 @@ def test_mixed_with_fp32_valued_operator_runs_all_cycles_in_fp32(oracle, rng):
     match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
 
 
+def test_restart_stages(oracle):
+    # restarts from the sum of the wanted Ritz vectors at 1e-2 and 1e-4: the
+    # same eigenpairs to the same tolerance
+    n = 1024
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    dA = DeviceMatrix.from_numpy(A.real, A.imag)
+    s = solver()
+    s.setMatrix(dA)
+    s.setRestartStages([1e-2, 1e-4])
+    re, im = np.zeros(8), np.zeros(8)
+    s.setOutputsEigenvalues(re, im)
+    vecs = [DeviceMatrix(1, n) for _ in range(8)]
+    s.setOutputsEigenvectors(vecs)
+    s.execute(40, 8, MatrixInfo(True, True, n, n))
+    ev = re + 1j * im
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+    with pytest.raises(OapError):
+        s.setRestartStages([1e-4, 1e-2])  # not descending
+    with pytest.raises(OapError):
+        s.setRestartStages([2.0])
+
+
 def test_callback_mode_matches_dense(oracle):
     n = 512
     ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
