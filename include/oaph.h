@@ -361,11 +361,13 @@ oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1
  * coefficient rounded to FP32.  A must be grid^2 x grid^2. */
 oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, double alpha);
 /* Launch shape of the streaming matvec (fp32 != 0: the FP32-operand form used
- * by OAPH_PREC_MIXED_FP32): variant index into a table of (rows per wave,
- * loads in flight); process-wide.  No reference counterpart (tuning knob). */
+ * by OAPH_PREC_MIXED_FP32): variant index into a table of (rows per wave read
+ * together, loads in flight per row and plane, groups of such rows per wave
+ * taken one after another); process-wide.  No reference counterpart (tuning
+ * knob). */
 oaph_status oaph_set_matvec_shape(int fp32, int variant);
 oaph_status oaph_matvec_shape_info(int fp32, int variant, int* rows_per_wave,
-                                   int* loads_in_flight);
+                                   int* loads_in_flight, int* row_groups);
 /* Average duration of `iters` back-to-back dotProduct calls (HIP events on
  * the api's stream), after one warm-up call. */
 oaph_status oaph_time_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,

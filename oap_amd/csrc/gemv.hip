@@ -29,7 +29,8 @@ __device__ __forceinline__ double wave_sum(double v) {
 // ---------------------------------------------------------------- complex FP64 GEMV
 // y[r] = sum_c A[r][c] x[c] for r < rows.  Requires cols even and 16-byte
 // aligned rows (lda even); the launcher falls back to k_zgemv_scalar otherwise.
-template <int R, int U, int MODE>
+// G groups of R rows per wave, one group after another.
+template <int R, int U, int MODE, int G = 1>
 __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uint32_t lda,
                                                const double* __restrict__ are,
                                                const double* __restrict__ aim,
@@ -39,11 +40,12 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
                                                double* __restrict__ yim) {
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t r0 = wave * R;
-  if (r0 >= rows) return;
   constexpr bool CPLX = MODE == 0;      // complex A (and x)
   constexpr bool XIM = MODE != 1;       // x has an imaginary plane
   const uint32_t npair = cols >> 1;
+  for (int g = 0; g < G; ++g) {
+  const uint32_t r0 = (wave * G + g) * R;
+  if (r0 >= rows) return;
   double sr[R], si[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) { sr[q] = 0.0; si[q] = 0.0; }
@@ -125,6 +127,7 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
     yre[r0 + lane] = vr;
     if (XIM) yim[r0 + lane] = vi;
   }
+  }
 }
 
 // any alignment: one wave per row, scalar loads
@@ -167,7 +170,8 @@ __global__ __launch_bounds__(256) void k_zgemv_scalar(uint32_t rows, uint32_t co
 // storage of A.  4 elements (16 B) per lane per load.
 typedef float v4f __attribute__((ext_vector_type(4)));
 // CPLX = false: real A (aim unused), complex x -- half the operator bytes
-template <int R, int U, bool CPLX = true>
+// G groups of R rows per wave, one group after another.
+template <int R, int U, bool CPLX = true, int G = 1>
 __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, uint32_t lda,
                                                  const float* __restrict__ are,
                                                  const float* __restrict__ aim,
@@ -177,9 +181,10 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
                                                  double* __restrict__ yim) {
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
-  const uint32_t r0 = wave * R;
-  if (r0 >= rows) return;
   const uint32_t nquad = cols >> 2;
+  for (int g = 0; g < G; ++g) {
+  const uint32_t r0 = (wave * G + g) * R;
+  if (r0 >= rows) return;
   double sr[R], si[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) { sr[q] = 0.0; si[q] = 0.0; }
@@ -249,6 +254,7 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
       if (lane == q) { vr = sr[q]; vi = si[q]; }
     yre[r0 + lane] = vr;
     yim[r0 + lane] = vi;
+  }
   }
 }
 
@@ -342,17 +348,22 @@ __global__ __launch_bounds__(256) void k_gemm(uint32_t W, uint32_t H, uint32_t K
 
 namespace {
 
-// Launch shapes of the streaming matvecs: R rows per wave, U loads in flight
-// per row and plane.  More rows per wave re-read x (from L2) less often per
-// byte of A; more loads in flight hide HBM latency.  Selected per process
-// with oaph_set_matvec_shape.  Entry 0 is the default, the fastest measured
-// on MI355X at n = 8192: FP64 (4, 2) 163 us = 6.59 TB/s, FP32 operand (4, 3)
-// 85 us = 6.35 TB/s (a 2 GiB read+write stream reaches 6.1 TB/s).
+// Launch shapes of the streaming matvecs: R rows per wave read together, U
+// loads in flight per row and plane, G groups of R rows per wave one after
+// another.  More rows per wave re-read x (from L2) less often per byte of A;
+// more loads in flight hide HBM latency; groups give fewer, longer-lived
+// waves (one per SIMD at n = 8192 with R G = 8).  Selected per process with
+// oaph_set_matvec_shape.  Entry 0 is the default, the fastest measured on
+// MI355X at n = 8192 (averages of 20 launches): FP64 (2, 3, 4) 160 us =
+// 6.71 TB/s vs (4, 2, 1) 162.5 us; FP32 operand (4, 3, 2) 80.5 us =
+// 6.68 TB/s vs (4, 3, 1) 84 us.
 struct Shape {
-  int R, U;
+  int R, U, G;
 };
-constexpr Shape kZShapes[] = {{4, 2}, {2, 4}, {4, 4}, {1, 8}, {8, 1}, {4, 1}, {8, 2}, {6, 2}};
-constexpr Shape kCShapes[] = {{4, 3}, {2, 4}, {4, 2}, {4, 4}, {8, 1}, {8, 2}, {6, 2}, {16, 1}};
+constexpr Shape kZShapes[] = {{2, 3, 4}, {4, 2, 1}, {2, 4, 4}, {4, 3, 2}, {2, 6, 4},
+                              {2, 4, 2}, {4, 6, 2}, {8, 2, 1}};
+constexpr Shape kCShapes[] = {{4, 3, 2}, {4, 3, 1}, {4, 2, 2}, {4, 4, 2}, {2, 2, 4},
+                              {2, 3, 4}, {8, 2, 1}, {4, 2, 1}};
 constexpr int kNumZ = sizeof(kZShapes) / sizeof(kZShapes[0]);
 constexpr int kNumC = sizeof(kCShapes) / sizeof(kCShapes[0]);
 int g_zshape = 0, g_cshape = 0;
@@ -362,15 +373,49 @@ using ZFn = void (*)(uint32_t, uint32_t, uint32_t, const double*, const double*,
 using CFn = void (*)(uint32_t, uint32_t, uint32_t, const float*, const float*, const double*,
                      const double*, double*, double*);
 
+// kZ / kC: each shape as listed; kZ1 / kC1: what runs when the grouped grid
+// would leave SIMDs without a wave -- the shape itself when G = 1, else the
+// fastest G = 1 shape (FP64 (4, 2), FP32 operand (4, 3)).
 template <int MODE>
-constexpr ZFn kZ[] = {k_zgemv<4, 2, MODE>, k_zgemv<2, 4, MODE>, k_zgemv<4, 4, MODE>,
-                      k_zgemv<1, 8, MODE>, k_zgemv<8, 1, MODE>, k_zgemv<4, 1, MODE>,
-                      k_zgemv<8, 2, MODE>, k_zgemv<6, 2, MODE>};
-constexpr CFn kC[] = {k_cgemv32<4, 3>, k_cgemv32<2, 4>, k_cgemv32<4, 2>, k_cgemv32<4, 4>,
-                      k_cgemv32<8, 1>, k_cgemv32<8, 2>, k_cgemv32<6, 2>, k_cgemv32<16, 1>};
-constexpr CFn kCR[] = {k_cgemv32<4, 3, false>, k_cgemv32<2, 4, false>, k_cgemv32<4, 2, false>,
-                       k_cgemv32<4, 4, false>, k_cgemv32<8, 1, false>, k_cgemv32<8, 2, false>,
-                       k_cgemv32<6, 2, false>, k_cgemv32<16, 1, false>};
+constexpr ZFn kZ[] = {k_zgemv<2, 3, MODE, 4>, k_zgemv<4, 2, MODE, 1>, k_zgemv<2, 4, MODE, 4>,
+                      k_zgemv<4, 3, MODE, 2>, k_zgemv<2, 6, MODE, 4>, k_zgemv<2, 4, MODE, 2>,
+                      k_zgemv<4, 6, MODE, 2>, k_zgemv<8, 2, MODE, 1>};
+template <int MODE>
+constexpr ZFn kZ1[] = {k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>,
+                       k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>,
+                       k_zgemv<4, 2, MODE, 1>, k_zgemv<8, 2, MODE, 1>};
+template <bool CPLX>
+constexpr CFn kC[] = {k_cgemv32<4, 3, CPLX, 2>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 2, CPLX, 2>,
+                      k_cgemv32<4, 4, CPLX, 2>, k_cgemv32<2, 2, CPLX, 4>, k_cgemv32<2, 3, CPLX, 4>,
+                      k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
+template <bool CPLX>
+constexpr CFn kC1[] = {k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
+                       k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
+                       k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
+static_assert(sizeof(kZ1<0>) / sizeof(ZFn) == kNumZ, "one kernel per FP64 shape");
+static_assert(sizeof(kC1<true>) / sizeof(CFn) == kNumC, "one kernel per FP32 shape");
+
+// Waves the device runs at once with one per SIMD (4 SIMDs per CU).
+unsigned simd_count() {
+  static unsigned n = [] {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    return 4u * (unsigned)cus;
+  }();
+  return n;
+}
+
+// Row groups pay off only while the grouped grid still gives every SIMD a
+// wave; smaller operators take the kZ1 / kC1 fallback (four rows per wave).
+bool use_groups(const Shape& sh, uint32_t rows) {
+  return rows / (uint32_t)(sh.R * sh.G) >= simd_count();
+}
+
+static_assert(sizeof(kZ<0>) / sizeof(ZFn) == kNumZ, "one kernel per FP64 shape");
+static_assert(sizeof(kC<true>) / sizeof(CFn) == kNumC, "one kernel per FP32 shape");
 
 }  // namespace
 
@@ -383,10 +428,13 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
                       ((((uintptr_t)are) | ((uintptr_t)aim) | ((uintptr_t)xre) |
                         ((uintptr_t)xim)) & 15) == 0 && cols >= 128;
   if (vec_ok) {
-    const int R = kZShapes[g_zshape].R;
-    const unsigned waves = (rows + R - 1) / R;
+    const Shape& sh = kZShapes[g_zshape];
+    const bool grp = sh.G == 1 || use_groups(sh, rows);
+    const int RG = grp ? sh.R * sh.G : 4;
+    const unsigned waves = (rows + RG - 1) / RG;
     const dim3 grid((waves + 3) / 4);
-    const ZFn fn = mode == 0 ? kZ<0>[g_zshape] : mode == 1 ? kZ<1>[g_zshape] : kZ<2>[g_zshape];
+    const ZFn fn = grp ? (mode == 0 ? kZ<0>[g_zshape] : mode == 1 ? kZ<1>[g_zshape] : kZ<2>[g_zshape])
+                       : (mode == 0 ? kZ1<0>[g_zshape] : mode == 1 ? kZ1<1>[g_zshape] : kZ1<2>[g_zshape]);
     hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim);
   } else {
     dim3 grid((rows + 3) / 4);
@@ -408,10 +456,14 @@ oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   OAPH_CHECK(((((uintptr_t)are) | ((uintptr_t)aim) | ((uintptr_t)xre) | ((uintptr_t)xim)) & 15) == 0,
              OAPH_ERR_UNSUPPORTED);
   OAPH_CHECK(xim != nullptr && yim != nullptr, OAPH_ERR_PLANE);
-  const int R = kCShapes[g_cshape].R;
-  const unsigned waves = (rows + R - 1) / R;
+  const Shape& sh = kCShapes[g_cshape];
+  const bool grp = sh.G == 1 || use_groups(sh, rows);
+  const int RG = grp ? sh.R * sh.G : 4;
+  const unsigned waves = (rows + RG - 1) / RG;
   const dim3 grid((waves + 3) / 4);
-  hipLaunchKernelGGL(aim ? kC[g_cshape] : kCR[g_cshape], grid, dim3(256), 0, s, rows, cols, lda,
+  const CFn fn = grp ? (aim ? kC<true>[g_cshape] : kC<false>[g_cshape])
+                     : (aim ? kC1<true>[g_cshape] : kC1<false>[g_cshape]);
+  hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda,
                      are, aim, xre, xim, yre, yim);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
@@ -481,13 +533,15 @@ oaph_status oaph_set_matvec_shape(int fp32, int variant) {
   return OAPH_OK;
 }
 
-oaph_status oaph_matvec_shape_info(int fp32, int variant, int* rows_per_wave, int* loads_in_flight) {
+oaph_status oaph_matvec_shape_info(int fp32, int variant, int* rows_per_wave, int* loads_in_flight,
+                                   int* row_groups) {
   using namespace oaph;
   const int n = fp32 ? kNumC : kNumZ;
   OAPH_CHECK(variant >= 0 && variant < n, OAPH_ERR_ARG);
   const Shape sh = fp32 ? kCShapes[variant] : kZShapes[variant];
   if (rows_per_wave) *rows_per_wave = sh.R;
   if (loads_in_flight) *loads_in_flight = sh.U;
+  if (row_groups) *row_groups = sh.G;
   return OAPH_OK;
 }
 

@@ -1,5 +1,5 @@
 """Every launch shape of the streaming matvecs (rows per wave x loads in
-flight) computes the product to FP64 accuracy (the shapes differ only in how
+flight x row groups per wave) computes the product to FP64 accuracy (the shapes differ only in how
 a lane's partial sums are grouped): dotProduct through each FP64 shape
 against the exact product within gamma_n sum |a||x|, and the mixed solver
 through each FP32-operand shape against the dense eigenvalues."""
@@ -17,10 +17,10 @@ pytestmark = pytest.mark.gpu
 def shapes(fp32):
     lib, out, v = load(), [], 0
     while True:
-        r, u = C.c_int(), C.c_int()
-        if lib.oaph_matvec_shape_info(fp32, v, C.byref(r), C.byref(u)) != 0:
+        r, u, g = C.c_int(), C.c_int(), C.c_int()
+        if lib.oaph_matvec_shape_info(fp32, v, C.byref(r), C.byref(u), C.byref(g)) != 0:
             return out
-        out.append((r.value, u.value))
+        out.append((r.value, u.value, g.value))
         v += 1
 
 
