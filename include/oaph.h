@@ -314,13 +314,20 @@ oaph_status oaph_arnoldi_set_max_restarts(oaph_arnoldi* a, uint32_t max_restarts
  * otherwise FP32 cycles to the mixed tolerance, then an FP64 restart from the
  * sum of the wanted Ritz vectors.  Residuals are explicit FP64 either way. */
 oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* a, int precision);
-/* Staged solve: at each tolerance tols[0] > tols[1] > ... (relative to the
- * largest |Ritz value|) the cycles stop and the solve restarts from the sum
- * of the wanted Ritz vectors (in the FP32 phase with OAPH_PREC_MIXED_FP32,
- * stages at or below the mixed tolerance ignored).  n = 0: no stages.
- * Default: none in FP64; {1e-2, 1e-4} in the mixed flow of an operator
- * that is not FP32-valued. */
+/* Staged solve (opt-in, default none): at each tolerance tols[0] > tols[1]
+ * > ... (relative to the largest |Ritz value|) the cycles stop and the solve
+ * restarts from the sum of the wanted (and guard) Ritz vectors (in the FP32
+ * phase with OAPH_PREC_MIXED_FP32, stages at or below the mixed tolerance
+ * ignored).  n = 0: no stages.  Such a restart discards every direction
+ * outside the current wanted set: when the wanted eigenvalues are close in
+ * the sort order to the next ones it can settle on a wrong set (see
+ * oaph_arnoldi_set_restart_guard).  No reference counterpart. */
 oaph_status oaph_arnoldi_set_restart_stages(oaph_arnoldi* a, const double* tols, uint32_t n);
+/* Ritz vectors after the k wanted ones (next in the wanted order) that a
+ * stage restart also adds into its sum, so that a wanted eigenvalue whose
+ * Ritz value is still ranked just below the k-th is kept in the new start
+ * vector (default 0; at most m - k are used).  No reference counterpart. */
+oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* a, uint32_t guard);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
  * the largest Ritz value (default 1e-6) */
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* a, double tol_low);

@@ -197,6 +197,7 @@ _SIGS = {
     "oaph_dataloader_column": (C.c_int, [C.c_void_p, U32, DP]),
     "oaph_dataloader_matrix": (C.c_int, [C.c_void_p, MP]),
     "oaph_arnoldi_set_restart_stages": (C.c_int, [C.c_void_p, DP, U32]),
+    "oaph_arnoldi_set_restart_guard": (C.c_int, [C.c_void_p, U32]),
     "oaph_rts_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_void_p, MP]),
     "oaph_rts_destroy": (C.c_int, [C.c_void_p]),
     "oaph_rts_info": (C.c_int, [C.c_void_p, C.POINTER(oaph_matrix_info)]),

@@ -436,10 +436,16 @@ class CuHArnoldiCallback:
         check(self._lib.oaph_arnoldi_set_precision(self._h, p), "setPrecision")
 
     def setRestartStages(self, tols):
-        """restart from the sum of the wanted Ritz vectors at each tolerance"""
+        """restart from the sum of the wanted (and guard) Ritz vectors at each
+        tolerance (opt-in: such restarts can settle on a wrong wanted set when
+        the k-th eigenvalue is close to the next)"""
         t = np.ascontiguousarray(tols, dtype=np.float64)
         check(self._lib.oaph_arnoldi_set_restart_stages(self._h, _dptr(t) if t.size else None,
                                                          t.size), "setRestartStages")
+
+    def setRestartGuard(self, guard):
+        """Ritz vectors after the wanted ones kept in each stage restart's sum"""
+        check(self._lib.oaph_arnoldi_set_restart_guard(self._h, guard), "setRestartGuard")
 
     def setMixedTolerance(self, tol_low):
         check(self._lib.oaph_arnoldi_set_mixed_tolerance(self._h, tol_low), "set_mixed_tolerance")

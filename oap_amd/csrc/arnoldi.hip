@@ -49,7 +49,8 @@ struct oaph_arnoldi {
   // tolerances (relative to max |Ritz value|, descending) at which the solve
   // restarts from the sum of the wanted Ritz vectors before going on
   std::vector<double> stage_tols;
-  bool stages_set = false;  // else the mixed flow uses {1e-2, 1e-4}
+  // Ritz vectors after the wanted ones that a stage restart keeps in its sum
+  uint32_t guard = 0;
   oaph_multiply_fn mfn = nullptr;
   void* muser = nullptr;
   oaph_check_fn cfn = nullptr;
@@ -61,6 +62,7 @@ struct oaph_arnoldi {
   std::vector<double> start_re, start_im;
   // state
   uint32_t n = 0, m = 0, k = 0;
+  uint32_t ky = 0;  // Ritz vectors kept in Y and X: k + guard, at most m
   bool cplx = true;
   DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32;
   size_t a32_elems = 0;
@@ -75,6 +77,7 @@ struct oaph_arnoldi {
 namespace {
 
 oaph_status alloc(oaph_arnoldi* s) {
+  s->ky = s->k + s->guard < s->m ? s->k + s->guard : s->m;
   const size_t n = s->n, m = s->m, k = s->k;
   OAPH_TRY(s->V.reserve(2 * (m + 1) * n * sizeof(double)));
   OAPH_TRY(s->V2.reserve(2 * (m + 1) * n * sizeof(double)));
@@ -82,8 +85,9 @@ oaph_status alloc(oaph_arnoldi* s) {
   OAPH_TRY(s->F.reserve(2 * n * sizeof(double)));
   OAPH_TRY(s->H.reserve(2 * m * m * sizeof(double)));
   OAPH_TRY(s->Q.reserve(4 * m * m * sizeof(double)));  // Q and the X work of the restart kernel
-  OAPH_TRY(s->Y.reserve(2 * m * k * sizeof(double)));
-  OAPH_TRY(s->X.reserve(2 * k * n * sizeof(double)));
+  const size_t ky = s->ky;
+  OAPH_TRY(s->Y.reserve(2 * m * ky * sizeof(double)));
+  OAPH_TRY(s->X.reserve(2 * ky * n * sizeof(double)));
   OAPH_TRY(s->AX.reserve(2 * k * n * sizeof(double)));
   OAPH_TRY(s->coef.reserve(6 * (m + 1) * sizeof(double2)));
   size_t pb = orth_partials_bytes(s->n, s->m);
@@ -183,11 +187,11 @@ oaph_status arnoldi_step(oaph_arnoldi* s, uint32_t j, bool low) {
   double2* c2 = c1 + (s->m + 1);
   OAPH_TRY(apply_operator(s, Vre(s, s->V, j), Vim(s, s->V, j), wre, wim, OAPH_TYPE_WV, low));
   const uint32_t nc = j + 1;
-  OAPH_TRY(orth(st, 0, w, nc, wre, wim, nullptr, nullptr, nullptr, nullptr, c1, 0, 0));
-  OAPH_TRY(orth(st, 1, w, nc, wre, wim, fre, fim, c1, nullptr, c2, 0, 0));
-  OAPH_TRY(orth(st, 2, w, nc, fre, fim, fre, fim, c2, c1, nullptr, (int)j, (int)j + 1));
-  if (j + 1 < s->m) OAPH_TRY(normalize(st, w, fre, fim, Vre(s, s->V, j + 1), Vim(s, s->V, j + 1)));
-  return OAPH_OK;
+  OAPH_TRY(cgs_project(st, w, nc, wre, wim));
+  OAPH_TRY(cgs_update_project(st, w, nc, wre, wim, fre, fim, c1));
+  const bool next = j + 1 < s->m;
+  return cgs_finish(st, w, nc, fre, fim, c1, c2, (int)j, (int)j + 1,
+                    next ? Vre(s, s->V, j + 1) : nullptr, next ? Vim(s, s->V, j + 1) : nullptr);
 }
 
 // orthonormalise F against V[:, 0..nc) and store it as column nc (breakdown restart)
@@ -201,11 +205,10 @@ oaph_status fresh_column(oaph_arnoldi* s, uint32_t nc, uint64_t seed) {
   double2* c1 = s->coef.as<double2>();
   double2* c2 = c1 + (s->m + 1);
   OAPH_TRY(random_vector(st, s->n, seed, true, wre, wim));
-  OAPH_TRY(orth(st, 0, w, nc, wre, wim, nullptr, nullptr, nullptr, nullptr, c1, 0, 0));
-  OAPH_TRY(orth(st, 1, w, nc, wre, wim, fre, fim, c1, nullptr, c2, 0, 0));
-  // H[nc][nc-1] stays 0: norm only (ncols = 0), written at hrow = m (nowhere)
-  OAPH_TRY(orth(st, 2, w, nc, fre, fim, fre, fim, c2, c1, nullptr, -1, (int)s->m));
-  return normalize(st, w, fre, fim, Vre(s, s->V, nc), Vim(s, s->V, nc));
+  // H[nc][nc-1] stays 0 (hcol = -1: no H writes, no breakdown flag)
+  OAPH_TRY(cgs_project(st, w, nc, wre, wim));
+  OAPH_TRY(cgs_update_project(st, w, nc, wre, wim, fre, fim, c1));
+  return cgs_finish(st, w, nc, fre, fim, c1, c2, -1, -1, Vre(s, s->V, nc), Vim(s, s->V, nc));
 }
 
 // Runs restart cycles from column `start` until convergence at `tol` (relative
@@ -226,11 +229,12 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     ra.Xre = ra.Qim + (size_t)s->m * s->m;
     ra.Xim = ra.Xre + (size_t)s->m * s->m;
     ra.Yre = s->Y.as<double>();
-    ra.Yim = ra.Yre + (size_t)s->m * s->k;
+    ra.Yim = ra.Yre + (size_t)s->m * s->ky;
     ra.st = s->status.as<Status>();
     ra.m = (int)s->m;
     ra.mact = (int)s->m;
     ra.k = (int)s->k;
+    ra.ky = (int)s->ky;
     ra.which = s->sort;
     ra.do_shifts = 1;
     ra.tol = tol;
@@ -286,8 +290,7 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
                 Vre(s, s->V2, 0), Vim(s, s->V2, 0), (int)kk, fre, fim, ra.st));
     std::swap(s->V, s->V2);
     Work w = work_of(s);
-    OAPH_TRY(orth(st, 2, w, 0, fre, fim, fre, fim, nullptr, nullptr, nullptr, (int)kk - 1,
-                  (int)kk));
+    OAPH_TRY(residual(st, w, 0, fre, fim, nullptr, (int)kk - 1, (int)kk));
     OAPH_TRY(normalize(st, w, fre, fim, Vre(s, s->V, kk), Vim(s, s->V, kk)));
     start = kk;
   }
@@ -295,13 +298,13 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
   return OAPH_OK;
 }
 
-// X[:, t] = V Y[:, t] for the k wanted pairs
-oaph_status ritz_vectors(oaph_arnoldi* s, uint32_t mact) {
+// X[:, t] = V Y[:, t] for the first `count` Ritz vectors in wanted order
+oaph_status ritz_vectors(oaph_arnoldi* s, uint32_t mact, uint32_t count) {
   double* Yre = s->Y.as<double>();
-  double* Yim = Yre + (size_t)s->m * s->k;
+  double* Yim = Yre + (size_t)s->m * s->ky;
   double* Xre = s->X.as<double>();
-  double* Xim = Xre + (size_t)s->k * s->n;
-  return vq(s->api->stream, s->n, mact, s->k, Vre(s, s->V, 0), Vim(s, s->V, 0), Yre, Yim, s->k,
+  double* Xim = Xre + (size_t)s->ky * s->n;
+  return vq(s->api->stream, s->n, mact, count, Vre(s, s->V, 0), Vim(s, s->V, 0), Yre, Yim, s->ky,
             Xre, Xim, -1, nullptr, nullptr, nullptr);
 }
 
@@ -341,16 +344,16 @@ oaph_status normalize_v0(oaph_arnoldi* s) {
                           hipMemcpyDeviceToDevice, st));
   OAPH_HIP(hipMemcpyAsync(fim, Vim(s, s->V, 0), (size_t)s->n * sizeof(double),
                           hipMemcpyDeviceToDevice, st));
-  OAPH_TRY(orth(st, 2, w, 0, fre, fim, fre, fim, nullptr, nullptr, nullptr, -1, (int)s->m));
+  OAPH_TRY(residual(st, w, 0, fre, fim, nullptr, -1, (int)s->m));
   return normalize(st, w, fre, fim, Vre(s, s->V, 0), Vim(s, s->V, 0));
 }
 
 oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
   hipStream_t st = s->api->stream;
   const uint32_t k = s->k, n = s->n;
-  OAPH_TRY(ritz_vectors(s, mact));
+  OAPH_TRY(ritz_vectors(s, mact, k));
   double* Xre = s->X.as<double>();
-  double* Xim = Xre + (size_t)k * n;
+  double* Xim = Xre + (size_t)s->ky * n;
   double* AXre = s->AX.as<double>();
   double* AXim = AXre + (size_t)k * n;
   for (uint32_t t = 0; t < k; ++t)
@@ -484,7 +487,12 @@ oaph_status oaph_arnoldi_set_restart_stages(oaph_arnoldi* s, const double* tols,
     if (i) OAPH_CHECK(tols[i] < tols[i - 1], OAPH_ERR_ARG);
   }
   s->stage_tols.assign(tols, tols + n);
-  s->stages_set = true;
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* s, uint32_t guard) {
+  OAPH_CHECK(s && guard <= 64, OAPH_ERR_ARG);
+  s->guard = guard;
   return OAPH_OK;
 }
 
@@ -592,37 +600,36 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
   // FP64 one and every cycle runs on it, with no FP64 restart
   const bool all_low = mixed && s->a32_exact;
   // restart from the sum of the wanted Ritz vectors (a fresh Arnoldi start)
-  auto restart_from_ritz_sum = [&]() -> oaph_status {
-    OAPH_TRY(ritz_vectors(s, s->m));
+  auto restart_from_ritz_sum = [&](uint32_t count) -> oaph_status {
+    OAPH_TRY(ritz_vectors(s, s->m, count));
     double* Xre = s->X.as<double>();
-    OAPH_TRY(colsum(st, s->n, s->k, Xre, Xre + (size_t)s->k * s->n, Vre(s, s->V, 0),
+    OAPH_TRY(colsum(st, s->n, count, Xre, Xre + (size_t)s->ky * s->n, Vre(s, s->V, 0),
                     Vim(s, s->V, 0)));
     OAPH_TRY(normalize_v0(s));
     OAPH_HIP(hipMemsetAsync(s->status.ptr, 0, sizeof(Status), st));
     OAPH_HIP(hipMemsetAsync(s->H.ptr, 0, 2 * (size_t)s->m * s->m * sizeof(double), st));
     return OAPH_OK;
   };
-  // stages of the low-precision (or, in FP64 mode, the FP64) cycles.  The
-  // mixed flow restarts from the Ritz-vector sum at 1e-2 and 1e-4 unless the
-  // caller chose stages: measured on the dense complex 8192 problem, 72.5 vs
-  // 48.5 eigenpairs/s over 12 start vectors (60.0 vs 54.1 on 12 others, 98.4
-  // vs 93.5 on another matrix), never slower; plain FP64 keeps the
-  // reference's single cycle sequence unless stages are set.
-  static const double kMixedStages[] = {1e-2, 1e-4};
+  // Optional stages (none unless the caller sets them): the cycles stop at
+  // each stage tolerance and the solve restarts from the sum of the wanted
+  // (and guard) Ritz vectors.  Off by default: such a restart drops every
+  // direction outside the current wanted set, and on the dense complex 8192
+  // problem (8 wanted eigenvalues 0.2 % apart in modulus from the next)
+  // {1e-2, 1e-4} without guards returned the 9th eigenvalue in place of the
+  // 8th for 6 of 48 start vectors.
   const bool stage_low = mixed;
-  std::vector<double> stages = s->stage_tols;
-  if (!s->stages_set && mixed && !all_low) stages.assign(kMixedStages, kMixedStages + 2);
-  for (double t : stages) {
+  for (double t : s->stage_tols) {
     if (mixed && !all_low && t <= s->tol_low) break;
     uint32_t c = 0;
     OAPH_TRY(run_cycles(s, stage_low, t, 0.0, &c, &converged));
     cycles += c;
-    OAPH_TRY(restart_from_ritz_sum());
+    OAPH_TRY(restart_from_ritz_sum(s->ky));
   }
   if (mixed && !all_low) {
-    // FP32 phase (tolerance relative to max |Ritz value|)
+    // FP32 phase (tolerance relative to max |Ritz value|), then FP64 from
+    // the sum of the wanted (and guard) Ritz vectors
     OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
-    OAPH_TRY(restart_from_ritz_sum());
+    OAPH_TRY(restart_from_ritz_sum(s->ky));
   }
   uint32_t c2 = 0;
   OAPH_TRY(run_cycles(s, all_low, s->tol, norm, &c2, &converged));

@@ -40,7 +40,7 @@ struct RestartArgs {
   double* Him;
   double* Qre;
   double* Qim;
-  double* Yre;  // m x k wanted Ritz vectors of H
+  double* Yre;  // m x ky Ritz vectors of H in wanted order (leading dimension ky)
   double* Yim;
   double* Xre;  // m x m work
   double* Xim;
@@ -48,15 +48,28 @@ struct RestartArgs {
   int m;        // leading dimension of H, Q, X
   int mact;     // active size
   int k;
+  int ky;       // Ritz vectors written to Y (k wanted + guards)
   int which;
   int do_shifts;
   double tol;
   double anorm;  // <= 0: relative to max |Ritz value|
 };
 
-oaph_status orth(hipStream_t s, int mode, const Work& w, uint32_t ncols, const double* xre,
-                 const double* xim, double* fre, double* fim, const double2* sub,
-                 const double2* add, double2* result, int hcol, int hrow);
+// One Arnoldi step's CGS2 against V[:, 0..ncols): cgs_project (V^H x),
+// cgs_update_project (f = x - V h, h stored), cgs_finish (f -= V s, s stored,
+// H column hcol and beta into row hrow when hcol >= 0, v = f / beta when vre
+// is given).  The three launches hand partial sums over w.partials.
+oaph_status cgs_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
+                        const double* xim);
+oaph_status cgs_update_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
+                               const double* xim, double* fre, double* fim, double2* h);
+oaph_status cgs_finish(hipStream_t s, const Work& w, uint32_t ncols, double* fre, double* fim,
+                       const double2* h, double2* sc, int hcol, int hrow, double* vre,
+                       double* vim);
+// f <- f - V[:, 0..ncols) sub, status beta = ||f||, H[hrow][hcol] = beta when
+// hcol >= 0 and hrow < m.
+oaph_status residual(hipStream_t s, const Work& w, uint32_t ncols, double* fre, double* fim,
+                     const double2* sub, int hcol, int hrow);
 size_t orth_partials_bytes(uint32_t n, uint32_t m);
 oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const double* fim,
                       double* vre, double* vim);

@@ -7,9 +7,11 @@
 // full transpose every Arnoldi step: ArnoldiProcedures.cpp:441-443.)
 //
 // Kernels:
-//   k_orth<MODE>     classical Gram-Schmidt pass pieces with a one-launch grid
-//                    reduction (last block finishes; counter self-resets)
-//   k_normalize      v_{j+1} = f / beta
+//   k_cgs<MODE>      one Arnoldi step's CGS2 in three launches (partial sums
+//                    handed to the next launch; v_{j+1} written by the last)
+//   k_residual       f - V s and ||f|| with a one-launch grid reduction (last
+//                    block finishes; counter self-resets)
+//   k_normalize      v = f / beta
 //   k_restart        one workgroup: complex Schur form of H by shifted QR, Ritz
 //                    vectors of H, residual estimates, wanted ordering,
 //                    exact-shift sweeps and the compressed H, Q
@@ -24,7 +26,7 @@ namespace iram {
 namespace {
 
 constexpr int kOrthThreads = 256;
-constexpr int kRows = 64;  // rows per block in k_orth
+constexpr int kRows = 64;  // rows per block in k_residual
 constexpr int kCPW = kMaxM / 4 + 1;  // basis columns per wave (ncols <= m + 1)
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -48,125 +50,249 @@ __device__ __forceinline__ bool last_block(unsigned* counter) {
   return is_last;
 }
 
-// MODE 0: s = V^H x.
-// MODE 1: f = x - V sub, s = V^H f (f written to fout).
-// MODE 2: f = x - V sub (written to fout), beta = ||f||; the last block writes
-//         beta to the status and, when hcol >= 0, H[c][hcol] = add[c] + sub[c]
-//         for c < ncols and beta to H[hrow][hcol] when hrow < m; breakdown is
-//         flagged (hcol >= 0 only) when beta is below 1e-13 ||(h, beta)||.
-template <int MODE>
-__global__ __launch_bounds__(kOrthThreads) void k_orth(
+// f = x - V sub over the first ncols basis columns (ncols may be 0), beta =
+// ||f||: the last block writes beta to the status and, when hcol >= 0,
+// beta to H[hrow][hcol] (hrow < m).  The start vector and the restart's
+// column kk take this path; Arnoldi steps use the k_cgs chain.
+__global__ __launch_bounds__(kOrthThreads) void k_residual(
     uint32_t n, uint32_t ncols, const double* __restrict__ Vre, const double* __restrict__ Vim,
-    const double* xre, const double* xim, double* fre, double* fim, const double2* sub,
-    const double2* add, double2* result, double* partials, unsigned* counter, double* Hre,
-    double* Him, uint32_t m, int hcol, int hrow, Status* st) {
+    double* fre, double* fim, const double2* sub, double* partials, unsigned* counter,
+    double* Hre, double* Him, uint32_t m, int hcol, int hrow, Status* st) {
   __shared__ double2 lds_part[4][kRows];
-  __shared__ double2 lds_fin[4][kMaxM + 1];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t i = blockIdx.x * kRows + lane;
   const bool in = i < n;
-  double xr = in ? xre[i] : 0.0, xi = in ? xim[i] : 0.0;
-  // this wave's basis columns c = wave + 4q, all loads issued before use
-  double vr[kCPW], vi[kCPW];
+  double xr = in ? fre[i] : 0.0, xi = in ? fim[i] : 0.0;
+  double pr = 0, pi = 0;
 #pragma unroll
   for (int q = 0; q < kCPW; ++q) {
     const uint32_t c = wave + 4 * q;
-    const bool ok = in && c < ncols;
-    vr[q] = ok ? Vre[(size_t)c * n + i] : 0.0;
-    vi[q] = ok ? Vim[(size_t)c * n + i] : 0.0;
-  }
-  if (MODE >= 1) {
-    // this wave's share of (V sub)[i]
-    double pr = 0, pi = 0;
-#pragma unroll
-    for (int q = 0; q < kCPW; ++q) {
-      const uint32_t c = wave + 4 * q;
-      if (c < ncols) {
-        const double2 s = sub[c];
-        pr += vr[q] * s.x - vi[q] * s.y;
-        pi += vr[q] * s.y + vi[q] * s.x;
-      }
-    }
-    lds_part[wave][lane] = make_double2(pr, pi);
-    __syncthreads();
-    double2 a = lds_part[0][lane], b = lds_part[1][lane], c2 = lds_part[2][lane],
-            d = lds_part[3][lane];
-    xr -= (a.x + b.x) + (c2.x + d.x);
-    xi -= (a.y + b.y) + (c2.y + d.y);
-    if (wave == 0 && in) {
-      fre[i] = xr;
-      fim[i] = xi;
+    if (in && c < ncols) {
+      const double vr = Vre[(size_t)c * n + i], vi = Vim[(size_t)c * n + i];
+      const double2 s = sub[c];
+      pr += vr * s.x - vi * s.y;
+      pi += vr * s.y + vi * s.x;
     }
   }
-  const size_t pstride = (size_t)(ncols > 0 ? ncols : 1);
-  if (MODE <= 1) {
-#pragma unroll
-    for (int q = 0; q < kCPW; ++q) {
-      const uint32_t c = wave + 4 * q;
-      if (c < ncols) {
-        // conj(v) * f
-        double sr = wave_sum(vr[q] * xr + vi[q] * xi);
-        double si = wave_sum(vr[q] * xi - vi[q] * xr);
-        if (lane == 0) {
-          partials[2 * ((size_t)blockIdx.x * pstride + c)] = sr;
-          partials[2 * ((size_t)blockIdx.x * pstride + c) + 1] = si;
-        }
-      }
-    }
-  } else {
-    if (wave == 0) {
-      double s = wave_sum(xr * xr + xi * xi);
-      if (lane == 0) partials[blockIdx.x] = s;
-    }
+  lds_part[wave][lane] = make_double2(pr, pi);
+  __syncthreads();
+  double2 a = lds_part[0][lane], b = lds_part[1][lane], c2 = lds_part[2][lane],
+          d = lds_part[3][lane];
+  xr -= (a.x + b.x) + (c2.x + d.x);
+  xi -= (a.y + b.y) + (c2.y + d.y);
+  if (wave == 0 && in) {
+    fre[i] = xr;
+    fim[i] = xi;
+  }
+  if (wave == 0) {
+    double s = wave_sum(xr * xr + xi * xi);
+    if (lane == 0) partials[blockIdx.x] = s;
   }
   if (!last_block(counter)) return;
-  if (MODE <= 1) {
-    // lane = column, wave w sums blocks w, w+4, ...; waves combined in order
-    for (uint32_t c = lane; c < ncols; c += 64) {
-      double sr = 0, si = 0;
-#pragma unroll 8
-      for (uint32_t b = wave; b < gridDim.x; b += 4) {
-        sr += __builtin_nontemporal_load(&partials[2 * ((size_t)b * pstride + c)]);
-        si += __builtin_nontemporal_load(&partials[2 * ((size_t)b * pstride + c) + 1]);
-      }
-      lds_fin[wave][c] = make_double2(sr, si);
-    }
-    __syncthreads();
-    for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) {
-      double2 a = lds_fin[0][c], b = lds_fin[1][c], c2 = lds_fin[2][c], d = lds_fin[3][c];
-      result[c] = make_double2((a.x + b.x) + (c2.x + d.x), (a.y + b.y) + (c2.y + d.y));
-    }
-  } else {
-    if (threadIdx.x < 64) {
-      double s = 0;
-      for (uint32_t b = threadIdx.x; b < gridDim.x; b += 64) s += __builtin_nontemporal_load(&partials[b]);
-      s = wave_sum(s);
-      double hs = 0;
-      if (hcol >= 0) {
-        for (uint32_t c = threadIdx.x; c < ncols; c += 64) {
-          double2 h = make_double2(add[c].x + sub[c].x, add[c].y + sub[c].y);
-          Hre[(size_t)c * m + hcol] = h.x;
-          Him[(size_t)c * m + hcol] = h.y;
-          hs += h.x * h.x + h.y * h.y;
-        }
-      }
-      hs = wave_sum(hs);
-      if (threadIdx.x == 0) {
-        double beta = sqrt(s);
-        bool broke = hcol >= 0 && ncols > 0 && beta <= 1e-13 * sqrt(hs + s);
-        if (broke) {
-          beta = 0.0;
-          if (st->breakdown == 0) st->breakdown = hrow;
-        }
-        st->beta = beta;
-        if (hcol >= 0 && hrow >= 0 && (uint32_t)hrow < m) {
-          Hre[(size_t)hrow * m + hcol] = beta;
-          Him[(size_t)hrow * m + hcol] = 0.0;
-        }
+  if (threadIdx.x < 64) {
+    double s = 0;
+    for (uint32_t b = threadIdx.x; b < gridDim.x; b += 64) s += __builtin_nontemporal_load(&partials[b]);
+    s = wave_sum(s);
+    if (threadIdx.x == 0) {
+      const double beta = sqrt(s);
+      st->beta = beta;
+      if (hcol >= 0 && hrow >= 0 && (uint32_t)hrow < m) {
+        Hre[(size_t)hrow * m + hcol] = beta;
+        Him[(size_t)hrow * m + hcol] = 0.0;
       }
     }
   }
   if (threadIdx.x == 0) *counter = 0u;
+}
+
+// ---------------------------------------------------------------- CGS2 chain
+// One Arnoldi step's orthogonalisation in three launches with no grid-wide
+// synchronisation inside a kernel: each launch leaves per-block partial sums
+// and the next one starts by summing them (every block in the same order, so
+// every block holds the same bits).  Block b owns rows [b rpb, (b+1) rpb),
+// walked in chunks of 64 (lane = row); wave w owns basis columns w + 4q.
+//   k_cgs<0>: P1[b] = V^H x over the block's rows.
+//   k_cgs<1>: h = sum P1 (block 0 stores h); f1 = x - V h (written);
+//             P2[b] = V^H f1, N[b] = ||f1||^2 over the block's rows.
+//   k_cgs<2>: s = sum P2 (block 0 stores s); beta^2 = sum N - ||s||^2;
+//             f = f1 - V s (written) and v = f / beta (when v is given);
+//             block 0 writes H[c][hcol] = h_c + s_c, H[hrow][hcol] = beta, the
+//             status beta, and flags breakdown (hcol >= 0) when beta <=
+//             1e-13 ||(h + s, beta)||.
+// beta from Pythagoras: f1 - V s is orthogonal to the (orthonormal) basis,
+// so ||f||^2 = ||f1||^2 - ||s||^2; after the first pass ||s|| is of the
+// order of the rounding of that pass, so the difference loses no digits
+// unless f1 itself is rounding noise -- the breakdown case, where beta is
+// flagged or tiny either way.  Saves the norm launch that would otherwise
+// separate forming f from scaling it.
+constexpr int kCgsMaxBlocks = 256;
+constexpr int kCgsStride = kMaxM + 1;  // double2 entries per block in P1, P2
+
+// coef[c] = sum_b part[b][c] for c < ncols, into LDS; every thread calls.
+__device__ void sum_partials(const double2* __restrict__ part, uint32_t nparts, uint32_t ncols,
+                             double2* coef) {
+  __shared__ double2 acc[4][kMaxM + 1];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t c = lane; c < ncols; c += 64) {
+    double sr = 0.0, si = 0.0;
+#pragma unroll 16
+    for (uint32_t b = wave; b < nparts; b += 4) {
+      const double2 p = part[(size_t)b * kCgsStride + c];
+      sr += p.x;
+      si += p.y;
+    }
+    acc[wave][c] = make_double2(sr, si);
+  }
+  __syncthreads();
+  for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) {
+    const double2 a = acc[0][c], b = acc[1][c], e = acc[2][c], d = acc[3][c];
+    coef[c] = make_double2((a.x + b.x) + (e.x + d.x), (a.y + b.y) + (e.y + d.y));
+  }
+  __syncthreads();
+}
+
+struct CgsArgs {
+  uint32_t n, ncols, rpb, nparts;
+  const double* Vre;
+  const double* Vim;
+  const double* xre;
+  const double* xim;
+  double* fre;
+  double* fim;
+  const double2* part_in;  // P1 (MODE 1) or P2 (MODE 2)
+  double2* part_out;       // P1 (MODE 0) or P2 (MODE 1)
+  double* nrm;             // N: written by MODE 1, read by MODE 2
+  double2* coef_out;       // h (MODE 1) or s (MODE 2), stored by block 0
+  const double2* h;        // MODE 2: the first pass's coefficients
+  double* vre;             // MODE 2: v = f / beta (nullptr: not written)
+  double* vim;
+  double* Hre;
+  double* Him;
+  uint32_t m;
+  int hcol, hrow;
+  Status* st;
+};
+
+template <int MODE>
+__global__ __launch_bounds__(kOrthThreads) void k_cgs(CgsArgs a) {
+  __shared__ double2 coef[kMaxM + 1];
+  __shared__ double2 lds_part[4][64];
+  __shared__ double inv_s;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t ncols = a.ncols, n = a.n;
+  if (MODE >= 1) {
+    sum_partials(a.part_in, a.nparts, ncols, coef);
+    if (blockIdx.x == 0)
+      for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) a.coef_out[c] = coef[c];
+  }
+  if (MODE == 2) {
+    if (threadIdx.x < 64) {
+      double f1n = 0, ss = 0, hs = 0;
+      for (uint32_t b = lane; b < a.nparts; b += 64) f1n += a.nrm[b];
+      for (uint32_t c = lane; c < ncols; c += 64) {
+        const double2 s = coef[c];
+        ss += s.x * s.x + s.y * s.y;
+        if (a.hcol >= 0) {
+          const double2 h = make_double2(a.h[c].x + s.x, a.h[c].y + s.y);
+          hs += h.x * h.x + h.y * h.y;
+          if (blockIdx.x == 0) {
+            a.Hre[(size_t)c * a.m + a.hcol] = h.x;
+            a.Him[(size_t)c * a.m + a.hcol] = h.y;
+          }
+        }
+      }
+      f1n = wave_sum(f1n);
+      ss = wave_sum(ss);
+      hs = wave_sum(hs);
+      if (lane == 0) {
+        const double b2 = f1n - ss;
+        double beta = b2 > 0.0 ? sqrt(b2) : 0.0;
+        const bool broke = a.hcol >= 0 && ncols > 0 && beta <= 1e-13 * sqrt(hs + beta * beta);
+        if (broke) beta = 0.0;
+        inv_s = beta > 0.0 ? 1.0 / beta : 0.0;
+        if (blockIdx.x == 0) {
+          if (broke && a.st->breakdown == 0) a.st->breakdown = a.hrow;
+          a.st->beta = beta;
+          if (a.hcol >= 0 && a.hrow >= 0 && (uint32_t)a.hrow < a.m) {
+            a.Hre[(size_t)a.hrow * a.m + a.hcol] = beta;
+            a.Him[(size_t)a.hrow * a.m + a.hcol] = 0.0;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const double inv = MODE == 2 ? inv_s : 0.0;
+  double sr[kCPW], si[kCPW];
+#pragma unroll
+  for (int q = 0; q < kCPW; ++q) { sr[q] = 0.0; si[q] = 0.0; }
+  double nrm = 0.0;
+  const uint32_t r0 = blockIdx.x * a.rpb;
+  const uint32_t r1 = r0 + a.rpb < n ? r0 + a.rpb : n;
+  for (uint32_t base = r0; base < r1; base += 64) {  // uniform over the block
+    const uint32_t i = base + lane;
+    const bool in = i < r1;
+    double xr = in ? a.xre[i] : 0.0, xi = in ? a.xim[i] : 0.0;
+    double vr[kCPW], vi[kCPW];
+#pragma unroll
+    for (int q = 0; q < kCPW; ++q) {
+      const uint32_t c = wave + 4 * q;
+      const bool ok = in && c < ncols;
+      vr[q] = ok ? a.Vre[(size_t)c * n + i] : 0.0;
+      vi[q] = ok ? a.Vim[(size_t)c * n + i] : 0.0;
+    }
+    if (MODE >= 1) {
+      // this wave's share of (V coef)[i], combined over the waves in LDS
+      double pr = 0, pi = 0;
+#pragma unroll
+      for (int q = 0; q < kCPW; ++q) {
+        const uint32_t c = wave + 4 * q;
+        if (c < ncols) {
+          const double2 s = coef[c];
+          pr += vr[q] * s.x - vi[q] * s.y;
+          pi += vr[q] * s.y + vi[q] * s.x;
+        }
+      }
+      if (base != r0) __syncthreads();  // the previous chunk's reads of lds_part are done
+      lds_part[wave][lane] = make_double2(pr, pi);
+      __syncthreads();
+      const double2 p0 = lds_part[0][lane], p1 = lds_part[1][lane], p2 = lds_part[2][lane],
+                    p3 = lds_part[3][lane];
+      xr -= (p0.x + p1.x) + (p2.x + p3.x);
+      xi -= (p0.y + p1.y) + (p2.y + p3.y);
+      if (wave == 0 && in) {
+        a.fre[i] = xr;
+        a.fim[i] = xi;
+        if (MODE == 2 && a.vre) {
+          a.vre[i] = xr * inv;
+          a.vim[i] = xi * inv;
+        }
+      }
+    }
+    if (MODE <= 1) {
+#pragma unroll
+      for (int q = 0; q < kCPW; ++q) {
+        // conj(v) * x
+        sr[q] += vr[q] * xr + vi[q] * xi;
+        si[q] += vr[q] * xi - vi[q] * xr;
+      }
+    }
+    if (MODE == 1) nrm += xr * xr + xi * xi;
+  }
+  if (MODE <= 1) {
+#pragma unroll
+    for (int q = 0; q < kCPW; ++q) {
+      const uint32_t c = wave + 4 * q;
+      if (c < ncols) {
+        const double pr = wave_sum(sr[q]), pi = wave_sum(si[q]);
+        if (lane == 0) a.part_out[(size_t)blockIdx.x * kCgsStride + c] = make_double2(pr, pi);
+      }
+    }
+  }
+  if (MODE == 1 && wave == 0) {
+    nrm = wave_sum(nrm);
+    if (lane == 0) a.nrm[blockIdx.x] = nrm;
+  }
 }
 
 __global__ void k_normalize(uint32_t n, const double* fre, const double* fim, double* vre,
@@ -345,30 +471,107 @@ __global__ void k_random(uint32_t n, uint64_t seed, int cplx, double* re, double
 // ------------------------------------------------------------------ launchers
 static unsigned orth_blocks(uint32_t n) { return (n + kRows - 1) / kRows; }
 
-oaph_status orth(hipStream_t s, int mode, const Work& w, uint32_t ncols, const double* xre,
-                 const double* xim, double* fre, double* fim, const double2* sub,
-                 const double2* add, double2* result, int hcol, int hrow) {
-  dim3 g(orth_blocks(w.n)), t(kOrthThreads);
-  switch (mode) {
-    case 0:
-      hipLaunchKernelGGL(k_orth<0>, g, t, 0, s, w.n, ncols, w.Vre, w.Vim, xre, xim, fre, fim, sub,
-                         add, result, w.partials, w.counter, w.Hre, w.Him, w.m, hcol, hrow, w.st);
-      break;
-    case 1:
-      hipLaunchKernelGGL(k_orth<1>, g, t, 0, s, w.n, ncols, w.Vre, w.Vim, xre, xim, fre, fim, sub,
-                         add, result, w.partials, w.counter, w.Hre, w.Him, w.m, hcol, hrow, w.st);
-      break;
-    default:
-      hipLaunchKernelGGL(k_orth<2>, g, t, 0, s, w.n, ncols, w.Vre, w.Vim, xre, xim, fre, fim, sub,
-                         add, result, w.partials, w.counter, w.Hre, w.Him, w.m, hcol, hrow, w.st);
-      break;
-  }
+oaph_status residual(hipStream_t s, const Work& w, uint32_t ncols, double* fre, double* fim,
+                     const double2* sub, int hcol, int hrow) {
+  OAPH_CHECK(ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  hipLaunchKernelGGL(k_residual, dim3(orth_blocks(w.n)), dim3(kOrthThreads), 0, s, w.n, ncols,
+                     w.Vre, w.Vim, fre, fim, sub, w.partials, w.counter, w.Hre, w.Him, w.m, hcol,
+                     hrow, w.st);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
 
+namespace {
+
+// blocks of the CGS2 chain and the rows each owns (a multiple of 64)
+unsigned cgs_blocks(uint32_t n) {
+  unsigned b = (n + 63) / 64;
+  return b < kCgsMaxBlocks ? (b ? b : 1) : kCgsMaxBlocks;
+}
+
+CgsArgs cgs_args(const Work& w, uint32_t ncols) {
+  CgsArgs a{};
+  const unsigned nb = cgs_blocks(w.n);
+  a.n = w.n;
+  a.ncols = ncols;
+  a.nparts = nb;
+  a.rpb = ((w.n + nb - 1) / nb + 63) / 64 * 64;
+  a.Vre = w.Vre;
+  a.Vim = w.Vim;
+  a.Hre = w.Hre;
+  a.Him = w.Him;
+  a.m = w.m;
+  a.st = w.st;
+  a.hcol = -1;
+  a.hrow = -1;
+  return a;
+}
+
+double2* cgs_p1(const Work& w) { return reinterpret_cast<double2*>(w.partials); }
+double2* cgs_p2(const Work& w) { return cgs_p1(w) + (size_t)kCgsMaxBlocks * kCgsStride; }
+double* cgs_nrm(const Work& w) {
+  return reinterpret_cast<double*>(cgs_p2(w) + (size_t)kCgsMaxBlocks * kCgsStride);
+}
+
+}  // namespace
+
 size_t orth_partials_bytes(uint32_t n, uint32_t m) {
-  return (size_t)orth_blocks(n) * (m + 1) * 2 * sizeof(double);
+  (void)m;
+  const size_t cgs = (size_t)2 * kCgsMaxBlocks * kCgsStride * sizeof(double2) +
+                     (size_t)kCgsMaxBlocks * sizeof(double);
+  const size_t res = (size_t)orth_blocks(n) * sizeof(double);
+  return cgs > res ? cgs : res;
+}
+
+oaph_status cgs_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
+                        const double* xim) {
+  OAPH_CHECK(ncols >= 1 && ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  CgsArgs a = cgs_args(w, ncols);
+  a.xre = xre;
+  a.xim = xim;
+  a.part_out = cgs_p1(w);
+  hipLaunchKernelGGL(k_cgs<0>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status cgs_update_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
+                               const double* xim, double* fre, double* fim, double2* h) {
+  OAPH_CHECK(ncols >= 1 && ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  CgsArgs a = cgs_args(w, ncols);
+  a.xre = xre;
+  a.xim = xim;
+  a.fre = fre;
+  a.fim = fim;
+  a.part_in = cgs_p1(w);
+  a.part_out = cgs_p2(w);
+  a.nrm = cgs_nrm(w);
+  a.coef_out = h;
+  hipLaunchKernelGGL(k_cgs<1>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status cgs_finish(hipStream_t s, const Work& w, uint32_t ncols, double* fre, double* fim,
+                       const double2* h, double2* sc, int hcol, int hrow, double* vre,
+                       double* vim) {
+  OAPH_CHECK(ncols >= 1 && ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  CgsArgs a = cgs_args(w, ncols);
+  a.xre = fre;
+  a.xim = fim;
+  a.fre = fre;
+  a.fim = fim;
+  a.part_in = cgs_p2(w);
+  a.nrm = cgs_nrm(w);
+  a.coef_out = sc;
+  a.h = h;
+  a.vre = vre;
+  a.vim = vim;
+  a.hcol = hcol;
+  a.hrow = hrow;
+  hipLaunchKernelGGL(k_cgs<2>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
 }
 
 oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const double* fim,

@@ -439,8 +439,9 @@ __device__ bool ritz_before(cd a, cd b, int which) {
 // column walk (one row per lane) spreads over the LDS banks
 // (4 mact lds_ld(mact) doubles).  Global work a.X: eigenvectors of T
 // (m x m complex, column i for T[i][i]).  Outputs: status (nconv, kk, sorted
-// Ritz values, estimates), Y (wanted Ritz vectors of H), and with
-// a.do_shifts the compressed H and the accumulated Q of the exact shifts.
+// Ritz values, estimates), Y (the first ky Ritz vectors of H in wanted
+// order), and with a.do_shifts the compressed H and the accumulated Q of the
+// exact shifts.
 __device__ void restart_body(const RestartArgs& a, double* smem) {
   const int mm = a.mact, ld = lds_ld(a.mact);
   Mat T{smem, smem + mm * ld, ld};
@@ -561,8 +562,8 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   }
   __syncthreads();
   if (lane == 0) st->clk[3] = clock64();
-  // wanted Ritz vectors of H: Y[:, t] = Z x_{order[t]} / ||x||, t < k
-  for (int t = 0; t < a.k && t < mm; ++t) {
+  // Ritz vectors of H in wanted order: Y[:, t] = Z x_{order[t]} / ||x||, t < ky
+  for (int t = 0; t < a.ky && t < mm; ++t) {
     int i = order[t];
     double nx = 0;
     for (int l = 0; l <= i; ++l)
@@ -573,8 +574,8 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
       cd y = {0, 0};
       for (int l = 0; l <= i; ++l)
         y = cadd(y, cmul(Z.get(r, l), {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]}));
-      a.Yre[(size_t)r * a.k + t] = y.re * inv;
-      a.Yim[(size_t)r * a.k + t] = y.im * inv;
+      a.Yre[(size_t)r * a.ky + t] = y.re * inv;
+      a.Yim[(size_t)r * a.ky + t] = y.im * inv;
     }
   }
   __syncthreads();
