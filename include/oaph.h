@@ -341,6 +341,9 @@ typedef struct oaph_arnoldi_stats {
   uint32_t nconv;
   double seconds;       /* wall time of execute */
   double max_resid;     /* max explicit ||A x - l x|| over the wanted pairs */
+  uint32_t fallbacks;   /* mixed flow: times the FP64 cycles met a wanted
+                           eigenvalue the FP32 phase had not found and went
+                           back to the FP32 operand */
 } oaph_arnoldi_stats;
 oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
 

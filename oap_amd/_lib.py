@@ -47,7 +47,8 @@ class oaph_matrix_info(C.Structure):
 
 class oaph_arnoldi_stats(C.Structure):
     _fields_ = [("restarts", C.c_uint32), ("matvecs", C.c_uint32), ("matvecs_low", C.c_uint32),
-                ("nconv", C.c_uint32), ("seconds", C.c_double), ("max_resid", C.c_double)]
+                ("nconv", C.c_uint32), ("seconds", C.c_double), ("max_resid", C.c_double),
+                ("fallbacks", C.c_uint32)]
 
 
 MP = C.POINTER(oaph_matrix)

@@ -211,11 +211,38 @@ oaph_status fresh_column(oaph_arnoldi* s, uint32_t nc, uint64_t seed) {
   return cgs_finish(st, w, nc, fre, fim, c1, c2, -1, -1, Vre(s, s->V, nc), Vim(s, s->V, nc));
 }
 
+// The wanted Ritz values an FP32 phase ended with, watched by the FP64
+// cycles that polish them (mixed flow).
+struct Watch {
+  std::vector<double> re, im;  // the k values
+  double low_tol = 0.0;        // the FP32 phase's tolerance
+  bool fell_back = false;      // out: a new wanted value appeared; the cycles
+                               // went back to FP32 and ended at low_tol
+};
+
+// True when some current wanted Ritz value is not within the watch distance
+// of any watched one: the FP32 phase had not yet found that eigenvalue.
+bool wanted_set_changed(const oaph_arnoldi* s, const Watch& w) {
+  const Status& hs = s->host_status;
+  const double dist = std::max(1e-4, 100.0 * w.low_tol) * hs.maxabs;
+  for (uint32_t t = 0; t < s->k; ++t) {
+    double best = INFINITY;
+    for (size_t i = 0; i < w.re.size(); ++i)
+      best = std::min(best, std::hypot(hs.ev_re[t] - w.re[i], hs.ev_im[t] - w.im[i]));
+    if (best > dist) return true;
+  }
+  return false;
+}
+
 // Runs restart cycles from column `start` until convergence at `tol` (relative
 // to norm, or to max |Ritz value| when norm <= 0).  On return the status holds
-// the final ordering and Y the wanted Ritz vectors of H.
+// the final ordering and Y the wanted Ritz vectors of H.  With `watch` (FP64
+// cycles after an FP32 phase), a restart whose wanted Ritz values differ from
+// the watched ones switches the remaining cycles to the FP32 operand at the
+// watch's tolerance (a wanted eigenvalue turned up late: its discovery runs
+// on the cheaper matvec) and returns when they reach it.
 oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint32_t* cycles,
-                       bool* converged) {
+                       bool* converged, Watch* watch = nullptr) {
   hipStream_t st = s->api->stream;
   uint32_t start = 0;
   *converged = false;
@@ -281,6 +308,14 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     if (stop || cycle + 1 >= s->max_restarts) {
       *cycles = cycle + 1;
       return OAPH_OK;
+    }
+    if (watch && !low && s->check == OAPH_CHECK_INTERNAL && wanted_set_changed(s, *watch)) {
+      // the factorisation goes on with FP32-operand steps: its FP64 columns
+      // differ from FP32 ones by far less than the FP32 tolerance
+      low = true;
+      tol = watch->low_tol;
+      norm = 0.0;
+      watch->fell_back = true;
     }
     // V <- V Q[:, 0..kk), f update with column kk; then v_kk = f / ||f||
     const uint32_t kk = (uint32_t)hs.kk;
@@ -625,14 +660,30 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     cycles += c;
     OAPH_TRY(restart_from_ritz_sum(s->ky));
   }
+  uint32_t c2 = 0;
   if (mixed && !all_low) {
     // FP32 phase (tolerance relative to max |Ritz value|), then FP64 from
-    // the sum of the wanted (and guard) Ritz vectors
+    // the sum of the wanted (and guard) Ritz vectors.  When the FP64 cycles
+    // meet a wanted eigenvalue the FP32 phase had not found, they go back to
+    // the FP32 operand until the new set reaches the FP32 tolerance, and the
+    // FP64 polish starts again from its Ritz-vector sum (at most
+    // kMaxFallbacks times; then the FP64 cycles finish on their own).
+    constexpr int kMaxFallbacks = 3;
     OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
-    OAPH_TRY(restart_from_ritz_sum(s->ky));
+    for (int round = 0;; ++round) {
+      Watch wt;
+      wt.re.assign(s->host_status.ev_re, s->host_status.ev_re + s->k);
+      wt.im.assign(s->host_status.ev_im, s->host_status.ev_im + s->k);
+      wt.low_tol = s->tol_low;
+      OAPH_TRY(restart_from_ritz_sum(s->ky));
+      OAPH_TRY(run_cycles(s, false, s->tol, norm, &c2, &converged,
+                          round < kMaxFallbacks ? &wt : nullptr));
+      if (!wt.fell_back) break;
+      s->stats.fallbacks++;
+    }
+  } else {
+    OAPH_TRY(run_cycles(s, all_low, s->tol, norm, &c2, &converged));
   }
-  uint32_t c2 = 0;
-  OAPH_TRY(run_cycles(s, all_low, s->tol, norm, &c2, &converged));
   int bd = s->host_status.breakdown;
   uint32_t mact = bd > 0 ? (uint32_t)bd : s->m;
   OAPH_TRY(finish(s, mact));

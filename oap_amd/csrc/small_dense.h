@@ -132,6 +132,49 @@ __device__ __forceinline__ cd gcol_right(cd a, cd b, double c, cd s) {
   return {-(a.re * s.re - a.im * s.im) + b.re * c, -(a.re * s.im + a.im * s.re) + b.im * c};
 }
 
+// One bulge step's row phase: the rotation from (f, g) (cgivens) and the
+// rotated row pair G (a; b).  With c = |f|^2 t and s = f conj(g) t, the pair
+// is t (|f|^2 a + f conj(g) b, -conj(f) g a + |f|^2 b): the products that
+// need only f, g, a and b are formed while the reciprocal square root is in
+// flight, and t scales them last (one dependent multiply after t instead of
+// the rotation's three).  The range test that picks the hypot-based
+// rotation runs beside it; the fallback recomputes everything.
+// givens_prep forms the parts that need only (f, g), so that a caller can
+// place them before the first use of row data still being loaded.
+struct GivensPrep {
+  double af2, ag2, pr, pi, x;
+};
+__device__ __forceinline__ GivensPrep givens_prep(cd f, cd g) {
+  GivensPrep q;
+  q.af2 = f.re * f.re + f.im * f.im;
+  q.ag2 = g.re * g.re + g.im * g.im;
+  q.pr = f.re * g.re + f.im * g.im;  // f conj(g)
+  q.pi = f.im * g.re - f.re * g.im;
+  q.x = q.af2 * (q.af2 + q.ag2);
+  return q;
+}
+__device__ __forceinline__ void givens_rows(const GivensPrep& q, cd f, cd g, cd a, cd b,
+                                            double& c, cd& s, cd& na, cd& nb) {
+  const double af2 = q.af2, ag2 = q.ag2, pr = q.pr, pi = q.pi;
+  const cd ua = {af2 * a.re + (pr * b.re - pi * b.im), af2 * a.im + (pr * b.im + pi * b.re)};
+  const cd ub = {af2 * b.re - (pr * a.re + pi * a.im), af2 * b.im - (pr * a.im - pi * a.re)};
+  const double t = rsqrt(q.x);
+  c = af2 * t;
+  s = {pr * t, pi * t};
+  na = {ua.re * t, ua.im * t};
+  nb = {ub.re * t, ub.im * t};
+  if (!(af2 > 1e-150 && af2 < 1e150 && ag2 < 1e150)) {
+    if (ag2 == 0.0) {
+      c = 1.0;
+      s = {0.0, 0.0};
+    } else {
+      cgivens_safe(f, g, c, s);
+    }
+    na = grow_top(a, b, c, s);
+    nb = grow_bot(a, b, c, s);
+  }
+}
+
 constexpr int kDefer = 8;  // rotations per deferred batch
 
 // The column halves of rotations [k1, k2) that no later rotation of the
@@ -269,9 +312,17 @@ __device__ void chase(Mat T, Mat Q, int kdim, int lo, int hi, cd mu, RotLog log)
 // entries, and left and right products commute.  Same result as the ns
 // sweeps up to rounding, with ns rotations per step instead of one.
 __device__ void chase_train(Mat T, Mat Q, int kdim, int lo, int hi, int ns, cd mu) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63;
+  // the wave index in a scalar register: the per-step conditions below are
+  // then scalar branches
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const int L = hi - lo;
   const int steps = L + 3 * (ns - 1);
+  const int jc = lane < kdim ? lane : kdim - 1;  // clamped lane index: loads stay in the matrix
+  // (f, g) of this bulge's next step: column k of rows k + 1 and k + 2 after
+  // its column phase, kept in registers (no other bulge writes them before
+  // that step), so the next rotation starts without waiting for LDS
+  cd fn = {0.0, 0.0}, gn = {0.0, 0.0};
   for (int t = 0; t < steps; ++t) {
     const int p = t - 3 * wave;
     const bool act = wave < ns && p >= 0 && p < L;
@@ -281,33 +332,42 @@ __device__ void chase_train(Mat T, Mat Q, int kdim, int lo, int hi, int ns, cd m
     if (act) {
       const int j = lane, j0 = k > lo ? k - 1 : k;
       const bool mine = j >= j0 && j < kdim;
-      cd a = {0.0, 0.0}, b = {0.0, 0.0};
-      if (mine) {
-        a = T.get(k, j);
-        b = T.get(k + 1, j);
-      }
-      // Q's column pair is written only by this wave's column phases
-      if (j < kdim) {
-        qx = Q.get(j, k);
-        qy = Q.get(j, k + 1);
-      }
       // (f, g) = column j0 of the row pair: the bulge column, or the shifted
       // leading column at the top of the block
-      cd f = lane_read(a, j0), g = lane_read(b, j0);
-      if (k == lo) f = csub(f, mu);
-      cgivens(f, g, c, s);
+      if (k == lo) {
+        fn = csub(T.get(lo, lo), mu);
+        gn = T.get(lo + 1, lo);
+      }
+      const cd f = fn, g = gn;
+      // every lane loads (clamped), so the LDS wait count after these loads
+      // does not depend on which lanes are active
+      const cd a = T.get(k, jc), b = T.get(k + 1, jc);
+      // Q's column pair is written only by this wave's column phases
+      qx = Q.get(jc, k);
+      qy = Q.get(jc, k + 1);
+      // the rotation's first products while the row loads are in flight
+      const GivensPrep gp = givens_prep(f, g);
+      __builtin_amdgcn_sched_barrier(0);
+      cd na, nb;
+      givens_rows(gp, f, g, a, b, c, s, na, nb);
       if (mine) {
-        T.set(k, j, grow_top(a, b, c, s));
-        T.set(k + 1, j, (k > lo && j == k - 1) ? cd{0.0, 0.0} : grow_bot(a, b, c, s));
+        T.set(k, j, na);
+        T.set(k + 1, j, (k > lo && j == k - 1) ? cd{0.0, 0.0} : nb);
       }
     }
     __syncthreads();
     if (act) {
       const int r = lane, rmax = k + 2 < hi ? k + 2 : hi;
+      const int rc = r <= rmax ? r : rmax;
+      const cd x = T.get(rc, k), y = T.get(rc, k + 1);
+      const cd nx = gcol_left(x, y, c, s);
       if (r <= rmax) {
-        const cd x = T.get(r, k), y = T.get(r, k + 1);
-        T.set(r, k, gcol_left(x, y, c, s));
+        T.set(r, k, nx);
         T.set(r, k + 1, gcol_right(x, y, c, s));
+      }
+      if (k + 1 < hi) {
+        fn = lane_read(nx, k + 1);
+        gn = lane_read(nx, k + 2);
       }
       if (r < kdim) {
         Q.set(r, k, gcol_left(qx, qy, c, s));
