@@ -191,6 +191,29 @@ def test_wanted_set_over_start_vectors(oracle):
     assert wanted_set_misses(A, dA, Precision.MIXED_FP32, 4, seeds) == []
 
 
+def test_mixed_fallback_to_fp32(oracle):
+    # start vector 262 on the n = 1024 C1 construction: the FP32 phase ends
+    # without one wanted eigenvalue, the FP64 cycles meet it and go back to
+    # the FP32 operand; the result is the dense top 8 to 1e-10
+    n = 1024
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    dA = DeviceMatrix.from_numpy(A.real, A.imag)
+    s = solver(seed=262)
+    s.setMatrix(dA)
+    s.setPrecision(Precision.MIXED_FP32)
+    re, im = np.zeros(8), np.zeros(8)
+    s.setOutputsEigenvalues(re, im)
+    vecs = [DeviceMatrix(1, n) for _ in range(8)]
+    s.setOutputsEigenvectors(vecs)
+    s.execute(40, 8, MatrixInfo(True, True, n, n))
+    st = s.stats()
+    assert st["fallbacks"] >= 1 and st["matvecs_low"] > 0
+    ev = re + 1j * im
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+
+
 def test_restart_guard(oracle):
     # stages with two guard Ritz vectors in each restart sum: same eigenpairs
     n = 1024
