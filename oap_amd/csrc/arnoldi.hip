@@ -90,7 +90,7 @@ oaph_status alloc(oaph_arnoldi* s) {
   OAPH_TRY(s->X.reserve(2 * ky * n * sizeof(double)));
   OAPH_TRY(s->AX.reserve(2 * k * n * sizeof(double)));
   OAPH_TRY(s->coef.reserve(6 * (m + 1) * sizeof(double2)));
-  size_t pb = orth_partials_bytes(s->n, s->m);
+  size_t pb = orth_partials_bytes(s->n);
   size_t pp = pairs_partials_bytes(s->n, s->k);
   OAPH_TRY(s->partials.reserve(pb > pp ? pb : pp));
   if (!s->counter.ptr) {

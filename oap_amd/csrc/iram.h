@@ -70,7 +70,7 @@ oaph_status cgs_finish(hipStream_t s, const Work& w, uint32_t ncols, double* fre
 // hcol >= 0 and hrow < m.
 oaph_status residual(hipStream_t s, const Work& w, uint32_t ncols, double* fre, double* fim,
                      const double2* sub, int hcol, int hrow);
-size_t orth_partials_bytes(uint32_t n, uint32_t m);
+size_t orth_partials_bytes(uint32_t n);
 oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const double* fim,
                       double* vre, double* vim);
 oaph_status restart(hipStream_t s, const RestartArgs& a);

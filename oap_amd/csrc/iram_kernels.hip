@@ -515,8 +515,7 @@ double* cgs_nrm(const Work& w) {
 
 }  // namespace
 
-size_t orth_partials_bytes(uint32_t n, uint32_t m) {
-  (void)m;
+size_t orth_partials_bytes(uint32_t n) {
   const size_t cgs = (size_t)2 * kCgsMaxBlocks * kCgsStride * sizeof(double2) +
                      (size_t)kCgsMaxBlocks * sizeof(double);
   const size_t res = (size_t)orth_blocks(n) * sizeof(double);
