@@ -355,6 +355,14 @@ class CuHArnoldi {
   void setMixedTolerance(double t) {
     oap::check(oaph_arnoldi_set_mixed_tolerance(m_a, t), "setMixedTolerance");
   }
+  void setRestartStages(const std::vector<double>& tols) {
+    oap::check(oaph_arnoldi_set_restart_stages(m_a, tols.empty() ? nullptr : tols.data(),
+                                               (uint32_t)tols.size()),
+               "setRestartStages");
+  }
+  void setRestartGuard(uintt g) {
+    oap::check(oaph_arnoldi_set_restart_guard(m_a, (uint32_t)g), "setRestartGuard");
+  }
   oaph_arnoldi_stats stats() const {
     oaph_arnoldi_stats s;
     oap::check(oaph_arnoldi_get_stats(m_a, &s), "stats");
