@@ -584,13 +584,13 @@ oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const dou
 
 oaph_status restart(hipStream_t s, const RestartArgs& a) {
   OAPH_CHECK(a.mact >= 2 && a.mact <= kMaxM, OAPH_ERR_UNSUPPORTED);
-  size_t lds = (size_t)4 * a.mact * lds_ld(a.mact) * sizeof(double);
+  const size_t lds = (size_t)restart_lds_bytes(a.mact);
   static bool attr_set = false;
   if (!attr_set) {
-    // T and Z of a 64 x 64 problem need 130 KiB of the 160 KiB LDS
+    // T and Z of a 64 x 64 problem need 130 KiB of the 160 KiB LDS; up to
+    // m = 54 X joins them (at most 144 KiB)
     OAPH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_restart),
-                                 hipFuncAttributeMaxDynamicSharedMemorySize,
-                                 4 * kMaxM * lds_ld(kMaxM) * (int)sizeof(double)));
+                                 hipFuncAttributeMaxDynamicSharedMemorySize, kRestartLdsMax));
     attr_set = true;
   }
   hipLaunchKernelGGL(k_restart, dim3(1), dim3(512), lds, s, a);

@@ -480,6 +480,15 @@ __device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
 }
 
 __host__ __device__ constexpr int lds_ld(int m) { return m | 1; }
+// Dynamic LDS of the restart kernel: T and Z, and the eigenvector work X
+// beside them when the three fit (m <= 54); larger m keep X in global memory.
+constexpr int kRestartLdsMax = 144 * 1024;
+__host__ __device__ constexpr bool x_in_lds(int m) {
+  return 6 * m * lds_ld(m) * (int)sizeof(double) <= kRestartLdsMax;
+}
+__host__ __device__ constexpr int restart_lds_bytes(int m) {
+  return (x_in_lds(m) ? 6 : 4) * m * lds_ld(m) * (int)sizeof(double);
+}
 
 __device__ bool ritz_before(cd a, cd b, int which) {
   double ma = cabs2(a), mb = cabs2(b);
@@ -497,8 +506,9 @@ __device__ bool ritz_before(cd a, cd b, int which) {
 // k_restart and the batched solver both call it).  smem: T and Z, mact x
 // mact complex each with the odd leading dimension lds_ld(mact), so that a
 // column walk (one row per lane) spreads over the LDS banks
-// (4 mact lds_ld(mact) doubles).  Global work a.X: eigenvectors of T
-// (m x m complex, column i for T[i][i]).  Outputs: status (nconv, kk, sorted
+// (4 mact lds_ld(mact) doubles), then X, the eigenvectors of T (column i for
+// T[i][i]; 2 mact lds_ld(mact) doubles) when x_in_lds(mact), else in the
+// global work a.X (m x m complex).  Outputs: status (nconv, kk, sorted
 // Ritz values, estimates), Y (the first ky Ritz vectors of H in wanted
 // order), and with a.do_shifts the compressed H and the accumulated Q of the
 // exact shifts.
@@ -547,32 +557,36 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   __syncthreads();
   const double small = fmax(tnorm_s * 2.220446049250313e-16, 1e-300);
   // eigenvectors of T by back substitution: lane i owns column i of X
-  double* Xre = a.Xre;
-  double* Xim = a.Xim;
+  // X in LDS when it fits (each lane's dependent back substitution then
+  // reads LDS, not L2), else the global work a.X
+  const bool xl = x_in_lds(mm);
+  double* Xre = xl ? smem + 4 * mm * ld : a.Xre;
+  double* Xim = xl ? smem + 5 * mm * ld : a.Xim;
+  const size_t xld = xl ? (size_t)ld : (size_t)a.m;
   const double beta = st->beta;
   for (int i = lane; i < mm; i += nthr) {
     cd li = ev[i];
     for (int j = 0; j < mm; ++j) {
-      Xre[(size_t)j * a.m + i] = j == i ? 1.0 : 0.0;
-      Xim[(size_t)j * a.m + i] = 0.0;
+      Xre[j * xld + i] = j == i ? 1.0 : 0.0;
+      Xim[j * xld + i] = 0.0;
     }
     for (int j = i - 1; j >= 0; --j) {
       cd s = {0, 0};
       for (int l = j + 1; l <= i; ++l) {
-        cd x = {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]};
+        cd x = {Xre[l * xld + i], Xim[l * xld + i]};
         s = cadd(s, cmul(T.get(j, l), x));
       }
       cd d = csub(T.get(j, j), li);
       if (hypot(d.re, d.im) < small) d = {small, 0.0};
       double den = cabs2(d);
-      Xre[(size_t)j * a.m + i] = -(s.re * d.re + s.im * d.im) / den;
-      Xim[(size_t)j * a.m + i] = -(s.im * d.re - s.re * d.im) / den;
+      Xre[j * xld + i] = -(s.re * d.re + s.im * d.im) / den;
+      Xim[j * xld + i] = -(s.im * d.re - s.re * d.im) / den;
     }
     // y = Z x: ||y|| = ||x|| (Z unitary); residual estimate beta |y[mm-1]| / ||y||
     double nx = 0;
     cd ylast = {0, 0};
     for (int l = 0; l <= i; ++l) {
-      cd x = {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]};
+      cd x = {Xre[l * xld + i], Xim[l * xld + i]};
       nx += cabs2(x);
       ylast = cadd(ylast, cmul(Z.get(mm - 1, l), x));
     }
@@ -627,13 +641,13 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
     int i = order[t];
     double nx = 0;
     for (int l = 0; l <= i; ++l)
-      nx += Xre[(size_t)l * a.m + i] * Xre[(size_t)l * a.m + i] +
-            Xim[(size_t)l * a.m + i] * Xim[(size_t)l * a.m + i];
+      nx += Xre[l * xld + i] * Xre[l * xld + i] +
+            Xim[l * xld + i] * Xim[l * xld + i];
     double inv = 1.0 / sqrt(nx);
     for (int r = lane; r < mm; r += nthr) {
       cd y = {0, 0};
       for (int l = 0; l <= i; ++l)
-        y = cadd(y, cmul(Z.get(r, l), {Xre[(size_t)l * a.m + i], Xim[(size_t)l * a.m + i]}));
+        y = cadd(y, cmul(Z.get(r, l), {Xre[l * xld + i], Xim[l * xld + i]}));
       a.Yre[(size_t)r * a.ky + t] = y.re * inv;
       a.Yim[(size_t)r * a.ky + t] = y.im * inv;
     }
