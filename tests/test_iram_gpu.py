@@ -298,6 +298,30 @@ def test_breakdown_first_value_is_one():
     np.testing.assert_allclose(sorted(re), [62, 63, 64], rtol=1e-10)
 
 
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.MIXED_FP32])
+def test_lucky_breakdown_invariant_subspace(precision):
+    # 10 distinct eigenvalues (each 6 or 7 times) on the diagonal: the Krylov
+    # space of any start vector has dimension 10 < m, so the factorisation
+    # breaks down at step 10 (>= k) and the leading block is invariant.  The
+    # values are thirds (not FP32 values), so the mixed flow runs its FP32
+    # phase and the FP64 polish, and both see the breakdown.
+    n, k, m = 64, 3, 16
+    vals = (np.arange(10) + 1.0) / 3.0
+    A = np.diag(vals[np.arange(n) % 10])
+    dA = DeviceMatrix.from_numpy(A, None)
+    s = solver()
+    s.setMatrix(dA)
+    s.setPrecision(precision)
+    re, im = np.zeros(k), np.zeros(k)
+    s.setOutputsEigenvalues(re, im)
+    vecs = [DeviceMatrix(1, n) for _ in range(k)]
+    s.setOutputsEigenvectors(vecs)
+    s.execute(m, k, MatrixInfo(True, False, n, n))
+    np.testing.assert_allclose(sorted(re), sorted(vals)[-k:], rtol=1e-12)
+    assert np.all(np.abs(im) <= 1e-12)
+    check_residuals(s, A, re + 1j * im, vecs, k)
+
+
 def test_check_counter_and_first_stop(oracle):
     A = oracle.gen_c0(64, 0x5EED0000)
     dA = DeviceMatrix.from_numpy(A, None)
