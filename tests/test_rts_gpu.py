@@ -193,3 +193,32 @@ def test_solver_through_implicit_product(api, rng):
     got = np.sort(re)[::-1]
     assert np.all(np.abs(got - want) <= 1e-10 * want[0]), (got, want)
     assert np.all(np.abs(im) <= 1e-10 * want[0])
+
+
+def test_solver_large_n_implicit_product(api, rng):
+    # n = 40000 unknowns: the Arnoldi steps' CGS2 then runs on 256 blocks of
+    # 192 rows each (several 64-row chunks per block, idle blocks past n);
+    # S = M M^T has rank 64 and its nonzero eigenvalues are the squared
+    # singular values of M
+    R, C = 40000, 64
+    M = rng.standard_normal((R, C)) / np.sqrt(R)
+    M[:, :4] *= np.array([9.0, 8.0, 7.0, 6.0])
+    rts = RecToSquareApi(api, dev(M, False))
+
+    def multiply(w, v, solver_api, user, mt):
+        user.multiply(w, v, solver_api)
+
+    s = CuHArnoldiCallback()
+    s.setQRType(QRType.QRGR)
+    s.setSortType(ArnUtils.SortLargestValues)
+    s.setCheckTolerance(1e-10)
+    s.setOutputType(ArnUtils.DEVICE)
+    s.setSeed(5)
+    s.setCallback(multiply, rts)
+    re, im = np.zeros(4), np.zeros(4)
+    s.setOutputsEigenvalues(re, im)
+    s.execute(24, 4, MatrixInfo(True, False, R, R))
+    want = np.sort(np.linalg.svd(M, compute_uv=False) ** 2)[::-1][:4]
+    got = np.sort(re)[::-1]
+    assert np.all(np.abs(got - want) <= 1e-10 * want[0]), (got, want)
+    assert np.all(np.abs(im) <= 1e-10 * want[0])
