@@ -1,8 +1,9 @@
 // small_dense.h -- device code for the small dense part of the Arnoldi
 // restart (complex Schur form by shifted QR, eigenvectors, wanted ordering,
-// exact-shift sweeps), shared by the single-problem kernel (k_restart) and
-// the batched solver.  Same iteration as orc_hess_schur / orc_iram in
-// oracle/oracle.c.  Every function is called by all threads of the block:
+// exact-shift sweeps) of the single-problem restart kernel (k_restart; the
+// batched solver's symmetric restart has its own code).  Same iteration as
+// orc_hess_schur / orc_iram in oracle/oracle.c.  Every function is called
+// by all threads of the block:
 // rotations split their row/column updates over the threads, barriers
 // separate the row and column halves of each rotation.
 #pragma once
@@ -502,8 +503,8 @@ __device__ bool ritz_before(cd a, cd b, int which) {
   }
 }
 
-// The restart of one problem by one workgroup (any block size; the
-// k_restart and the batched solver both call it).  smem: T and Z, mact x
+// The restart of one problem by one workgroup (any block size; k_restart
+// launches 8 waves).  smem: T and Z, mact x
 // mact complex each with the odd leading dimension lds_ld(mact), so that a
 // column walk (one row per lane) spreads over the LDS banks
 // (4 mact lds_ld(mact) doubles), then X, the eigenvectors of T (column i for
