@@ -266,25 +266,23 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     ra.do_shifts = 1;
     ra.tol = tol;
     ra.anorm = norm;
-    // a breakdown seen during this factorisation limits the active block
-    OAPH_TRY(read_status(s));
-    int bd = s->host_status.breakdown;
-    if (bd > 0) {
-      if ((uint32_t)bd < s->k || bd < 2) {
-        // invariant subspace smaller than k: continue from a fresh vector
-        int zero = 0;
-        OAPH_HIP(hipMemcpyAsync(&s->status.as<Status>()->breakdown, &zero, sizeof(int),
-                                hipMemcpyHostToDevice, st));
-        OAPH_TRY(fresh_column(s, (uint32_t)bd, s->seed + 7919u * (cycle + 1)));
-        start = (uint32_t)bd;
-        if (cycle >= s->max_restarts) break;
-        continue;
-      }
-      ra.mact = bd;
-      ra.do_shifts = 0;
-    }
+    // a breakdown seen during this factorisation limits the active block;
+    // the restart kernel reads it from the status itself (no host round
+    // trip before the launch)
     OAPH_TRY(restart(st, ra));
     OAPH_TRY(read_status(s));
+    const int bd = s->host_status.breakdown;
+    if (bd > 0 && ((uint32_t)bd < s->k || bd < 2)) {
+      // invariant subspace smaller than k (the kernel did nothing): continue
+      // from a fresh vector
+      int zero = 0;
+      OAPH_HIP(hipMemcpyAsync(&s->status.as<Status>()->breakdown, &zero, sizeof(int),
+                              hipMemcpyHostToDevice, st));
+      OAPH_TRY(fresh_column(s, (uint32_t)bd, s->seed + 7919u * (cycle + 1)));
+      start = (uint32_t)bd;
+      if (cycle >= s->max_restarts) break;
+      continue;
+    }
     s->stats.restarts++;
     const Status& hs = s->host_status;
     if (hs.schur_fail) {

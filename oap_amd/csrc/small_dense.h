@@ -514,7 +514,21 @@ __device__ bool ritz_before(cd a, cd b, int which) {
 // order), and with a.do_shifts the compressed H and the accumulated Q of the
 // exact shifts.
 __device__ void restart_body(const RestartArgs& a, double* smem) {
-  const int mm = a.mact, ld = lds_ld(a.mact);
+  // A breakdown flagged during this factorisation (the same value for every
+  // thread: the orthogonalisation kernels have finished) limits the active
+  // block to the invariant leading block, with no shifts; one below k or 2
+  // leaves the restart to the host (a fresh vector), so nothing is done.
+  int mm = a.mact;
+  bool do_shifts = a.do_shifts != 0;
+  {
+    const int bd = a.st->breakdown;
+    if (bd > 0) {
+      if (bd < a.k || bd < 2) return;  // uniform: every thread returns
+      if (bd < mm) mm = bd;
+      do_shifts = false;
+    }
+  }
+  const int ld = lds_ld(mm);
   Mat T{smem, smem + mm * ld, ld};
   Mat Z{smem + 2 * mm * ld, smem + 3 * mm * ld, ld};
   __shared__ cd ev[kMaxM];
@@ -558,9 +572,10 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   __syncthreads();
   const double small = fmax(tnorm_s * 2.220446049250313e-16, 1e-300);
   // eigenvectors of T by back substitution: lane i owns column i of X
-  // X in LDS when it fits (each lane's dependent back substitution then
-  // reads LDS, not L2), else the global work a.X
-  const bool xl = x_in_lds(mm);
+  // X in LDS when it fits in what the launch allocated for a.mact (each
+  // lane's dependent back substitution then reads LDS, not L2), else the
+  // global work a.X
+  const bool xl = 6 * mm * ld * (int)sizeof(double) <= restart_lds_bytes(a.mact);
   double* Xre = xl ? smem + 4 * mm * ld : a.Xre;
   double* Xim = xl ? smem + 5 * mm * ld : a.Xim;
   const size_t xld = xl ? (size_t)ld : (size_t)a.m;
@@ -655,7 +670,7 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   }
   __syncthreads();
   if (lane == 0) st->clk[4] = clock64();
-  if (!a.do_shifts) return;  // uniform: every thread returns
+  if (!do_shifts) return;  // uniform: every thread returns
   // exact shifts: T <- H, Q <- I (Q reuses Z), sweeps with the unwanted values
   const int kk = kk_s;
   for (int idx = lane; idx < mm * mm; idx += nthr) {
