@@ -69,6 +69,7 @@ struct oaph_arnoldi {
   const oaph_matrix* a32_of = nullptr;
   bool a32_exact = false;  // every operator entry is an FP32 value
   Status host_status{};
+  Status* pinned_status = nullptr;  // page-locked landing buffer of read_status
   std::vector<double> ev_re, ev_im, resid;
   oaph_arnoldi_stats stats{};
   bool executed = false;
@@ -149,10 +150,16 @@ oaph_status apply_operator(oaph_arnoldi* s, double* xre, double* xim, double* yr
   return OAPH_OK;
 }
 
+// The status lands in page-locked memory (a direct copy, no staging through
+// a pageable bounce buffer: the host waits for it once per restart cycle).
 oaph_status read_status(oaph_arnoldi* s) {
-  OAPH_HIP(hipMemcpyAsync(&s->host_status, s->status.ptr, sizeof(Status), hipMemcpyDeviceToHost,
+  if (!s->pinned_status)
+    OAPH_HIP(hipHostMalloc(reinterpret_cast<void**>(&s->pinned_status), sizeof(Status),
+                           hipHostMallocDefault));
+  OAPH_HIP(hipMemcpyAsync(s->pinned_status, s->status.ptr, sizeof(Status), hipMemcpyDeviceToHost,
                           s->api->stream));
   OAPH_HIP(hipStreamSynchronize(s->api->stream));
+  s->host_status = *s->pinned_status;
   return OAPH_OK;
 }
 
@@ -468,6 +475,7 @@ oaph_status oaph_arnoldi_destroy(oaph_arnoldi* s) {
   for (DevBuf* b : {&s->V, &s->V2, &s->W, &s->F, &s->H, &s->Q, &s->Y, &s->X, &s->AX, &s->coef,
                     &s->partials, &s->counter, &s->status, &s->A32})
     b->release();
+  if (s->pinned_status) (void)hipHostFree(s->pinned_status);
   oaph_api_destroy(s->api);
   delete s;
   return OAPH_OK;
