@@ -349,7 +349,10 @@ oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
 
 /* Average duration (HIP events on the solver's stream) of the solver's own
  * operator application for the dense operator set with
- * oaph_arnoldi_set_matrix: FP64 (low = 0) or the FP32 copy (low = 1). */
+ * oaph_arnoldi_set_matrix: FP64 (low = 0) or the FP32 copy (low = 1), as the
+ * last Arnoldi step of a cycle launches it (the matvec kernel also forms the
+ * per-block partial sums of V^H y against the m basis vectors, the first
+ * Gram-Schmidt pass's reduction, when its grid has at most 256 blocks). */
 oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* a, int low, uint32_t iters, double* avg_ms);
 /* Shader-clock stamps of the phases of the last restart kernel: load+Schur,
  * eigenvectors+estimates, ordering, wanted Ritz vectors, exact shifts

@@ -26,6 +26,39 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// The projection epilogue (MvProj): the block's RB output rows are in ys;
+// lane c sums conj(V[c][r]) y[r] over the rows r = wave, wave + 4, ... of the
+// block, and the four waves' sums are added in wave order.
+template <int RB>
+__device__ void block_project(const MvProj& pj, uint32_t rows, const double2* ys) {
+  __shared__ double2 acc[4][64];
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const uint32_t rb = blockIdx.x * RB;
+  const uint32_t nb = rows - rb < (uint32_t)RB ? rows - rb : (uint32_t)RB;
+  for (uint32_t c0 = 0; c0 < pj.ncols; c0 += 64) {
+    const uint32_t c = c0 + lane;
+    double sr = 0.0, si = 0.0;
+    if (c < pj.ncols) {
+      const double* vr = pj.Vre + (size_t)c * pj.ldv + rb;
+      const double* vi = pj.Vim + (size_t)c * pj.ldv + rb;
+      for (uint32_t r = wib; r < nb; r += 4) {
+        const double2 y = ys[r];
+        const double a = vr[r], b = vi[r];
+        sr = fma(a, y.x, fma(b, y.y, sr));
+        si = fma(a, y.y, fma(-b, y.x, si));
+      }
+    }
+    acc[wib][lane] = make_double2(sr, si);
+    __syncthreads();
+    if (wib == 0 && c < pj.ncols) {
+      const double2 p0 = acc[0][lane], p1 = acc[1][lane], p2 = acc[2][lane], p3 = acc[3][lane];
+      pj.part[(size_t)blockIdx.x * pj.ldp + c] =
+          make_double2((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
+    }
+    __syncthreads();
+  }
+}
+
 // ---------------------------------------------------------------- complex FP64 GEMV
 // y[r] = sum_c A[r][c] x[c] for r < rows.  Requires cols even and 16-byte
 // aligned rows (lda even); the launcher falls back to k_zgemv_scalar otherwise.
@@ -37,7 +70,8 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
                                                const double* __restrict__ xre,
                                                const double* __restrict__ xim,
                                                double* __restrict__ yre,
-                                               double* __restrict__ yim) {
+                                               double* __restrict__ yim, MvProj pj) {
+  __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   constexpr bool CPLX = MODE == 0;      // complex A (and x)
@@ -45,7 +79,7 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
   const uint32_t npair = cols >> 1;
   for (int g = 0; g < G; ++g) {
   const uint32_t r0 = (wave * G + g) * R;
-  if (r0 >= rows) return;
+  if (r0 >= rows) break;
   double sr[R], si[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) { sr[q] = 0.0; si[q] = 0.0; }
@@ -126,7 +160,14 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
       if (lane == q) { vr = sr[q]; vi = si[q]; }
     yre[r0 + lane] = vr;
     if (XIM) yim[r0 + lane] = vi;
+    ys[((threadIdx.x >> 6) * G + g) * R + lane] = make_double2(vr, vi);
   }
+  }
+  if constexpr (XIM) {
+    if (pj.part) {  // uniform over the grid
+      __syncthreads();
+      block_project<4 * G * R>(pj, rows, ys);
+    }
   }
 }
 
@@ -178,13 +219,14 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
                                                  const double* __restrict__ xre,
                                                  const double* __restrict__ xim,
                                                  double* __restrict__ yre,
-                                                 double* __restrict__ yim) {
+                                                 double* __restrict__ yim, MvProj pj) {
+  __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t nquad = cols >> 2;
   for (int g = 0; g < G; ++g) {
   const uint32_t r0 = (wave * G + g) * R;
-  if (r0 >= rows) return;
+  if (r0 >= rows) break;
   double sr[R], si[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) { sr[q] = 0.0; si[q] = 0.0; }
@@ -254,7 +296,12 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
       if (lane == q) { vr = sr[q]; vi = si[q]; }
     yre[r0 + lane] = vr;
     yim[r0 + lane] = vi;
+    ys[((threadIdx.x >> 6) * G + g) * R + lane] = make_double2(vr, vi);
   }
+  }
+  if (pj.part) {  // uniform over the grid
+    __syncthreads();
+    block_project<4 * G * R>(pj, rows, ys);
   }
 }
 
@@ -369,9 +416,9 @@ constexpr int kNumC = sizeof(kCShapes) / sizeof(kCShapes[0]);
 int g_zshape = 0, g_cshape = 0;
 
 using ZFn = void (*)(uint32_t, uint32_t, uint32_t, const double*, const double*, const double*,
-                     const double*, double*, double*);
+                     const double*, double*, double*, MvProj);
 using CFn = void (*)(uint32_t, uint32_t, uint32_t, const float*, const float*, const double*,
-                     const double*, double*, double*);
+                     const double*, double*, double*, MvProj);
 
 // kZ / kC: each shape as listed; kZ1 / kC1: what runs when the grouped grid
 // would leave SIMDs without a wave -- the shape itself when G = 1, else the
@@ -417,11 +464,20 @@ bool use_groups(const Shape& sh, uint32_t rows) {
 static_assert(sizeof(kZ<0>) / sizeof(ZFn) == kNumZ, "one kernel per FP64 shape");
 static_assert(sizeof(kC<true>) / sizeof(CFn) == kNumC, "one kernel per FP32 shape");
 
+// The epilogue arguments a launch of grid blocks passes: the caller's when it
+// asked for the projection, y is complex and the partial sums fit (blocks set
+// to the grid size), else none (blocks = 0).
+MvProj fused_projection(MvProj* pj, bool complex_y, uint32_t grid) {
+  if (!pj) return MvProj{};
+  pj->blocks = (complex_y && pj->part && grid <= pj->maxparts) ? grid : 0;
+  return pj->blocks ? *pj : MvProj{};
+}
+
 }  // namespace
 
 oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                          const double* are, const double* aim, const double* xre,
-                         const double* xim, double* yre, double* yim) {
+                         const double* xim, double* yre, double* yim, MvProj* pj) {
   // MODE 0: complex A, complex x; 1: real A, real x; 2: real A, complex x
   const int mode = aim ? 0 : (xim ? 2 : 1);
   const bool vec_ok = (cols % 2 == 0) && (lda % 2 == 0) &&
@@ -435,8 +491,11 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
     const dim3 grid((waves + 3) / 4);
     const ZFn fn = grp ? (mode == 0 ? kZ<0>[g_zshape] : mode == 1 ? kZ<1>[g_zshape] : kZ<2>[g_zshape])
                        : (mode == 0 ? kZ1<0>[g_zshape] : mode == 1 ? kZ1<1>[g_zshape] : kZ1<2>[g_zshape]);
-    hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim);
+    const MvProj kp = fused_projection(pj, mode != 1, grid.x);
+    hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim,
+                       kp);
   } else {
+    if (pj) pj->blocks = 0;
     dim3 grid((rows + 3) / 4);
     if (mode == 0)
       hipLaunchKernelGGL(k_zgemv_scalar<0>, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim);
@@ -451,7 +510,7 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
 
 oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                            const float* are, const float* aim, const double* xre,
-                           const double* xim, double* yre, double* yim) {
+                           const double* xim, double* yre, double* yim, MvProj* pj) {
   OAPH_CHECK(cols % 4 == 0 && lda % 4 == 0, OAPH_ERR_UNSUPPORTED);
   OAPH_CHECK(((((uintptr_t)are) | ((uintptr_t)aim) | ((uintptr_t)xre) | ((uintptr_t)xim)) & 15) == 0,
              OAPH_ERR_UNSUPPORTED);
@@ -463,8 +522,9 @@ oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   const dim3 grid((waves + 3) / 4);
   const CFn fn = grp ? (aim ? kC<true>[g_cshape] : kC<false>[g_cshape])
                      : (aim ? kC1<true>[g_cshape] : kC1<false>[g_cshape]);
+  const MvProj kp = fused_projection(pj, true, grid.x);
   hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda,
-                     are, aim, xre, xim, yre, yim);
+                     are, aim, xre, xim, yre, yim, kp);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

@@ -89,9 +89,22 @@ int reduce_blocks_for(size_t n);
 // gemv.hip
 oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
                         const oaph_matrix* b, uint32_t W, uint32_t H, uint32_t K, bool dims_mode);
+// Optional epilogue of a matvec y = A x with complex x, y: each workgroup
+// also writes part[block * ldp + c] = sum over its rows r of conj(V[c][r]) y[r]
+// for c < ncols (V column c at Vre + c * ldv), the per-block partial sums of
+// V^H y that the first Gram-Schmidt pass of an Arnoldi step needs.  The
+// launcher fuses it when the grid has at most maxparts workgroups and sets
+// blocks to the grid size; blocks = 0: not fused, y only.
+struct MvProj {
+  const double* Vre = nullptr;
+  const double* Vim = nullptr;
+  uint32_t ldv = 0, ncols = 0, ldp = 0, maxparts = 0;
+  double2* part = nullptr;
+  uint32_t blocks = 0;
+};
 oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                          const double* are, const double* aim, const double* xre,
-                         const double* xim, double* yre, double* yim);
+                         const double* xim, double* yre, double* yim, MvProj* pj = nullptr);
 // out(x, y) = sum_k A[y][k] B[x][k] (A * B^T, B read in place), W x H, K terms
 oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, const double* are,
                            const double* aim, uint32_t lda, const double* bre, const double* bim,
@@ -99,7 +112,7 @@ oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, co
 // FP32 operand (aim NULL: real operator), complex FP64 x and y
 oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                            const float* are, const float* aim, const double* xre,
-                           const double* xim, double* yre, double* yim);
+                           const double* xim, double* yre, double* yim, MvProj* pj = nullptr);
 // small.hip
 oaph_status launch_qrgr(hipStream_t s, oaph_matrix* Q, oaph_matrix* R, const oaph_matrix* A);
 oaph_status launch_is_upper_triangular(hipStream_t s, const oaph_matrix* m, int* dflag);

@@ -58,11 +58,16 @@ struct RestartArgs {
 // One Arnoldi step's CGS2 against V[:, 0..ncols): cgs_project (V^H x),
 // cgs_update_project (f = x - V h, h stored), cgs_finish (f -= V s, s stored,
 // H column hcol and beta into row hrow when hcol >= 0, v = f / beta when vre
-// is given).  The three launches hand partial sums over w.partials.
+// is given).  The three launches hand partial sums over w.partials.  The
+// matvec that forms x can write cgs_project's partial sums itself
+// (cgs_matvec_projection); cgs_update_project then sums p1_parts of them
+// (0: the cgs_project launch's).
 oaph_status cgs_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
                         const double* xim);
+MvProj cgs_matvec_projection(const Work& w, uint32_t ncols);
 oaph_status cgs_update_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
-                               const double* xim, double* fre, double* fim, double2* h);
+                               const double* xim, double* fre, double* fim, double2* h,
+                               uint32_t p1_parts = 0);
 oaph_status cgs_finish(hipStream_t s, const Work& w, uint32_t ncols, double* fre, double* fim,
                        const double2* h, double2* sc, int hcol, int hrow, double* vre,
                        double* vim);

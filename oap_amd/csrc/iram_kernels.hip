@@ -153,6 +153,7 @@ __device__ void sum_partials(const double2* __restrict__ part, uint32_t nparts, 
 
 struct CgsArgs {
   uint32_t n, ncols, rpb, nparts;
+  uint32_t nin;  // partial sums in part_in (MODE 1: P1 may come from the matvec's blocks)
   const double* Vre;
   const double* Vim;
   const double* xre;
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(kOrthThreads) void k_cgs(CgsArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t ncols = a.ncols, n = a.n;
   if (MODE >= 1) {
-    sum_partials(a.part_in, a.nparts, ncols, coef);
+    sum_partials(a.part_in, a.nin, ncols, coef);
     if (blockIdx.x == 0)
       for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) a.coef_out[c] = coef[c];
   }
@@ -495,6 +496,7 @@ CgsArgs cgs_args(const Work& w, uint32_t ncols) {
   a.n = w.n;
   a.ncols = ncols;
   a.nparts = nb;
+  a.nin = nb;
   a.rpb = ((w.n + nb - 1) / nb + 63) / 64 * 64;
   a.Vre = w.Vre;
   a.Vim = w.Vim;
@@ -534,10 +536,25 @@ oaph_status cgs_project(hipStream_t s, const Work& w, uint32_t ncols, const doub
   return OAPH_OK;
 }
 
+MvProj cgs_matvec_projection(const Work& w, uint32_t ncols) {
+  MvProj p;
+  p.Vre = w.Vre;
+  p.Vim = w.Vim;
+  p.ldv = w.n;
+  p.ncols = ncols;
+  p.ldp = kCgsStride;
+  p.maxparts = kCgsMaxBlocks;
+  p.part = cgs_p1(w);
+  return p;
+}
+
 oaph_status cgs_update_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
-                               const double* xim, double* fre, double* fim, double2* h) {
+                               const double* xim, double* fre, double* fim, double2* h,
+                               uint32_t p1_parts) {
   OAPH_CHECK(ncols >= 1 && ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(p1_parts <= kCgsMaxBlocks, OAPH_ERR_ARG);
   CgsArgs a = cgs_args(w, ncols);
+  if (p1_parts) a.nin = p1_parts;
   a.xre = xre;
   a.xim = xim;
   a.fre = fre;
