@@ -26,36 +26,55 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
-// The projection epilogue (MvProj): the block's RB output rows are in ys;
-// lane c sums conj(V[c][r]) y[r] over the rows r = wave, wave + 4, ... of the
-// block, and the four waves' sums are added in wave order.
+// The projection epilogue (MvProj; ncols <= 64): lane c sums conj(V[c][r]) y[r]
+// over the rows r = wave, wave + 4, ... of the block, and the four waves' sums
+// are added in wave order.  The V entries are loaded into registers when the
+// FP32-operand kernel starts (project_prefetch), so their latency hides
+// behind the operator stream; the block's rows of y are in ys.
 template <int RB>
-__device__ void block_project(const MvProj& pj, uint32_t rows, const double2* ys) {
+struct ProjRegs {
+  double a[RB / 4], b[RB / 4];
+};
+
+template <int RB>
+__device__ __forceinline__ void project_prefetch(const MvProj& pj, uint32_t rows,
+                                                 ProjRegs<RB>& v) {
+  const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
+  const uint32_t rb = blockIdx.x * RB;
+  const bool col = (uint32_t)lane < pj.ncols;
+  const double* vr = pj.Vre + (size_t)(col ? lane : 0) * pj.ldv + rb;
+  const double* vi = pj.Vim + (size_t)(col ? lane : 0) * pj.ldv + rb;
+#pragma unroll
+  for (int i = 0; i < RB / 4; ++i) {
+    const uint32_t r = wib + 4 * i;
+    const bool in = col && rb + r < rows;
+    v.a[i] = in ? vr[r] : 0.0;
+    v.b[i] = in ? vi[r] : 0.0;
+  }
+}
+
+template <int RB>
+__device__ void block_project(const MvProj& pj, uint32_t rows, const double2* ys,
+                              const ProjRegs<RB>& v) {
   __shared__ double2 acc[4][64];
   const int lane = threadIdx.x & 63, wib = threadIdx.x >> 6;
   const uint32_t rb = blockIdx.x * RB;
-  const uint32_t nb = rows - rb < (uint32_t)RB ? rows - rb : (uint32_t)RB;
-  for (uint32_t c0 = 0; c0 < pj.ncols; c0 += 64) {
-    const uint32_t c = c0 + lane;
-    double sr = 0.0, si = 0.0;
-    if (c < pj.ncols) {
-      const double* vr = pj.Vre + (size_t)c * pj.ldv + rb;
-      const double* vi = pj.Vim + (size_t)c * pj.ldv + rb;
-      for (uint32_t r = wib; r < nb; r += 4) {
-        const double2 y = ys[r];
-        const double a = vr[r], b = vi[r];
-        sr = fma(a, y.x, fma(b, y.y, sr));
-        si = fma(a, y.y, fma(-b, y.x, si));
-      }
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int i = 0; i < RB / 4; ++i) {
+    const uint32_t r = wib + 4 * i;
+    if (rb + r < rows) {
+      const double2 y = ys[r];
+      sr = fma(v.a[i], y.x, fma(v.b[i], y.y, sr));
+      si = fma(v.a[i], y.y, fma(-v.b[i], y.x, si));
     }
-    acc[wib][lane] = make_double2(sr, si);
-    __syncthreads();
-    if (wib == 0 && c < pj.ncols) {
-      const double2 p0 = acc[0][lane], p1 = acc[1][lane], p2 = acc[2][lane], p3 = acc[3][lane];
-      pj.part[(size_t)blockIdx.x * pj.ldp + c] =
-          make_double2((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
-    }
-    __syncthreads();
+  }
+  acc[wib][lane] = make_double2(sr, si);
+  __syncthreads();
+  if (wib == 0 && (uint32_t)lane < pj.ncols) {
+    const double2 p0 = acc[0][lane], p1 = acc[1][lane], p2 = acc[2][lane], p3 = acc[3][lane];
+    pj.part[(size_t)blockIdx.x * pj.ldp + lane] =
+        make_double2((p0.x + p1.x) + (p2.x + p3.x), (p0.y + p1.y) + (p2.y + p3.y));
   }
 }
 
@@ -165,8 +184,12 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
   }
   if constexpr (XIM) {
     if (pj.part) {  // uniform over the grid
+      // V loaded here, not at the start: registers held across the FP64
+      // stream slowed it (171 vs 163 us per launch at n = 8192)
+      ProjRegs<4 * G * R> pv;
+      project_prefetch(pj, rows, pv);
       __syncthreads();
-      block_project<4 * G * R>(pj, rows, ys);
+      block_project<4 * G * R>(pj, rows, ys, pv);
     }
   }
 }
@@ -224,6 +247,8 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t nquad = cols >> 2;
+  ProjRegs<4 * G * R> pv;
+  if (pj.part) project_prefetch(pj, rows, pv);
   for (int g = 0; g < G; ++g) {
   const uint32_t r0 = (wave * G + g) * R;
   if (r0 >= rows) break;
@@ -301,7 +326,7 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
   }
   if (pj.part) {  // uniform over the grid
     __syncthreads();
-    block_project<4 * G * R>(pj, rows, ys);
+    block_project<4 * G * R>(pj, rows, ys, pv);
   }
 }
 
@@ -469,7 +494,7 @@ static_assert(sizeof(kC<true>) / sizeof(CFn) == kNumC, "one kernel per FP32 shap
 // to the grid size), else none (blocks = 0).
 MvProj fused_projection(MvProj* pj, bool complex_y, uint32_t grid) {
   if (!pj) return MvProj{};
-  pj->blocks = (complex_y && pj->part && grid <= pj->maxparts) ? grid : 0;
+  pj->blocks = (complex_y && pj->part && pj->ncols <= 64 && grid <= pj->maxparts) ? grid : 0;
   return pj->blocks ? *pj : MvProj{};
 }
 

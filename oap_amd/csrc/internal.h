@@ -93,8 +93,8 @@ oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
 // also writes part[block * ldp + c] = sum over its rows r of conj(V[c][r]) y[r]
 // for c < ncols (V column c at Vre + c * ldv), the per-block partial sums of
 // V^H y that the first Gram-Schmidt pass of an Arnoldi step needs.  The
-// launcher fuses it when the grid has at most maxparts workgroups and sets
-// blocks to the grid size; blocks = 0: not fused, y only.
+// launcher fuses it when ncols <= 64 and the grid has at most maxparts
+// workgroups, and sets blocks to the grid size; blocks = 0: not fused, y only.
 struct MvProj {
   const double* Vre = nullptr;
   const double* Vim = nullptr;
