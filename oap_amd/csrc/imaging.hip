@@ -270,25 +270,41 @@ using namespace oaph;
 
 namespace {
 
-// pixel = r<<16 | g<<8 | b (grey: p, p, p; RGBA: alpha ignored); value =
-// pixel / 0xFFFFFF, over the window rows and columns (row-major)
-oaph_status image_column(const oaph_dataloader* d, uint32_t i, double* out) {
+// pixel = r<<16 | g<<8 | b (grey: p, p, p; RGBA: alpha ignored) over the
+// window rows and columns (row-major); the column value is pixel / 0xFFFFFF
+oaph_status image_pixels(const oaph_dataloader* d, uint32_t i, uint32_t* out) {
   const Png& im = d->images[i];
   const Window W = d->plan.out_w[i], H = d->plan.out_h[i];
   OAPH_CHECK(W.begin + W.size <= im.width && H.begin + H.size <= im.height, OAPH_ERR_SHAPE);
   OAPH_CHECK(im.colors == 1 || im.colors == 3 || im.colors == 4, OAPH_ERR_UNSUPPORTED);
   const size_t stride = (size_t)im.width * im.colors;
-  const double maxp = 16777215.0;
   size_t k = 0;
   for (uint32_t y = H.begin; y < H.begin + H.size; ++y) {
     const uint8_t* row = &im.rows[(size_t)y * stride];
     for (uint32_t x = W.begin; x < W.begin + W.size; ++x) {
       const uint8_t* p = row + (size_t)x * im.colors;
       const uint32_t r = p[0], g = im.colors == 1 ? p[0] : p[1], b = im.colors == 1 ? p[0] : p[2];
-      out[k++] = (double)(r << 16 | g << 8 | b) / maxp;
+      out[k++] = r << 16 | g << 8 | b;
     }
   }
   return OAPH_OK;
+}
+
+constexpr double kMaxPixel = 16777215.0;
+
+oaph_status image_column(const oaph_dataloader* d, uint32_t i, double* out) {
+  std::vector<uint32_t> px(d->length);
+  OAPH_TRY(image_pixels(d, i, px.data()));
+  for (uint32_t k = 0; k < d->length; ++k) out[k] = (double)px[k] / kMaxPixel;
+  return OAPH_OK;
+}
+
+// value = pixel / 0xFFFFFF (correctly rounded division, the host's bits)
+__global__ __launch_bounds__(256) void k_pixel_values(const uint32_t* __restrict__ px,
+                                                      double* __restrict__ out, size_t n) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    out[i] = (double)px[i] / kMaxPixel;
 }
 
 }  // namespace
@@ -458,16 +474,49 @@ oaph_status oaph_dataloader_column(const oaph_dataloader* d, uint32_t image, dou
   return image_column(d, image, out);
 }
 
+// M (one image per column): each image's packed pixels are gathered on the
+// host, contiguous per image, uploaded once (4 bytes a pixel), converted to
+// values and transposed into place on the device (a host scatter at a
+// stride of one row of M per pixel was most of the assembly time).
 oaph_status oaph_dataloader_matrix(const oaph_dataloader* d, oaph_matrix* m) {
   OAPH_CHECK(d && m && m->reValues, OAPH_ERR_ARG);
   const uint32_t cols = (uint32_t)d->images.size();
   OAPH_CHECK(m->columns == cols && m->rows == d->length, OAPH_ERR_SHAPE);
-  std::vector<double> host((size_t)cols * d->length), col(d->length);
-  for (uint32_t i = 0; i < cols; ++i) {
-    OAPH_TRY(image_column(d, i, col.data()));
-    for (uint32_t r = 0; r < d->length; ++r) host[(size_t)r * cols + i] = col[r];
+  const size_t n = (size_t)cols * d->length;
+  if ((d->length + 31) / 32 > 65535) {  // beyond the transpose grid: scatter on the host
+    std::vector<double> rows(n), col(d->length);
+    for (uint32_t i = 0; i < cols; ++i) {
+      OAPH_TRY(image_column(d, i, col.data()));
+      for (uint32_t r = 0; r < d->length; ++r) rows[(size_t)r * cols + i] = col[r];
+    }
+    return oaph_matrix_copy_h2d(m, rows.data(), nullptr);
   }
-  return oaph_matrix_copy_h2d(m, host.data(), nullptr);
+  std::vector<uint32_t> host(n);
+  for (uint32_t i = 0; i < cols; ++i) OAPH_TRY(image_pixels(d, i, host.data() + (size_t)i * d->length));
+  OAPH_TRY(ensure_init());
+  DevBuf px, vals;
+  oaph_status st = px.reserve(n * sizeof(uint32_t));
+  if (st == OAPH_OK) st = vals.reserve(n * sizeof(double));
+  if (st == OAPH_OK &&
+      hipMemcpy(px.ptr, host.data(), n * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess)
+    st = OAPH_ERR_HIP;
+  if (st == OAPH_OK) {
+    const size_t blocks = (n + 255) / 256;
+    hipLaunchKernelGGL(k_pixel_values, dim3(blocks < 65536 ? (unsigned)blocks : 65536u), dim3(256),
+                       0, nullptr, px.as<uint32_t>(), vals.as<double>(), n);
+    if (hipGetLastError() != hipSuccess) st = OAPH_ERR_HIP;
+  }
+  if (st == OAPH_OK) {
+    // images as the rows of a cols x length matrix; M is its transpose
+    oaph_matrix a{d->length, cols, d->length, cols, vals.as<double>(), nullptr};
+    oaph_matrix out = *m;
+    out.imValues = nullptr;
+    st = launch_transpose(nullptr, &out, &a, false);
+    if (st == OAPH_OK && hipStreamSynchronize(nullptr) != hipSuccess) st = OAPH_ERR_HIP;
+  }
+  px.release();
+  vals.release();
+  return st;
 }
 
 }  // extern "C"

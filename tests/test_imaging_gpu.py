@@ -32,3 +32,16 @@ def test_eigenimages_of_a_synthetic_stack(tmp_path, rng):
     for lam, v in zip(out.eigenvalues, out.eigenvectors):
         assert np.linalg.norm(S @ v - lam * v) <= 1e-10 * want[0] * np.linalg.norm(v)
     assert np.all(out.errors <= 1e-10 * np.linalg.norm(S))
+
+
+@pytest.mark.parametrize("count,H,W", [(3, 5, 7), (37, 33, 29), (64, 40, 40)])
+def test_data_matrix_is_the_stacked_columns(tmp_path, rng, count, H, W):
+    # createMatrix (image-major upload + device transpose) holds exactly the
+    # columns createColumnVector gives, image i in column i
+    for i in range(count):
+        imaging.write_png(tmp_path / f"img{i}.png", rng.integers(0, 256, (H, W, 3), dtype=np.uint8))
+    loader = imaging.DataLoader.from_dir(tmp_path, "img", count)
+    want = np.stack([loader.createColumnVector(i) for i in range(count)], axis=1)
+    got = loader.createMatrix().to_complex()
+    assert got.shape == want.shape
+    assert np.array_equal(got.real, want)
