@@ -76,3 +76,38 @@ def test_fp32_shapes_bit_identical():
     for e, _, _ in ev:
         for w in want:
             assert min(abs(g - w) for g in e) <= 1e-10 * abs(want[0])
+
+
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_shapes_in_solves_at_8192(fp32):
+    # n = 8192 runs every shape with its row groups (workgroups of 16 or 32
+    # rows), and the Arnoldi steps take the first Gram-Schmidt pass's partial
+    # sums from the matvec's epilogue: each shape's solve must converge to
+    # the same eigenvalues, with explicit residuals within the tolerance
+    from oap_amd.api import CuHArnoldiCallback, CuProceduresApi, MatrixInfo, Precision
+    lib = load()
+    n, k = 8192, 4
+    A = DeviceMatrix(n, n)
+    check(lib.oaph_synth_matrix(A.ptr, 1, 0x5EED0001, 1e-2, 500.0), "synth")
+    fro = CuProceduresApi().magnitude(A)
+    runs = []
+    try:
+        for v, sh in enumerate(shapes(fp32)):
+            check(lib.oaph_set_matvec_shape(fp32, v), "shape")
+            s = CuHArnoldiCallback()
+            s.setMatrix(A)
+            s.setPrecision(Precision.MIXED_FP32 if fp32 else Precision.FP64)
+            s.setSortType(0)
+            s.setCheckTolerance(1e-10)
+            s.setSeed(11)
+            re, im = np.zeros(k), np.zeros(k)
+            s.setOutputsEigenvalues(re, im)
+            s.execute(24, k, MatrixInfo(True, True, n, n))
+            for t in range(k):
+                assert s.testOutcome(t) <= 1e-10 * fro, (sh, t, s.testOutcome(t))
+            runs.append((sh, np.sort_complex(re + 1j * im)))
+    finally:
+        check(lib.oaph_set_matvec_shape(fp32, 0), "shape")
+    ref = runs[0][1]
+    for sh, ev in runs[1:]:
+        assert np.all(np.abs(ev - ref) <= 1e-10 * np.abs(ref).max()), (sh, ev, ref)
