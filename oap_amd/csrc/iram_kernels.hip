@@ -281,13 +281,38 @@ __global__ __launch_bounds__(kOrthThreads) void k_cgs(CgsArgs a) {
     if (MODE == 1) nrm += xr * xr + xi * xi;
   }
   if (MODE <= 1) {
+    // the per-lane sums of every column reduced over the lanes through LDS
+    // (one plane at a time): thread t sums lanes 16 (t % 4) .. +15 of
+    // column t / 4 in order, and the four quarters are added as
+    // (q0 + q1) + (q2 + q3)
+    __shared__ double red[kMaxM + 1][64];
+    double out[2];
 #pragma unroll
-    for (int q = 0; q < kCPW; ++q) {
-      const uint32_t c = wave + 4 * q;
-      if (c < ncols) {
-        const double pr = wave_sum(sr[q]), pi = wave_sum(si[q]);
-        if (lane == 0) a.part_out[(size_t)blockIdx.x * kCgsStride + c] = make_double2(pr, pi);
+    for (int plane = 0; plane < 2; ++plane) {
+      if (plane) __syncthreads();  // the first plane's reads are done
+#pragma unroll
+      for (int q = 0; q < kCPW; ++q) {
+        const uint32_t c = wave + 4 * q;
+        if (c < ncols) red[c][lane] = plane ? si[q] : sr[q];
       }
+      __syncthreads();
+      double v = 0.0;
+      const uint32_t c = threadIdx.x >> 2, l0 = (threadIdx.x & 3) * 16;
+      if (c < ncols) {
+#pragma unroll
+        for (int l = 0; l < 16; ++l) v += red[c][l0 + l];
+      }
+      v += __shfl_xor(v, 1, 64);
+      v += __shfl_xor(v, 2, 64);
+      out[plane] = v;
+    }
+    const uint32_t c = threadIdx.x >> 2;
+    if ((threadIdx.x & 3) == 0 && c < ncols)
+      a.part_out[(size_t)blockIdx.x * kCgsStride + c] = make_double2(out[0], out[1]);
+    if (ncols > 64) {  // column 64 (ncols = kMaxM + 1)
+      __syncthreads();
+      const double pr = wave_sum(wave == 0 ? sr[16] : 0.0), pi = wave_sum(wave == 0 ? si[16] : 0.0);
+      if (threadIdx.x == 0) a.part_out[(size_t)blockIdx.x * kCgsStride + 64] = make_double2(pr, pi);
     }
   }
   if (MODE == 1 && wave == 0) {
