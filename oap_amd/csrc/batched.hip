@@ -509,15 +509,21 @@ __global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
 #pragma unroll
             for (int q = 0; q < CPT; ++q) p[u] += vv[u][q] * w[q];
           }
+          // the four sums over the lanes with halves exchanged: partners
+          // 32 apart swap two of the four partials, partners 16 apart one of
+          // the remaining two, then one value per lane (7 shuffles, not 24);
+          // lane 16 u ends with column l0 + u
+          const int lane = tid & 63;
+          const bool h32 = (lane & 32) != 0, h16 = (lane & 16) != 0;
+          double a0 = h32 ? p[2] : p[0], a1 = h32 ? p[3] : p[1];
+          a0 += __shfl_xor(h32 ? p[0] : p[2], 32, 64);
+          a1 += __shfl_xor(h32 ? p[1] : p[3], 32, 64);
+          double k = h16 ? a1 : a0;
+          k += __shfl_xor(h16 ? a0 : a1, 16, 64);
 #pragma unroll
-          for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-            for (int u = 0; u < 4; ++u) p[u] += __shfl_xor(p[u], off, 64);
-          if ((tid & 63) == 0) {
-#pragma unroll
-            for (int u = 0; u < 4; ++u)
-              if (l0 + u <= j) red[(tid >> 6) * (kMaxMB + 2) + l0 + u] = p[u];
-          }
+          for (int off = 8; off > 0; off >>= 1) k += __shfl_xor(k, off, 64);
+          if ((lane & 15) == 0 && l0 + (lane >> 4) <= j)
+            red[(tid >> 6) * (kMaxMB + 2) + l0 + (lane >> 4)] = k;
         }
         if (with_norm) {
           double p = 0;
