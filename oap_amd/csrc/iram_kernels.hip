@@ -359,10 +359,12 @@ __global__ __launch_bounds__(256) void k_vq(uint32_t n, uint32_t mrows, uint32_t
   __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  for (uint32_t c0 = 0; c0 < ncols; c0 += kVqChunk) {
+  {
+    const uint32_t c0 = blockIdx.y * kVqChunk;  // one chunk of columns per block row
     double ar[kVqChunk], ai[kVqChunk];
 #pragma unroll
     for (int q = 0; q < kVqChunk; ++q) { ar[q] = 0; ai[q] = 0; }
+#pragma unroll 4
     for (uint32_t l = 0; l < mrows; ++l) {
       double vr = Vre[(size_t)l * n + i], vi = Vim[(size_t)l * n + i];
 #pragma unroll
@@ -645,7 +647,9 @@ oaph_status vq(hipStream_t s, uint32_t n, uint32_t mrows, uint32_t ncols, const 
                double* Ore, double* Oim, int fcol, double* fre, double* fim, const Status* st) {
   size_t lds = (size_t)2 * mrows * ncols * sizeof(double);
   OAPH_CHECK(lds <= 64 * 1024, OAPH_ERR_UNSUPPORTED);
-  hipLaunchKernelGGL(k_vq, dim3((n + 255) / 256), dim3(256), lds, s, n, mrows, ncols, Vre, Vim,
+  // rows x column chunks: enough workgroups to spread over the CUs
+  const dim3 grid((n + 127) / 128, (ncols + kVqChunk - 1) / kVqChunk);
+  hipLaunchKernelGGL(k_vq, grid, dim3(128), lds, s, n, mrows, ncols, Vre, Vim,
                      Mre, Mim, ldm, Ore, Oim, fcol, fre, fim, st);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
