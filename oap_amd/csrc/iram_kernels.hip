@@ -341,7 +341,7 @@ __global__ __launch_bounds__(512) void k_restart(RestartArgs a) {
 // out[:, c] = sum_l V[:, l] M[l][c], c < ncols (out may not alias V).
 // When fcol >= 0, column fcol is not stored; instead
 // f <- (V M)[:, fcol] * beta_k + f * sigma (coefficients from the status).
-constexpr int kVqChunk = 8;
+constexpr int kVqChunk = 4;
 __global__ __launch_bounds__(256) void k_vq(uint32_t n, uint32_t mrows, uint32_t ncols,
                                             const double* __restrict__ Vre,
                                             const double* __restrict__ Vim,
@@ -364,7 +364,7 @@ __global__ __launch_bounds__(256) void k_vq(uint32_t n, uint32_t mrows, uint32_t
     double ar[kVqChunk], ai[kVqChunk];
 #pragma unroll
     for (int q = 0; q < kVqChunk; ++q) { ar[q] = 0; ai[q] = 0; }
-#pragma unroll 4
+#pragma unroll 8
     for (uint32_t l = 0; l < mrows; ++l) {
       double vr = Vre[(size_t)l * n + i], vi = Vim[(size_t)l * n + i];
 #pragma unroll
@@ -648,8 +648,8 @@ oaph_status vq(hipStream_t s, uint32_t n, uint32_t mrows, uint32_t ncols, const 
   size_t lds = (size_t)2 * mrows * ncols * sizeof(double);
   OAPH_CHECK(lds <= 64 * 1024, OAPH_ERR_UNSUPPORTED);
   // rows x column chunks: enough workgroups to spread over the CUs
-  const dim3 grid((n + 127) / 128, (ncols + kVqChunk - 1) / kVqChunk);
-  hipLaunchKernelGGL(k_vq, grid, dim3(128), lds, s, n, mrows, ncols, Vre, Vim,
+  const dim3 grid((n + 63) / 64, (ncols + kVqChunk - 1) / kVqChunk);
+  hipLaunchKernelGGL(k_vq, grid, dim3(64), lds, s, n, mrows, ncols, Vre, Vim,
                      Mre, Mim, ldm, Ore, Oim, fcol, fre, fim, st);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
