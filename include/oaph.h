@@ -312,7 +312,10 @@ oaph_status oaph_arnoldi_set_max_restarts(oaph_arnoldi* a, uint32_t max_restarts
  * again after changing the operator's values.  When every entry is an FP32
  * value the copy IS the operator and all cycles use it (no FP64 restart);
  * otherwise FP32 cycles to the mixed tolerance, then an FP64 restart from the
- * sum of the wanted Ritz vectors.  Residuals are explicit FP64 either way. */
+ * sum of the wanted and guard Ritz vectors.  Residuals are explicit FP64
+ * either way.  An operator whose row length is not a multiple of 4, or with
+ * an entry beyond the FP32 range, has no usable FP32 copy: the solve then
+ * runs FP64 cycles only (stats.mixed_declined = 1). */
 oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* a, int precision);
 /* Staged solve (opt-in, default none): at each tolerance tols[0] > tols[1]
  * > ... (relative to the largest |Ritz value|) the cycles stop and the solve
@@ -324,9 +327,10 @@ oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* a, int precision);
  * oaph_arnoldi_set_restart_guard).  No reference counterpart. */
 oaph_status oaph_arnoldi_set_restart_stages(oaph_arnoldi* a, const double* tols, uint32_t n);
 /* Ritz vectors after the k wanted ones (next in the wanted order) that a
- * stage restart also adds into its sum, so that a wanted eigenvalue whose
- * Ritz value is still ranked just below the k-th is kept in the new start
- * vector (default 0; at most m - k are used).  No reference counterpart. */
+ * Ritz-sum restart (the mixed flow's FP64 start, stages) also adds into its
+ * sum, so that a wanted eigenvalue whose Ritz value is still ranked just
+ * below the k-th is kept in the new start vector (default 4; at most m - k
+ * are used).  No reference counterpart. */
 oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* a, uint32_t guard);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
  * the largest Ritz value (default 1e-6) */
@@ -344,6 +348,10 @@ typedef struct oaph_arnoldi_stats {
   uint32_t fallbacks;   /* mixed flow: times the FP64 cycles met a wanted
                            eigenvalue the FP32 phase had not found and went
                            back to the FP32 operand */
+  uint32_t mixed_declined; /* 1: OAPH_PREC_MIXED_FP32 was asked for but the
+                              FP32 copy is unusable (row length not a multiple
+                              of 4, or an entry beyond the FP32 range); the
+                              solve ran FP64 cycles only */
 } oaph_arnoldi_stats;
 oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
 
@@ -385,6 +393,14 @@ oaph_status oaph_matvec_shape_info(int fp32, int variant, int* rows_per_wave,
  * the api's stream), after one warm-up call. */
 oaph_status oaph_time_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
                                   const oaph_matrix* b, uint32_t iters, double* avg_ms);
+/* Average duration of `iters` back-to-back magnitude (sumsq != 0) or sum
+ * reductions of m on the api's stream (the kernel alone: the result stays on
+ * the device), after one warm-up launch. */
+oaph_status oaph_time_reduction(oaph_api* api, int sumsq, const oaph_matrix* m, uint32_t iters,
+                                double* avg_ms);
+/* Workgroups of the magnitude/sum reductions at most (1..2048, default 256:
+ * one wave per SIMD); process-wide tuning knob. */
+oaph_status oaph_set_reduce_blocks(int blocks);
 
 /* ------------------------------------------------------------------ */
 /* Activation procedures (CuProceduresApi.h:239-290): every plane of the

@@ -48,7 +48,7 @@ class oaph_matrix_info(C.Structure):
 class oaph_arnoldi_stats(C.Structure):
     _fields_ = [("restarts", C.c_uint32), ("matvecs", C.c_uint32), ("matvecs_low", C.c_uint32),
                 ("nconv", C.c_uint32), ("seconds", C.c_double), ("max_resid", C.c_double),
-                ("fallbacks", C.c_uint32)]
+                ("fallbacks", C.c_uint32), ("mixed_declined", C.c_uint32)]
 
 
 MP = C.POINTER(oaph_matrix)
@@ -136,6 +136,8 @@ _SIGS = {
                                          C.POINTER(C.c_int)]),
     "oaph_synth_cdr": (C.c_int, [MP, U32, C.c_double, C.c_double, C.c_double]),
     "oaph_time_dot_product": (C.c_int, [C.c_void_p, MP, MP, MP, U32, DP]),
+    "oaph_time_reduction": (C.c_int, [C.c_void_p, C.c_int, MP, U32, DP]),
+    "oaph_set_reduce_blocks": (C.c_int, [C.c_int]),
     "oaph_activation": (C.c_int, [C.c_void_p, C.c_int, MP, MP]),
     "oaph_activation_dims": (C.c_int, [C.c_void_p, C.c_int, MP, MP, C.POINTER(U32)]),
     "oaph_activation_dims_periodic": (C.c_int, [C.c_void_p, C.c_int, MP, MP, C.POINTER(U32)]),

@@ -444,7 +444,8 @@ class CuHArnoldiCallback:
                                                          t.size), "setRestartStages")
 
     def setRestartGuard(self, guard):
-        """Ritz vectors after the wanted ones kept in each stage restart's sum"""
+        """Ritz vectors after the wanted ones kept in each Ritz-sum restart's
+        sum (the mixed flow's FP64 start and the stages; default 4)"""
         check(self._lib.oaph_arnoldi_set_restart_guard(self._h, guard), "setRestartGuard")
 
     def setMixedTolerance(self, tol_low):

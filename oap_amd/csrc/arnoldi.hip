@@ -49,8 +49,13 @@ struct oaph_arnoldi {
   // tolerances (relative to max |Ritz value|, descending) at which the solve
   // restarts from the sum of the wanted Ritz vectors before going on
   std::vector<double> stage_tols;
-  // Ritz vectors after the wanted ones that a stage restart keeps in its sum
-  uint32_t guard = 0;
+  // Ritz vectors after the wanted ones that a Ritz-sum restart (the mixed
+  // flow's FP64 start, stages) keeps in its sum.  Four by default: without
+  // them the mixed flow returned the 9th eigenvalue in place of the 8th for 2
+  // of 600 start vectors of the dense complex 1024 C1 construction (the 8
+  // wanted eigenvalues 0.2 % apart in modulus from the next); with four, for
+  // none.
+  uint32_t guard = 4;
   oaph_multiply_fn mfn = nullptr;
   void* muser = nullptr;
   oaph_check_fn cfn = nullptr;
@@ -67,7 +72,8 @@ struct oaph_arnoldi {
   DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32;
   size_t a32_elems = 0;
   const oaph_matrix* a32_of = nullptr;
-  bool a32_exact = false;  // every operator entry is an FP32 value
+  bool a32_exact = false;   // every operator entry is an FP32 value
+  bool a32_usable = false;  // the FP32-operand matvec can run on the copy
   Status host_status{};
   Status* pinned_status = nullptr;  // page-locked landing buffer of read_status
   std::vector<double> ev_re, ev_im, resid;
@@ -163,25 +169,34 @@ oaph_status read_status(oaph_arnoldi* s) {
   return OAPH_OK;
 }
 
-// One Arnoldi step: column j of H and v_{j+1} (CGS2; ArnoldiProcedures.cpp:416-459)
-// The FP32 copy of the dense operator (both planes) and whether it is exact.
+// The FP32 copy of the dense operator (both planes), whether it is exact and
+// whether the FP32-operand matvec can use it: its float4 row loads need a
+// row length that is a multiple of 4, and an entry beyond the FP32 range
+// would turn into an infinity.  An unusable copy is not made; the solve then
+// runs FP64 cycles.
 oaph_status make_fp32_copy(oaph_arnoldi* s) {
   hipStream_t st = s->api->stream;
+  s->a32_of = s->A;
+  s->a32_exact = false;
+  s->a32_usable = false;
+  if (s->A->columns % 4 != 0) return OAPH_OK;
   const size_t elems = (size_t)s->A->rows * s->A->columns;
   OAPH_TRY(s->A32.reserve((s->A->imValues ? 2 : 1) * elems * sizeof(float)));
   s->a32_elems = elems;
   float* a32 = s->A32.as<float>();
-  unsigned* flag = s->counter.as<unsigned>() + 48;
-  OAPH_HIP(hipMemsetAsync(flag, 0, sizeof(unsigned), st));
-  OAPH_TRY(to_f32(st, elems, s->A->reValues, a32, flag));
-  if (s->A->imValues) OAPH_TRY(to_f32(st, elems, s->A->imValues, a32 + elems, flag));
-  unsigned inexact = 1;
-  OAPH_HIP(hipMemcpyAsync(&inexact, flag, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+  unsigned* flags = s->counter.as<unsigned>() + 48;
+  OAPH_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(unsigned), st));
+  OAPH_TRY(to_f32(st, elems, s->A->reValues, a32, flags));
+  if (s->A->imValues) OAPH_TRY(to_f32(st, elems, s->A->imValues, a32 + elems, flags));
+  unsigned h[2] = {1, 1};
+  OAPH_HIP(hipMemcpyAsync(h, flags, sizeof(h), hipMemcpyDeviceToHost, st));
   OAPH_HIP(hipStreamSynchronize(st));
-  s->a32_exact = inexact == 0;
-  s->a32_of = s->A;
+  s->a32_exact = h[0] == 0;
+  s->a32_usable = h[1] == 0;
   return OAPH_OK;
 }
+
+// One Arnoldi step: column j of H and v_{j+1} (CGS2; ArnoldiProcedures.cpp:416-459)
 
 oaph_status arnoldi_step(oaph_arnoldi* s, uint32_t j, bool low) {
   hipStream_t st = s->api->stream;
@@ -284,9 +299,7 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     if (bd > 0 && ((uint32_t)bd < s->k || bd < 2)) {
       // invariant subspace smaller than k (the kernel did nothing): continue
       // from a fresh vector
-      int zero = 0;
-      OAPH_HIP(hipMemcpyAsync(&s->status.as<Status>()->breakdown, &zero, sizeof(int),
-                              hipMemcpyHostToDevice, st));
+      OAPH_HIP(hipMemsetAsync(&s->status.as<Status>()->breakdown, 0, sizeof(int), st));
       OAPH_TRY(fresh_column(s, (uint32_t)bd, s->seed + 7919u * (cycle + 1)));
       start = (uint32_t)bd;
       if (cycle >= s->max_restarts) break;
@@ -632,8 +645,12 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     norm = sqrt(h);
     s->norm_a = norm;
   }
-  const bool mixed = s->precision == OAPH_PREC_MIXED_FP32 && s->A != nullptr;
+  bool mixed = s->precision == OAPH_PREC_MIXED_FP32 && s->A != nullptr;
   if (mixed && s->a32_of != s->A) OAPH_TRY(make_fp32_copy(s));
+  if (mixed && !s->a32_usable) {
+    mixed = false;
+    s->stats.mixed_declined = 1;
+  }
   OAPH_TRY(start_vector(s));
   OAPH_TRY(normalize_v0(s));
   uint32_t cycles = 0;
@@ -644,7 +661,13 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
   const bool all_low = mixed && s->a32_exact;
   // restart from the sum of the wanted Ritz vectors (a fresh Arnoldi start)
   auto restart_from_ritz_sum = [&](uint32_t count) -> oaph_status {
-    OAPH_TRY(ritz_vectors(s, s->m, count));
+    // after a lucky breakdown the restart kernel worked on the invariant
+    // leading block only: Y holds Ritz vectors of that block (rows below it
+    // and columns beyond it are not written)
+    const int bd = s->host_status.breakdown;
+    const uint32_t mact = bd > 0 && (uint32_t)bd < s->m ? (uint32_t)bd : s->m;
+    if (count > mact) count = mact;
+    OAPH_TRY(ritz_vectors(s, mact, count));
     double* Xre = s->X.as<double>();
     OAPH_TRY(colsum(st, s->n, count, Xre, Xre + (size_t)s->ky * s->n, Vre(s, s->V, 0),
                     Vim(s, s->V, 0)));
@@ -724,6 +747,7 @@ oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* s, int low, uint32_t iters, d
     OAPH_TRY(alloc(s));
   }
   if (low && s->a32_of != s->A) OAPH_TRY(make_fp32_copy(s));
+  OAPH_CHECK(!low || s->a32_usable, OAPH_ERR_UNSUPPORTED);
   double* xre = Vre(s, s->V, 0);
   double* xim = Vim(s, s->V, 0);
   double* yre = s->W.as<double>();

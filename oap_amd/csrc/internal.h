@@ -86,6 +86,8 @@ oaph_status launch_sumsq(hipStream_t s, const double* re, const double* im, size
 oaph_status launch_sum2(hipStream_t s, const double* re, const double* im, size_t n,
                         double* partials, unsigned* counter, double* result);
 int reduce_blocks_for(size_t n);
+// magnitude (sumsq) or sum of m into api->scalars, enqueued on api->stream
+oaph_status enqueue_reduction(oaph_api* api, const oaph_matrix* m, bool sumsq);
 // gemv.hip
 oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
                         const oaph_matrix* b, uint32_t W, uint32_t H, uint32_t K, bool dims_mode);

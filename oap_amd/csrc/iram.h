@@ -86,8 +86,9 @@ oaph_status pairs(hipStream_t s, int mode, uint32_t n, uint32_t k, const double*
                   const double* Xim, const double* Yre, const double* Yim, double2* lambda,
                   double2* out, double* partials, unsigned* counter);
 size_t pairs_partials_bytes(uint32_t n, uint32_t k);
-// dst = (float)src; *inexact (may be NULL) set to 1 when an element rounds
-oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsigned* inexact);
+// dst = (float)src; flags (may be NULL): flags[0] set to 1 when an element
+// rounds, flags[1] when a finite element overflows the FP32 range
+oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsigned* flags);
 oaph_status colsum(hipStream_t s, uint32_t n, uint32_t k, const double* Xre, const double* Xim,
                    double* vre, double* vim);
 oaph_status random_vector(hipStream_t s, uint32_t n, uint64_t seed, bool cplx, double* re,

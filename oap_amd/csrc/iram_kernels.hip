@@ -452,17 +452,21 @@ __global__ __launch_bounds__(256) void k_pairs(uint32_t n, uint32_t k, const dou
 // FP64 -> FP32 copy of the operator planes (mixed-precision matvec)
 // dst = (float)src; *inexact is set to 1 (plain stores, all alike) when some
 // element is not representable in FP32
+// flags[0] <- 1 when an element rounds, flags[1] <- 1 when a finite element
+// is beyond the FP32 range (its copy is infinite)
 __global__ void k_to_f32(size_t n, const double* __restrict__ src, float* __restrict__ dst,
-                         unsigned* inexact) {
-  bool bad = false;
+                         unsigned* flags) {
+  bool bad = false, over = false;
   for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x) {
     const double v = src[i];
     const float f = (float)v;
     dst[i] = f;
     bad |= (double)f != v;
+    over |= isinf(f) && !isinf(v);
   }
-  if (bad && inexact) *inexact = 1u;
+  if (bad && flags) flags[0] = 1u;
+  if (over && flags) flags[1] = 1u;
 }
 
 // x = sum_t X[:, t] (start vector of the polishing phase), written to V[:, 0]
@@ -673,10 +677,10 @@ size_t pairs_partials_bytes(uint32_t n, uint32_t k) {
   return (size_t)((n + 255) / 256) * k * 3 * sizeof(double);
 }
 
-oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsigned* inexact) {
+oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsigned* flags) {
   size_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
-  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, n, src, dst, inexact);
+  hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, n, src, dst, flags);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

@@ -26,6 +26,21 @@ oaph_status check_api(oaph_api* api) {
 
 }  // namespace
 
+namespace oaph {
+oaph_status enqueue_reduction(oaph_api* api, const oaph_matrix* m, bool sumsq) {
+  size_t n = numel(m);
+  int blocks = reduce_blocks_for(n);
+  OAPH_TRY(api->partials.reserve((size_t)blocks * 2 * sizeof(double)));
+  double* result = api->scalars.as<double>();
+  unsigned* counter = api->counters.as<unsigned>();
+  if (sumsq)
+    return launch_sumsq(api->stream, m->reValues, m->imValues, n, api->partials.as<double>(),
+                        counter, result);
+  return launch_sum2(api->stream, m->reValues, m->imValues, n, api->partials.as<double>(), counter,
+                     result);
+}
+}  // namespace oaph
+
 extern "C" {
 
 oaph_status oaph_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
@@ -203,17 +218,8 @@ oaph_status oaph_get_vector(oaph_api* api, oaph_matrix* v, uint32_t length, cons
 }
 
 static oaph_status reduce_common(oaph_api* api, const oaph_matrix* m, bool sumsq) {
-  size_t n = numel(m);
-  int blocks = reduce_blocks_for(n);
-  OAPH_TRY(api->partials.reserve((size_t)blocks * 2 * sizeof(double)));
+  OAPH_TRY(enqueue_reduction(api, m, sumsq));
   double* result = api->scalars.as<double>();
-  unsigned* counter = api->counters.as<unsigned>();
-  if (sumsq)
-    OAPH_TRY(launch_sumsq(api->stream, m->reValues, m->imValues, n, api->partials.as<double>(),
-                          counter, result));
-  else
-    OAPH_TRY(launch_sum2(api->stream, m->reValues, m->imValues, n, api->partials.as<double>(),
-                         counter, result));
   OAPH_HIP(hipMemcpyAsync(api->host_scalars, result, 2 * sizeof(double), hipMemcpyDeviceToHost,
                           api->stream));
   OAPH_HIP(hipStreamSynchronize(api->stream));

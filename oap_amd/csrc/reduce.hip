@@ -16,7 +16,7 @@ namespace oaph {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kMaxBlocks = 2048;
+constexpr int kMaxBlocks = 2048;  // partial-sum capacity the callers reserve
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -82,75 +82,121 @@ __device__ __forceinline__ void grid_finish(double (&v)[NV], double* partials, u
 
 typedef double v2d __attribute__((ext_vector_type(2)));
 
-// sum of re^2 + im^2 (either plane may be absent)
+// Loads in flight per lane and plane: each wave streams a contiguous chunk
+// of 16-byte pairs, issuing kU loads of each plane before it consumes them,
+// with one accumulator per load slot (no dependent FMA chain across slots).
+constexpr int kU = 8;
+
+// KIND 0: sum of re^2 + im^2 (v[0]); KIND 1: (sum re, sum im).  Either plane
+// may be absent.  Wave w of the grid takes pairs [w * per, (w + 1) * per).
+template <int KIND>
+__device__ __forceinline__ void stream_planes(const double* __restrict__ re,
+                                              const double* __restrict__ im, size_t n,
+                                              double (&v)[KIND == 0 ? 1 : 2]) {
+  const int lane = threadIdx.x & 63;
+  const size_t nw = (size_t)gridDim.x * (kThreads / 64);
+  const size_t w = (size_t)blockIdx.x * (kThreads / 64) + (threadIdx.x >> 6);
+  const bool aligned = ((((uintptr_t)re) | ((uintptr_t)im)) & 15) == 0;
+  const size_t npair = aligned ? n / 2 : 0;
+  const size_t group = 64 * kU;
+  const size_t per = ((npair + nw - 1) / nw + group - 1) / group * group;
+  const size_t beg = w * per;
+  const size_t end = beg + per < npair ? beg + per : npair;
+  const v2d* r2 = reinterpret_cast<const v2d*>(re);
+  const v2d* i2 = reinterpret_cast<const v2d*>(im);
+  double ar[kU], ai[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) ar[u] = ai[u] = 0.0;
+  size_t p = beg + lane;
+  for (; p + (kU - 1) * 64 < end; p += group) {
+    v2d x[kU], y[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      x[u] = re ? __builtin_nontemporal_load(r2 + p + u * 64) : v2d{0.0, 0.0};
+      y[u] = im ? __builtin_nontemporal_load(i2 + p + u * 64) : v2d{0.0, 0.0};
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (KIND == 0) {
+        ar[u] = fma(x[u].x, x[u].x, ar[u]);
+        ar[u] = fma(x[u].y, x[u].y, ar[u]);
+        ai[u] = fma(y[u].x, y[u].x, ai[u]);
+        ai[u] = fma(y[u].y, y[u].y, ai[u]);
+      } else {
+        ar[u] += x[u].x + x[u].y;
+        ai[u] += y[u].x + y[u].y;
+      }
+    }
+  }
+  for (; p < end; p += 64) {
+    const v2d x = re ? r2[p] : v2d{0.0, 0.0};
+    const v2d y = im ? i2[p] : v2d{0.0, 0.0};
+    if (KIND == 0) {
+      ar[0] = fma(x.x, x.x, fma(x.y, x.y, ar[0]));
+      ai[0] = fma(y.x, y.x, fma(y.y, y.y, ai[0]));
+    } else {
+      ar[0] += x.x + x.y;
+      ai[0] += y.x + y.y;
+    }
+  }
+  // the elements the pairs leave (odd n, or unaligned planes)
+  const size_t tid = (size_t)blockIdx.x * kThreads + threadIdx.x;
+  for (size_t i = 2 * npair + tid; i < n; i += (size_t)gridDim.x * kThreads) {
+    const double x = re ? re[i] : 0.0, y = im ? im[i] : 0.0;
+    if (KIND == 0) {
+      ar[0] = fma(x, x, ar[0]);
+      ai[0] = fma(y, y, ai[0]);
+    } else {
+      ar[0] += x;
+      ai[0] += y;
+    }
+  }
+  double sr = 0.0, si = 0.0;
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    sr += ar[u];
+    si += ai[u];
+  }
+  if (KIND == 0) {
+    v[0] = sr + si;
+  } else {
+    v[0] = sr;
+    v[KIND == 0 ? 0 : 1] = si;
+  }
+}
+
 __global__ __launch_bounds__(256) void k_sumsq(const double* __restrict__ re,
                                                const double* __restrict__ im, size_t n,
                                                double* partials, unsigned* counter,
                                                double* result) {
   __shared__ double lds[kThreads / 64][1];
-  double v[1] = {0.0};
-  const size_t tid = blockIdx.x * (size_t)kThreads + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * kThreads;
-  const bool aligned = ((((uintptr_t)re) | ((uintptr_t)im)) & 15) == 0;
-  size_t npair = aligned ? n / 2 : 0;
-  double a0 = 0, a1 = 0;
-  for (size_t p = tid; p < npair; p += stride) {
-    if (re) {
-      v2d x = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(re) + p);
-      a0 += x.x * x.x;
-      a1 += x.y * x.y;
-    }
-    if (im) {
-      v2d y = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(im) + p);
-      a0 += y.x * y.x;
-      a1 += y.y * y.y;
-    }
-  }
-  for (size_t i = 2 * npair + tid; i < n; i += stride) {
-    if (re) a0 += re[i] * re[i];
-    if (im) a0 += im[i] * im[i];
-  }
-  v[0] = a0 + a1;
+  double v[1];
+  stream_planes<0>(re, im, n, v);
   block_sum<1>(v, lds);
   grid_finish<1>(v, partials, counter, result);
 }
 
-// (sum re, sum im)
 __global__ __launch_bounds__(256) void k_sum2(const double* __restrict__ re,
                                               const double* __restrict__ im, size_t n,
                                               double* partials, unsigned* counter,
                                               double* result) {
   __shared__ double lds[kThreads / 64][2];
-  double v[2] = {0.0, 0.0};
-  const size_t tid = blockIdx.x * (size_t)kThreads + threadIdx.x;
-  const size_t stride = (size_t)gridDim.x * kThreads;
-  const bool aligned = ((((uintptr_t)re) | ((uintptr_t)im)) & 15) == 0;
-  size_t npair = aligned ? n / 2 : 0;
-  for (size_t p = tid; p < npair; p += stride) {
-    if (re) {
-      v2d x = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(re) + p);
-      v[0] += x.x + x.y;
-    }
-    if (im) {
-      v2d y = __builtin_nontemporal_load(reinterpret_cast<const v2d*>(im) + p);
-      v[1] += y.x + y.y;
-    }
-  }
-  for (size_t i = 2 * npair + tid; i < n; i += stride) {
-    if (re) v[0] += re[i];
-    if (im) v[1] += im[i];
-  }
+  double v[2];
+  stream_planes<1>(re, im, n, v);
   block_sum<2>(v, lds);
   grid_finish<2>(v, partials, counter, result);
 }
 
 }  // namespace
 
+// Blocks per grid: at least one 16-byte load group per lane, at most
+// g_reduce_blocks (default 256: one wave per SIMD of the 256 CUs, each
+// streaming a contiguous chunk with kU loads in flight per plane).
+int g_reduce_blocks = 256;
 int reduce_blocks_for(size_t n) {
-  // ~8 elements per thread minimum; at most 2048 blocks (8 per CU)
-  size_t b = (n + (size_t)kThreads * 16 - 1) / ((size_t)kThreads * 16);
+  size_t b = (n / 2 + (size_t)kThreads * kU - 1) / ((size_t)kThreads * kU);
   if (b < 1) b = 1;
-  if (b > kMaxBlocks) b = kMaxBlocks;
+  if (b > (size_t)g_reduce_blocks) b = (size_t)g_reduce_blocks;
   return (int)b;
 }
 
@@ -173,3 +219,9 @@ oaph_status launch_sum2(hipStream_t s, const double* re, const double* im, size_
 }
 
 }  // namespace oaph
+
+extern "C" oaph_status oaph_set_reduce_blocks(int blocks) {
+  OAPH_CHECK(blocks >= 1 && blocks <= oaph::kMaxBlocks, OAPH_ERR_ARG);
+  oaph::g_reduce_blocks = blocks;
+  return OAPH_OK;
+}

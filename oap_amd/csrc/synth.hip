@@ -133,4 +133,23 @@ oaph_status oaph_time_dot_product(oaph_api* api, oaph_matrix* out, const oaph_ma
   return OAPH_OK;
 }
 
+oaph_status oaph_time_reduction(oaph_api* api, int sumsq, const oaph_matrix* m, uint32_t iters,
+                                double* avg_ms) {
+  OAPH_CHECK(api && m && avg_ms && iters > 0, OAPH_ERR_ARG);
+  OAPH_TRY(enqueue_reduction(api, m, sumsq != 0));  // warm-up
+  hipEvent_t e0, e1;
+  OAPH_HIP(hipEventCreate(&e0));
+  OAPH_HIP(hipEventCreate(&e1));
+  OAPH_HIP(hipEventRecord(e0, api->stream));
+  for (uint32_t i = 0; i < iters; ++i) OAPH_TRY(enqueue_reduction(api, m, sumsq != 0));
+  OAPH_HIP(hipEventRecord(e1, api->stream));
+  OAPH_HIP(hipEventSynchronize(e1));
+  float ms = 0;
+  OAPH_HIP(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *avg_ms = ms / iters;
+  return OAPH_OK;
+}
+
 }  // extern "C"

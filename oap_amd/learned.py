@@ -74,6 +74,41 @@ def eig_decode(y):
     return np.exp(10.0 * np.asarray(y, dtype=np.float64))
 
 
+def closed_form_smallest(pe: float, c: float, alpha: float, k: int = K,
+                         grid: int = GRID) -> np.ndarray:
+    """The k smallest eigenvalues of a family member in closed form.  The
+    assembled operator (oaph_synth_cdr) has five coefficients, each rounded
+    to FP32: the diagonal d, the x neighbours (xm, xp) and the y neighbours
+    (ym, yp).  It is the Kronecker sum of two tridiagonal Toeplitz matrices,
+    so its eigenvalues are d + 2 sqrt(xm xp) cos(i pi h) + 2 sqrt(ym yp)
+    cos(j pi h), i, j = 1..grid (all real: the cell Peclet number pe h / 2
+    stays below alpha on this box)."""
+    h = 1.0 / (grid + 1)
+    ih2, adv = 1.0 / (h * h), pe / (2.0 * h)
+    f32 = lambda v: float(np.float32(v))
+    d = f32(2.0 * alpha * ih2 + 2.0 * ih2 + c)
+    xm, xp = f32(-alpha * ih2 - adv), f32(-alpha * ih2 + adv)
+    ym, yp = f32(-ih2 - adv), f32(-ih2 + adv)
+    cs = np.cos(np.arange(1, grid + 1) * np.pi * h)
+    ex, ey = 2.0 * np.sqrt(xm * xp) * cs, 2.0 * np.sqrt(ym * yp) * cs
+    return np.sort((ex[:, None] + ey[None, :]).ravel() + d)[:k]
+
+
+def condition_estimate(pe: float, alpha: float) -> float:
+    """Growth of the eigenvector condition number with convection: the
+    diagonal similarity D that symmetrises the operator has entries ratio
+    ((1 + r_x) / (1 - r_x))^(GRID / 2) ((1 + r_y) / (1 - r_y))^(GRID / 2),
+    r_x = pe h / (2 alpha), r_y = pe h / 2 (cond(D) up to the grid-size
+    factor).  Above ~1e6 an FP64 eigensolver returns values from the
+    pseudospectrum whose residuals still meet 1e-10 ||A||_F."""
+    h = 1.0 / (GRID + 1)
+    rx, ry = pe * h / (2 * alpha), pe * h / 2
+    return float(((1 + rx) / (1 - rx)) ** (GRID / 2) * ((1 + ry) / (1 - ry)) ** (GRID / 2))
+
+
+WELL_CONDITIONED = 1e6   # condition_estimate bound of the members FP64 resolves
+
+
 def log_profile(pe: float, alpha: float) -> np.ndarray:
     """log D on the grid (index ix + GRID iy), shifted so that its maximum is 0."""
     h = 1.0 / (GRID + 1)
@@ -132,6 +167,7 @@ class Solution:
     fallback: bool           # the random start was needed
     predicted: np.ndarray | None = None  # network's eigenvalue estimates
     vectors: np.ndarray | None = None
+    fro: float = 0.0         # ||A||_F, the residual scale
 
 
 class CdrProblem:
@@ -183,7 +219,8 @@ class CdrProblem:
             mv += st["matvecs"]
             rs += st["restarts"]
             fb = True
-        return Solution(ev.copy(), resid, ok, mv, rs, time.perf_counter() - t0, fb, None, vecs)
+        return Solution(ev.copy(), resid, ok, mv, rs, time.perf_counter() - t0, fb, None, vecs,
+                        self.fro)
 
 
 class LearnedStart:

@@ -142,16 +142,32 @@ def c0_eigs():
 
 
 def c1_top8():
+    """The 16 largest-|lambda| eigenvalues of the dense complex 8192 x 8192
+    C1 matrix (the solver wants 8; the next 8 name a wrong wanted set).  Two
+    ARPACK runs from different start vectors must agree, and every pair's
+    explicit residual is recorded."""
     import scipy.sparse.linalg as sla
     from oracle.pyoracle import Oracle
-    n = 8192
+    n, kk = 8192, 16
     ar, ai = Oracle().gen_c1(n, 0x5EED0001, 1e-2, 500.0)
     A = ar + 1j * ai
     del ar, ai
-    ev, _ = sla.eigs(A, k=8, which="LM", ncv=48, tol=1e-13, maxiter=20000)
-    ev = ev[np.argsort(-np.abs(ev), kind="stable")]
-    return {"n": n, "seed": 0x5EED0001, "re": ev.real.tolist(), "im": ev.imag.tolist(),
-            "fro": float(np.linalg.norm(A)), "method": "ARPACK zneupd tol 1e-13"}
+    runs = []
+    for seed in (1, 2):
+        v0 = np.random.default_rng(seed).standard_normal(n) + 0j
+        ev, vec = sla.eigs(A, k=kk, which="LM", ncv=64, tol=1e-14, maxiter=50000, v0=v0)
+        order = np.argsort(-np.abs(ev), kind="stable")
+        runs.append((ev[order], vec[:, order]))
+    (ev, vec), (ev2, _) = runs
+    agree = max(float(np.min(np.abs(ev2 - z))) for z in ev)
+    resid = [float(np.linalg.norm(A @ vec[:, t] - ev[t] * vec[:, t]) / np.linalg.norm(vec[:, t]))
+             for t in range(kk)]
+    fro = float(np.linalg.norm(A))
+    assert agree <= 1e-13 and max(resid) <= 1e-13 * fro, (agree, max(resid))
+    return {"n": n, "seed": 0x5EED0001, "k": 8, "re": ev.real.tolist(), "im": ev.imag.tolist(),
+            "fro": fro, "residuals": resid, "two_start_agreement": agree,
+            "method": "ARPACK zneupd (scipy.sparse.linalg.eigs), k=16, ncv=64, tol 1e-14, "
+                      "two start vectors"}
 
 
 def main():

@@ -1,13 +1,23 @@
 """The C4 operator generator of the oracle (orc_gen_cdr) against the closed
 form of its spectrum: the operator is the Kronecker sum of two tridiagonal
 Toeplitz matrices, so its eigenvalues are
-c + a_x + 2 sqrt(b_x d_x) cos(j pi h) + a_y + 2 sqrt(b_y d_y) cos(k pi h)
-(FP32 rounding of the coefficients aside)."""
+d + 2 sqrt(xm xp) cos(i pi h) + 2 sqrt(ym yp) cos(j pi h) with the
+generator's FP32-rounded coefficients (learned.closed_form_smallest)."""
 import numpy as np
 import pytest
 
+from oap_amd.learned import closed_form_smallest
 
-def closed_form(N, pe, c, alpha):
+
+@pytest.mark.parametrize("pe,c,alpha", [(0.0, 0.0, 1.0), (0.0, 3.0, 0.5), (6.0, 2.0, 1.5),
+                                        (10.0, 7.0, 2.0), (3.0, 5.0, 0.6)])
+def test_cdr_spectrum_closed_form(oracle, pe, c, alpha):
+    N = 12
+    A = oracle.gen_cdr(N, pe, c, alpha)
+    ev = np.sort(np.linalg.eigvals(A).real)
+    want = closed_form_smallest(pe, c, alpha, N * N, N)
+    assert np.max(np.abs(ev - want)) <= 1e-12 * np.max(np.abs(want))
+    # exact coefficients would miss by the FP32 rounding of the diagonal
     h = 1.0 / (N + 1)
     j = np.arange(1, N + 1)
 
@@ -15,17 +25,8 @@ def closed_form(N, pe, c, alpha):
         lo, up = -a / h**2 - pe / (2 * h), -a / h**2 + pe / (2 * h)
         return 2 * a / h**2 + 2 * np.sqrt(lo * up) * np.cos(j * np.pi * h)
 
-    return np.sort((ev1(alpha)[:, None] + ev1(1.0)[None, :]).ravel() + c)
-
-
-@pytest.mark.parametrize("pe,c,alpha", [(0.0, 0.0, 1.0), (0.0, 3.0, 0.5), (6.0, 2.0, 1.5),
-                                        (10.0, 7.0, 2.0)])
-def test_cdr_spectrum_closed_form(oracle, pe, c, alpha):
-    N = 12
-    A = oracle.gen_cdr(N, pe, c, alpha)
-    ev = np.sort(np.linalg.eigvals(A).real)
-    want = closed_form(N, pe, c, alpha)
-    assert np.max(np.abs(ev - want)) <= 1e-5 * np.max(np.abs(want))
+    exact = np.sort((ev1(alpha)[:, None] + ev1(1.0)[None, :]).ravel() + c)
+    assert np.max(np.abs(exact - want)) <= 1e-5 * np.max(np.abs(want))
 
 
 def test_cdr_structure(oracle):

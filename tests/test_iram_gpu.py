@@ -129,6 +129,34 @@ def test_mixed_with_fp32_valued_operator_runs_all_cycles_in_fp32(oracle, rng):
     match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
 
 
+@pytest.mark.parametrize("n", [1022, 1023])
+def test_mixed_without_usable_fp32_copy_runs_fp64(oracle, n):
+    # the FP32-operand matvec reads rows as float4: a row length that is not
+    # a multiple of 4 has no usable FP32 copy, and the solve runs FP64 cycles
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    s, ev, vecs = run_dense(A, 8, 40, precision=Precision.MIXED_FP32)
+    st = s.stats()
+    assert st["mixed_declined"] == 1 and st["matvecs_low"] == 0
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+
+
+def test_mixed_beyond_fp32_range_runs_fp64(oracle):
+    # diagonal entries of 1e39 would be infinite in the FP32 copy
+    n = 512
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = (ar + 1j * ai) * 1e39
+    s, ev, vecs = run_dense(A, 6, 32, precision=Precision.MIXED_FP32)
+    st = s.stats()
+    assert st["mixed_declined"] == 1 and st["matvecs_low"] == 0
+    match(ev, top_k(np.linalg.eigvals(A), 6, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 6)
+    # the same operator scaled into range takes the mixed flow
+    s, ev, _ = run_dense(A * 1e-39, 6, 32, precision=Precision.MIXED_FP32, vectors=False)
+    assert s.stats()["mixed_declined"] == 0 and s.stats()["matvecs_low"] > 0
+
+
 def test_restart_stages(oracle):
     # restarts from the sum of the wanted Ritz vectors at 1e-2 and 1e-4: the
     # same eigenpairs to the same tolerance
@@ -159,7 +187,8 @@ def wanted_set_misses(A, dA, precision, guard, seeds, m=40):
     s = solver()
     s.setMatrix(dA)
     s.setPrecision(precision)
-    s.setRestartGuard(guard)
+    if guard is not None:
+        s.setRestartGuard(guard)
     re, im = np.zeros(8), np.zeros(8)
     s.setOutputsEigenvalues(re, im)
     misses = []
@@ -178,17 +207,17 @@ def test_wanted_set_over_start_vectors(oracle):
     # (a start vector weak in the 8th eigenvector converges on the 9th first).
     # Stopped at 1e-6 the plain solver returns a wrong set for 26 of 600
     # start vectors (10000..10599), at 1e-10 for none.  The mixed flow stops
-    # its FP32 cycles at 1e-6 and restarts FP64 from the Ritz-vector sum: 2
-    # of those 600 stay wrong; 4 guard Ritz vectors in the sum remove them
-    # (at 29 % more matvecs).  Seeds 10176..10223 hold one of the two.
+    # its FP32 cycles at 1e-6 and restarts FP64 from the sum of the wanted
+    # Ritz vectors and the next four (the default guard).  Seeds 10176..10223
+    # include start vector 10181, for which a sum of the wanted Ritz vectors
+    # alone gave the 9th eigenvalue in place of the 8th.
     n = 1024
     ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
     A = ar + 1j * ai
     dA = DeviceMatrix.from_numpy(A.real, A.imag)
     seeds = range(10176, 10224)
-    assert wanted_set_misses(A, dA, Precision.FP64, 0, seeds) == []
-    assert wanted_set_misses(A, dA, Precision.MIXED_FP32, 0, seeds) == [10181]
-    assert wanted_set_misses(A, dA, Precision.MIXED_FP32, 4, seeds) == []
+    assert wanted_set_misses(A, dA, Precision.FP64, None, seeds) == []
+    assert wanted_set_misses(A, dA, Precision.MIXED_FP32, None, seeds) == []
 
 
 def test_mixed_fallback_to_fp32(oracle):
@@ -317,7 +346,19 @@ def test_lucky_breakdown_invariant_subspace(precision):
     vecs = [DeviceMatrix(1, n) for _ in range(k)]
     s.setOutputsEigenvectors(vecs)
     s.execute(m, k, MatrixInfo(True, False, n, n))
-    np.testing.assert_allclose(sorted(re), sorted(vals)[-k:], rtol=1e-12)
+    krylov = sorted(vals)[-k:]  # one value per eigenspace the start vector spans
+    if precision == Precision.FP64:
+        np.testing.assert_allclose(sorted(re), krylov, rtol=1e-12)
+    else:
+        # the FP64 polish starts from the sum of the FP32 phase's 7 leading
+        # Ritz vectors: its factorisation is invariant at step 7 only up to
+        # ~1e-15 components elsewhere, above the breakdown threshold, so it may
+        # go on and meet further vectors of the 7-fold eigenvalue 10/3 -- the
+        # three largest eigenvalues counted with multiplicity, as a dense
+        # solver returns them
+        got = sorted(re)
+        assert (np.allclose(got, krylov, rtol=1e-12, atol=0)
+                or np.allclose(got, [vals[-1]] * k, rtol=1e-12, atol=0)), got
     assert np.all(np.abs(im) <= 1e-12)
     check_residuals(s, A, re + 1j * im, vecs, k)
 

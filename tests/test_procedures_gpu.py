@@ -182,6 +182,32 @@ def test_magnitude_and_sum(api, oracle, rng, n):
     assert abs(s[1] - a.imag.sum()) <= bound + n * EPS * np.abs(a.imag).sum() * 0.01
 
 
+def test_magnitude_and_sum_2p26_exact(api):
+    # the C1 standalone size, 2^26 complex elements: integer values whose
+    # every partial sum is an integer below 2^53, so any summation order
+    # gives the exact result
+    n = 1 << 26
+    i = np.arange(n, dtype=np.int64)
+    re = (i % 97).astype(np.float64)
+    im = (i % 5).astype(np.float64)
+    d = DeviceMatrix.from_numpy(re.reshape(1 << 13, 1 << 13), im.reshape(1 << 13, 1 << 13))
+    assert api.sum(d) == (re.sum(), im.sum())
+    sq = float(np.dot(re, re) + np.dot(im, im))
+    assert api.magnitude(d) == np.sqrt(sq)
+
+
+def test_magnitude_and_sum_2p26_random(api, rng):
+    # 2^26 + 3 complex elements: the odd tail after the 16-byte pairs
+    n = (1 << 26) + 3
+    a = cplx(rng, n, 1)
+    d = dev(a)
+    want = np.sqrt(np.sum(np.abs(a) ** 2))
+    assert abs(api.magnitude(d) - want) <= 4 * np.sqrt(n) * EPS * want
+    s = api.sum(d)
+    bound = 2 * np.log2(n + 1) * EPS * np.abs(a).sum()
+    assert abs(s[0] - a.real.sum()) <= bound and abs(s[1] - a.imag.sum()) <= bound
+
+
 def test_magnitude_closed_forms(api):
     # oapMatrixCudaTest.cpp:418-443 (1..10 -> sqrt(385)), :473-502 (one 3 among 16001)
     v = np.arange(1, 11, dtype=float)
