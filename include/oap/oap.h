@@ -363,6 +363,9 @@ class CuHArnoldi {
   void setRestartGuard(uintt g) {
     oap::check(oaph_arnoldi_set_restart_guard(m_a, (uint32_t)g), "setRestartGuard");
   }
+  void setRestartGuardWeight(double w) {
+    oap::check(oaph_arnoldi_set_restart_guard_weight(m_a, w), "setRestartGuardWeight");
+  }
   oaph_arnoldi_stats stats() const {
     oaph_arnoldi_stats s;
     oap::check(oaph_arnoldi_get_stats(m_a, &s), "stats");

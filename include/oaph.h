@@ -332,6 +332,9 @@ oaph_status oaph_arnoldi_set_restart_stages(oaph_arnoldi* a, const double* tols,
  * below the k-th is kept in the new start vector (default 4; at most m - k
  * are used).  No reference counterpart. */
 oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* a, uint32_t guard);
+/* Weight of the guard Ritz vectors in a Ritz-sum restart (0..1, default
+ * 1e-3; the wanted ones have weight 1).  No reference counterpart. */
+oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* a, double weight);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
  * the largest Ritz value (default 1e-6) */
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* a, double tol_low);

@@ -126,6 +126,7 @@ _SIGS = {
     "oaph_arnoldi_set_max_restarts": (C.c_int, [C.c_void_p, U32]),
     "oaph_arnoldi_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_mixed_tolerance": (C.c_int, [C.c_void_p, C.c_double]),
+    "oaph_arnoldi_set_restart_guard_weight": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_start_vector": (C.c_int, [C.c_void_p, DP, DP, U32]),
     "oaph_arnoldi_get_stats": (C.c_int, [C.c_void_p, C.POINTER(oaph_arnoldi_stats)]),
     "oaph_arnoldi_time_matvec": (C.c_int, [C.c_void_p, C.c_int, U32, DP]),

@@ -448,6 +448,11 @@ class CuHArnoldiCallback:
         sum (the mixed flow's FP64 start and the stages; default 4)"""
         check(self._lib.oaph_arnoldi_set_restart_guard(self._h, guard), "setRestartGuard")
 
+    def setRestartGuardWeight(self, weight):
+        """weight of the guard Ritz vectors in a Ritz-sum restart (wanted: 1)"""
+        check(self._lib.oaph_arnoldi_set_restart_guard_weight(self._h, weight),
+              "setRestartGuardWeight")
+
     def setMixedTolerance(self, tol_low):
         check(self._lib.oaph_arnoldi_set_mixed_tolerance(self._h, tol_low), "set_mixed_tolerance")
 

@@ -89,8 +89,9 @@ size_t pairs_partials_bytes(uint32_t n, uint32_t k);
 // dst = (float)src; flags (may be NULL): flags[0] set to 1 when an element
 // rounds, flags[1] when a finite element overflows the FP32 range
 oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsigned* flags);
-oaph_status colsum(hipStream_t s, uint32_t n, uint32_t k, const double* Xre, const double* Xim,
-                   double* vre, double* vim);
+// V[:, 0] = sum of the first k columns of X plus w times the next count - k
+oaph_status colsum(hipStream_t s, uint32_t n, uint32_t k, uint32_t count, double w,
+                   const double* Xre, const double* Xim, double* vre, double* vim);
 oaph_status random_vector(hipStream_t s, uint32_t n, uint64_t seed, bool cplx, double* re,
                           double* im);
 

@@ -469,17 +469,25 @@ __global__ void k_to_f32(size_t n, const double* __restrict__ src, float* __rest
   if (over && flags) flags[1] = 1u;
 }
 
-// x = sum_t X[:, t] (start vector of the polishing phase), written to V[:, 0]
-__global__ void k_colsum(uint32_t n, uint32_t k, const double* Xre, const double* Xim,
-                         double* vre, double* vim) {
+// x = sum_{t < k} X[:, t] + w sum_{k <= t < count} X[:, t] (start vector of
+// a Ritz-sum restart: the wanted Ritz vectors and the weighted guards),
+// written to V[:, 0]
+__global__ void k_colsum(uint32_t n, uint32_t k, uint32_t count, double w, const double* Xre,
+                         const double* Xim, double* vre, double* vim) {
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    double sr = 0, si = 0;
-    for (uint32_t t = 0; t < k; ++t) {
-      sr += Xre[(size_t)t * n + i];
-      si += Xim[(size_t)t * n + i];
+    double sr = 0, si = 0, gr = 0, gi = 0;
+    for (uint32_t t = 0; t < count; ++t) {
+      const double xr = Xre[(size_t)t * n + i], xi = Xim[(size_t)t * n + i];
+      if (t < k) {
+        sr += xr;
+        si += xi;
+      } else {
+        gr += xr;
+        gi += xi;
+      }
     }
-    vre[i] = sr;
-    vim[i] = si;
+    vre[i] = sr + w * gr;
+    vim[i] = si + w * gi;
   }
 }
 
@@ -685,11 +693,11 @@ oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsig
   return OAPH_OK;
 }
 
-oaph_status colsum(hipStream_t s, uint32_t n, uint32_t k, const double* Xre, const double* Xim,
-                   double* vre, double* vim) {
+oaph_status colsum(hipStream_t s, uint32_t n, uint32_t k, uint32_t count, double w,
+                   const double* Xre, const double* Xim, double* vre, double* vim) {
   unsigned blocks = (n + 255) / 256;
   if (blocks > 1024) blocks = 1024;
-  hipLaunchKernelGGL(k_colsum, dim3(blocks), dim3(256), 0, s, n, k, Xre, Xim, vre, vim);
+  hipLaunchKernelGGL(k_colsum, dim3(blocks), dim3(256), 0, s, n, k, count, w, Xre, Xim, vre, vim);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

@@ -49,11 +49,15 @@ def test_fp64_shapes_bit_identical(n, cplx):
 
 
 def test_fp32_shapes_bit_identical():
+    # the C1 construction at n = 2048 with the bench's k = 8, m = 48: plain
+    # FP64 and the mixed flow return the dense top 8 for 100 of 100 start
+    # vectors there (with k = 4, m = 32 on this spectrum IRAM itself returns a
+    # wrong set for ~1 start vector in 5, plain FP64 included)
     lib = load()
     from oap_amd.api import CuHArnoldiCallback, Precision
-    n = 2048
+    n, k, m = 2048, 8, 48
     A = DeviceMatrix(n, n)
-    check(lib.oaph_synth_matrix(A.ptr, 1, 7, 1e-2, 500.0), "synth")
+    check(lib.oaph_synth_matrix(A.ptr, 1, 0x5EED0001, 1e-2, 500.0), "synth")
     s = CuHArnoldiCallback()
     s.setMatrix(A)
     s.setPrecision(Precision.MIXED_FP32)
@@ -63,16 +67,16 @@ def test_fp32_shapes_bit_identical():
         for v, sh in enumerate(shapes(1)):
             check(lib.oaph_set_matvec_shape(1, v), "shape")
             s.setSeed(3)
-            re, im = np.zeros(4), np.zeros(4)
+            re, im = np.zeros(k), np.zeros(k)
             s.setOutputsEigenvalues(re, im)
             from oap_amd.api import MatrixInfo
-            s.execute(32, 4, MatrixInfo(True, True, n, n))
+            s.execute(m, k, MatrixInfo(True, True, n, n))
             st = s.stats()
             ev.append((re + 1j * im, st["matvecs"], st["matvecs_low"]))
     finally:
         check(lib.oaph_set_matvec_shape(1, 0), "shape")
     Ad = A.to_complex()
-    want = sorted(np.linalg.eigvals(Ad), key=lambda z: -abs(z))[:4]
+    want = sorted(np.linalg.eigvals(Ad), key=lambda z: -abs(z))[:k]
     for e, _, _ in ev:
         for w in want:
             assert min(abs(g - w) for g in e) <= 1e-10 * abs(want[0])
