@@ -96,6 +96,7 @@ _SIGS = {
     "oaph_set_vector": (C.c_int, [C.c_void_p, MP, U32, MP, U32]),
     "oaph_get_vector": (C.c_int, [C.c_void_p, MP, U32, MP, U32]),
     "oaph_magnitude": (C.c_int, [C.c_void_p, DP, MP]),
+    "oaph_magnitude_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, MP]),
     "oaph_sum": (C.c_int, [C.c_void_p, DP, DP, MP]),
     "oaph_qrgr": (C.c_int, [C.c_void_p, MP, MP, MP, MP, MP, MP, MP]),
     "oaph_is_upper_triangular": (C.c_int, [C.c_void_p, MP, C.POINTER(C.c_int)]),

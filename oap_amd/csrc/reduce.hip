@@ -9,6 +9,8 @@
 // device memory -- one launch, no host round trip, deterministic for a given
 // grid.  The counter is returned to zero by that last block, so no memset is
 // needed between launches (safe under graph capture).
+#include <mutex>
+
 #include "internal.h"
 
 namespace oaph {
@@ -48,7 +50,7 @@ __device__ __forceinline__ void block_sum(double (&v)[NV], double (*lds)[NV]) {
 // Last-block finish: block partials [gridDim.x][NV] summed in block order.
 template <int NV>
 __device__ __forceinline__ void grid_finish(double (&v)[NV], double* partials, unsigned* counter,
-                                            double* result) {
+                                            double* result, bool root = false) {
   __shared__ bool is_last;
   if (threadIdx.x == 0) {
 #pragma unroll
@@ -74,7 +76,7 @@ __device__ __forceinline__ void grid_finish(double (&v)[NV], double* partials, u
     for (int q = 0; q < NV; ++q) acc[q] = wave_sum(acc[q]);
     if (threadIdx.x == 0) {
 #pragma unroll
-      for (int q = 0; q < NV; ++q) result[q] = acc[q];
+      for (int q = 0; q < NV; ++q) result[q] = (root && NV == 1) ? sqrt(acc[q]) : acc[q];
       *counter = 0u;
     }
   }
@@ -165,16 +167,19 @@ __device__ __forceinline__ void stream_planes(const double* __restrict__ re,
   }
 }
 
+// root: the result is the square root of the sum (the magnitude)
 __global__ __launch_bounds__(256) void k_sumsq(const double* __restrict__ re,
                                                const double* __restrict__ im, size_t n,
                                                double* partials, unsigned* counter,
-                                               double* result) {
+                                               double* result, int root) {
   __shared__ double lds[kThreads / 64][1];
   double v[1];
   stream_planes<0>(re, im, n, v);
   block_sum<1>(v, lds);
-  grid_finish<1>(v, partials, counter, result);
+  grid_finish<1>(v, partials, counter, result, root != 0);
 }
+
+__global__ void k_clear_counter(unsigned* counter) { *counter = 0u; }
 
 __global__ __launch_bounds__(256) void k_sum2(const double* __restrict__ re,
                                               const double* __restrict__ im, size_t n,
@@ -204,7 +209,7 @@ oaph_status launch_sumsq(hipStream_t s, const double* re, const double* im, size
                          double* partials, unsigned* counter, double* result) {
   int blocks = reduce_blocks_for(n);
   hipLaunchKernelGGL(k_sumsq, dim3(blocks), dim3(kThreads), 0, s, re, im, n, partials, counter,
-                     result);
+                     result, 0);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
@@ -223,5 +228,59 @@ oaph_status launch_sum2(hipStream_t s, const double* re, const double* im, size_
 extern "C" oaph_status oaph_set_reduce_blocks(int blocks) {
   OAPH_CHECK(blocks >= 1 && blocks <= oaph::kMaxBlocks, OAPH_ERR_ARG);
   oaph::g_reduce_blocks = blocks;
+  return OAPH_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free magnitude.  The block partials and the last-block counter
+// live in `scratch`, a file-scope device array (every GPU the library runs on
+// holds its own copy until the process ends): kMaxBlocks partial sums, then
+// the counter.  Host code never passes `scratch` itself (its host-side name is
+// not a device address); the device address comes from hipGetSymbolAddress,
+// once per device with that device current, and is kept per device.
+namespace oaph {
+namespace {
+constexpr int kScratchDoubles = kMaxBlocks + 32;  // partials, then the counter word
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+double* g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+oaph_status scratch_for_current_device(double** out) {
+  int dev = 0;
+  OAPH_HIP(hipGetDevice(&dev));
+  OAPH_CHECK(dev >= 0 && dev < kMaxDevices, OAPH_ERR_UNSUPPORTED);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    OAPH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return OAPH_OK;
+}
+}  // namespace
+}  // namespace oaph
+
+extern "C" oaph_status oaph_magnitude_noalloc(oaph_api* api, double* out, const oaph_matrix* m) {
+  using namespace oaph;
+  OAPH_CHECK(api && out && m, OAPH_ERR_ARG);
+  OAPH_TRY(ensure_init());
+  const size_t n = numel(m);
+  const int blocks = reduce_blocks_for(n);
+  if (blocks > kMaxBlocks) {
+    set_last_message("oaph_magnitude_noalloc: more block partials than the static scratch holds");
+    return OAPH_ERR_UNSUPPORTED;
+  }
+  double* base = nullptr;
+  OAPH_TRY(scratch_for_current_device(&base));
+  double* partials = base;
+  unsigned* counter = reinterpret_cast<unsigned*>(base + kMaxBlocks);
+  // the counter must start at zero (a fill kernel, not a memset: the call may
+  // be recorded into a graph)
+  hipLaunchKernelGGL(k_clear_counter, dim3(1), dim3(1), 0, api->stream, counter);
+  hipLaunchKernelGGL(k_sumsq, dim3(blocks), dim3(kThreads), 0, api->stream, m->reValues,
+                     m->imValues, n, partials, counter, out, 1);
+  OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
