@@ -204,11 +204,10 @@ oaph_status oaph_magnitude(oaph_api* api, double* out, const oaph_matrix* m);
  * capture, tight serving loops): calls no hipMalloc/hipFree.  `out` is one
  * double in DEVICE memory that the caller owns; sqrt(sum re^2 + im^2) is
  * written there in the order of the api's stream (read it after
- * oaph_api_synchronize).  The block partial sums and the finish counter live
- * in a static device array of the library (2048 partial sums and a counter,
- * about 16 KiB per GPU, held until the process ends); any matrix size is
- * accepted (at most 2048 workgroups), an api whose reduction grid would
- * exceed it gets OAPH_ERR_UNSUPPORTED.  Calls that overlap on one GPU share
+ * oaph_api_synchronize).  The block partial sums live in a static device
+ * array of the library (2048 doubles, 16 KiB per GPU, held until the process
+ * ends); any matrix size is accepted (the reduction grid has at most 2048
+ * workgroups; a larger one would get OAPH_ERR_UNSUPPORTED).  Calls that overlap on one GPU share
  * that array and race: serialise them (one stream, or events). */
 oaph_status oaph_magnitude_noalloc(oaph_api* api, double* out, const oaph_matrix* m);
 /* sum(re&, im&, m) (CuProceduresApi.cpp:358-388) */

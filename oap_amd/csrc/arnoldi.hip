@@ -647,7 +647,7 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     OAPH_TRY(s->partials.reserve(4096 * sizeof(double)));
     double* res = s->coef.as<double>();
     OAPH_TRY(launch_sumsq(st, s->A->reValues, s->A->imValues, (size_t)s->n * s->n,
-                          s->partials.as<double>(), s->counter.as<unsigned>() + 32, res));
+                          s->partials.as<double>(), res));
     double h = 0;
     OAPH_HIP(hipMemcpyAsync(&h, res, sizeof(double), hipMemcpyDeviceToHost, st));
     OAPH_HIP(hipStreamSynchronize(st));

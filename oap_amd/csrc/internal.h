@@ -82,9 +82,9 @@ oaph_status launch_get_vector(hipStream_t s, oaph_matrix* v, uint32_t length, co
                               uint32_t column);
 // reduce.hip
 oaph_status launch_sumsq(hipStream_t s, const double* re, const double* im, size_t n,
-                         double* partials, unsigned* counter, double* result);
+                         double* partials, double* result);
 oaph_status launch_sum2(hipStream_t s, const double* re, const double* im, size_t n,
-                        double* partials, unsigned* counter, double* result);
+                        double* partials, double* result);
 int reduce_blocks_for(size_t n);
 // magnitude (sumsq) or sum of m into api->scalars, enqueued on api->stream
 oaph_status enqueue_reduction(oaph_api* api, const oaph_matrix* m, bool sumsq);

@@ -32,12 +32,10 @@ oaph_status enqueue_reduction(oaph_api* api, const oaph_matrix* m, bool sumsq) {
   int blocks = reduce_blocks_for(n);
   OAPH_TRY(api->partials.reserve((size_t)blocks * 2 * sizeof(double)));
   double* result = api->scalars.as<double>();
-  unsigned* counter = api->counters.as<unsigned>();
   if (sumsq)
     return launch_sumsq(api->stream, m->reValues, m->imValues, n, api->partials.as<double>(),
-                        counter, result);
-  return launch_sum2(api->stream, m->reValues, m->imValues, n, api->partials.as<double>(), counter,
-                     result);
+                        result);
+  return launch_sum2(api->stream, m->reValues, m->imValues, n, api->partials.as<double>(), result);
 }
 }  // namespace oaph
 
