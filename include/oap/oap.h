@@ -55,6 +55,28 @@ struct MatrixInfo {
 };
 }  // namespace math
 
+// MatrixEx (oapMatrix/MatrixEx.h:26-55): a region of a matrix.  Here a host
+// struct passed by pointer (the reference allocates it on the device).
+struct MatrixEx {
+  uintt column;
+  uintt columns;
+  uintt row;
+  uintt rows;
+};
+static_assert(sizeof(MatrixEx) == sizeof(oaph_matrix_ex), "MatrixEx must match oaph_matrix_ex");
+#define mex_erow(matrixex) matrixex.row + matrixex.rows
+#define mex_ecolumn(matrixex) matrixex.column + matrixex.columns
+inline bool adjustColumns(MatrixEx& ex, uintt val) {
+  if (ex.column > val) return false;
+  if (ex.column + ex.columns > val) ex.columns = val - ex.column;
+  return true;
+}
+inline bool adjustRows(MatrixEx& ex, uintt val) {
+  if (ex.row > val) return false;
+  if (ex.row + ex.rows > val) ex.rows = val - ex.row;
+  return true;
+}
+
 namespace oap {
 
 class Error : public std::runtime_error {
@@ -102,6 +124,79 @@ inline void CopyDeviceMatrixToHostArrays(floatt* re, floatt* im, const math::Mat
 inline void CopyDeviceMatrixToDeviceMatrix(math::Matrix* dst, const math::Matrix* src) {
   check(oaph_matrix_copy_d2d(c(dst), c(src)), "CopyDeviceMatrixToDeviceMatrix");
 }
+// dims[0] = {{column, columns}, {row, rows}} of dst, dims[1] of src
+// (oapCudaMatrixUtils.h:102-116); a host matrix is a math::Matrix whose
+// planes are host pointers
+inline void CopyDeviceMatrixToHostMatrixDims(math::Matrix* dst, const math::Matrix* src,
+                                             uintt dims[2][2][2]) {
+  check(oaph_matrix_copy_d2h_dims(c(dst), c(src), dims), "CopyDeviceMatrixToHostMatrixDims");
+}
+inline void CopyHostMatrixToDeviceMatrixDims(math::Matrix* dst, const math::Matrix* src,
+                                             uintt dims[2][2][2]) {
+  check(oaph_matrix_copy_h2d_dims(c(dst), c(src), dims), "CopyHostMatrixToDeviceMatrixDims");
+}
+inline void CopyDeviceMatrixToDeviceMatrixDims(math::Matrix* dst, const math::Matrix* src,
+                                               uintt dims[2][2][2]) {
+  check(oaph_matrix_copy_d2d_dims(c(dst), c(src), dims), "CopyDeviceMatrixToDeviceMatrixDims");
+}
+// SetMatrix / SetReMatrix / SetImMatrix (oapCudaMatrixUtils.h:142-146)
+inline void SetMatrix(math::Matrix* m, math::Matrix* m1, uintt column, uintt row) {
+  check(oaph_matrix_set_submatrix(c(m), c(m1), column, row, 3), "SetMatrix");
+}
+inline void SetReMatrix(math::Matrix* m, math::Matrix* m1, uintt column, uintt row) {
+  check(oaph_matrix_set_submatrix(c(m), c(m1), column, row, 1), "SetReMatrix");
+}
+inline void SetImMatrix(math::Matrix* m, math::Matrix* m1, uintt column, uintt row) {
+  check(oaph_matrix_set_submatrix(c(m), c(m1), column, row, 2), "SetImMatrix");
+}
+// values by index or (column, row) (oapCudaMatrixUtils.h:182-200)
+inline void GetValue(floatt& re, floatt& im, const math::Matrix* m, uintt index) {
+  check(oaph_matrix_get_value(c(m), index, &re, &im), "GetValue");
+}
+inline floatt GetReValue(const math::Matrix* m, uintt index) {
+  floatt re = 0, im = 0;
+  GetValue(re, im, m, index);
+  return re;
+}
+inline floatt GetReValue(const math::Matrix* m, uintt column, uintt row) {
+  return GetReValue(m, column + row * m->columns);
+}
+inline floatt GetImValue(const math::Matrix* m, uintt index) {
+  floatt re = 0, im = 0;
+  GetValue(re, im, m, index);
+  return im;
+}
+inline floatt GetImValue(const math::Matrix* m, uintt column, uintt row) {
+  return GetImValue(m, column + row * m->columns);
+}
+inline void SetValue(math::Matrix* m, floatt re, floatt im, uintt column, uintt row) {
+  check(oaph_matrix_set_value(c(m), column + row * m->columns, re, im), "SetValue");
+}
+inline void SetReValue(math::Matrix* m, floatt value, uintt index) {
+  floatt re = 0, im = 0;
+  GetValue(re, im, m, index);
+  check(oaph_matrix_set_value(c(m), index, value, im), "SetReValue");
+}
+inline void SetReValue(math::Matrix* m, floatt value, uintt column, uintt row) {
+  SetReValue(m, value, column + row * m->columns);
+}
+inline void SetImValue(math::Matrix* m, floatt value, uintt index) {
+  floatt re = 0, im = 0;
+  GetValue(re, im, m, index);
+  check(oaph_matrix_set_value(c(m), index, re, value), "SetImValue");
+}
+inline void SetImValue(math::Matrix* m, floatt value, uintt column, uintt row) {
+  SetImValue(m, value, column + row * m->columns);
+}
+inline void SetZeroRow(math::Matrix* m, uintt index, bool re = true, bool im = true) {
+  check(oaph_matrix_set_zero_row(c(m), index, re ? 1 : 0, im ? 1 : 0), "SetZeroRow");
+}
+// MatrixEx objects (oapCudaMatrixUtils.h:148-167): host structs here
+inline MatrixEx* NewDeviceMatrixEx() { return new MatrixEx{0, 0, 0, 0}; }
+inline MatrixEx* NewDeviceMatrixExCopy(const MatrixEx& ex) { return new MatrixEx(ex); }
+inline void SetMatrixEx(MatrixEx* dst, const MatrixEx* src) { *dst = *src; }
+inline void GetMatrixEx(MatrixEx* dst, const MatrixEx* src) { *dst = *src; }
+inline void DeleteDeviceMatrixEx(MatrixEx* ex) { delete ex; }
 }  // namespace cuda
 
 class CuProceduresApi {
@@ -137,8 +232,67 @@ class CuProceduresApi {
     uintt dims[3][2] = {{o->columns, o->rows}, {a->columns, a->rows}, {b->columns, b->rows}};
     tensorProduct(o, a, b, dims);
   }
+  void tensorProduct(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt od[2], uintt ad[2],
+                     uintt bd[2]) {
+    uintt dims[3][2] = {{od[0], od[1]}, {ad[0], ad[1]}, {bd[0], bd[1]}};
+    tensorProduct(o, a, b, dims);
+  }
+  void tensorProduct(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt columns, uintt rows) {
+    check(oaph_tensor_product_cr(m_api, c(o), c(a), c(b), columns, rows), "tensorProduct");
+  }
   void hadamardProduct(math::Matrix* o, math::Matrix* a, math::Matrix* b) {
     check(oaph_hadamard_product(m_api, c(o), c(a), c(b)), "hadamardProduct");
+  }
+  void hadamardProduct(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt columns, uintt rows) {
+    check(oaph_hadamard_product_cr(m_api, c(o), c(a), c(b), columns, rows), "hadamardProduct");
+  }
+  void hadamardProductVec(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt columns,
+                          uintt rows) {
+    check(oaph_hadamard_product_vec_cr(m_api, c(o), c(a), c(b), columns, rows),
+          "hadamardProductVec");
+  }
+  // the reference kernel writes output = a b (CuAddDotProductProcedures.h:26-84)
+  void addDotProduct(math::Matrix* o, math::Matrix* a, math::Matrix* b) {
+    check(oaph_add_dot_product(m_api, c(o), c(a), c(b)), "addDotProduct");
+  }
+  void addDotProduct(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt, uintt) {
+    addDotProduct(o, a, b);
+  }
+  void dotProductPeriodic(math::Matrix* o, math::Matrix* a, math::Matrix* b) {
+    check(oaph_dot_product_periodic(m_api, c(o), c(a), c(b)), "dotProductPeriodic");
+  }
+  void dotProductDimPeriodic(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt dims[3][2],
+                             uintt periodicRows) {
+    check(oaph_dot_product_dim_periodic(m_api, c(o), c(a), c(b), dims, periodicRows),
+          "dotProductDimPeriodic");
+  }
+  void dotProductDimPeriodic(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt dims[3][2]) {
+    dotProductDimPeriodic(o, a, b, dims, a->rows);
+  }
+  void dotProductEx(math::Matrix* o, math::Matrix* a, math::Matrix* b, MatrixEx* ex, uintt, uintt) {
+    check(oaph_dot_product_ex(m_api, c(o), c(a), c(b), reinterpret_cast<oaph_matrix_ex*>(ex)),
+          "dotProductEx");
+  }
+  void dotProductEx(math::Matrix* o, math::Matrix* a, math::Matrix* b, MatrixEx* ex) {
+    dotProductEx(o, a, b, ex, ex->columns, ex->rows);
+  }
+  void dotProductExOpt(math::Matrix* o, math::Matrix* a, math::Matrix* b, MatrixEx* ex) {
+    dotProductEx(o, a, b, ex);
+  }
+  void dotProductOpt(math::Matrix* o, math::Matrix* a, math::Matrix* b) { dotProduct(o, a, b); }
+  void dotProductOpt(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt, uintt, uintt,
+                     uintt) {
+    dotProduct(o, a, b);
+  }
+  void transposeEx(math::Matrix* o, math::Matrix* a, MatrixEx* ex) {
+    check(oaph_transpose_ex(m_api, c(o), c(a), reinterpret_cast<oaph_matrix_ex*>(ex)),
+          "transposeEx");
+  }
+  // calculateQTHQ (CuProceduresApi.cpp:194-199): output = Q H Q^T by way of aux
+  void calculateQTHQ(math::Matrix* o, math::Matrix* H, math::Matrix* Q, math::Matrix* aux) {
+    transpose(o, Q);
+    dotProduct(aux, H, o);
+    dotProduct(o, Q, aux);
   }
   void hadamardProductVec(math::Matrix* o, math::Matrix* a, math::Matrix* b) {
     check(oaph_hadamard_product_vec(m_api, c(o), c(a), c(b)), "hadamardProductVec");
@@ -150,6 +304,19 @@ class CuProceduresApi {
   void add(math::Matrix* o, math::Matrix* a, math::Matrix* b) { check(oaph_add(m_api, c(o), c(a), c(b)), "add"); }
   void substract(math::Matrix* o, math::Matrix* a, math::Matrix* b) {
     check(oaph_substract(m_api, c(o), c(a), c(b)), "substract");
+  }
+  void add(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt columns, uintt rows) {
+    check(oaph_add_cr(m_api, c(o), c(a), c(b), columns, rows), "add");
+  }
+  void substract(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt columns, uintt rows) {
+    check(oaph_substract_cr(m_api, c(o), c(a), c(b), columns, rows), "substract");
+  }
+  // output += a - b (CuAddSubstractionProcedures.h:48-62)
+  void addSubstract(math::Matrix* o, math::Matrix* a, math::Matrix* b, uintt columns, uintt rows) {
+    check(oaph_add_substract(m_api, c(o), c(a), c(b), columns, rows), "addSubstract");
+  }
+  void addSubstract(math::Matrix* o, math::Matrix* a, math::Matrix* b) {
+    addSubstract(o, a, b, o->columns, o->rows);
   }
   void multiplyReConstant(math::Matrix* o, math::Matrix* a, floatt re) {
     check(oaph_multiply_re_constant(m_api, c(o), c(a), re), "multiplyReConstant");
@@ -172,6 +339,11 @@ class CuProceduresApi {
     getVector(v, v->columns * v->rows, V, column);
   }
   void magnitude(floatt& out, math::Matrix* m) { check(oaph_magnitude(m_api, &out, c(m)), "magnitude"); }
+  void magnitudeOpt(floatt& out, math::Matrix* m) { magnitude(out, m); }
+  void magnitudeOptVer2(floatt& out, math::Matrix* m) { magnitude(out, m); }
+  void magnitude2(floatt& out, math::Matrix* m) { check(oaph_magnitude2(m_api, &out, c(m)), "magnitude2"); }
+  void magnitude2Opt(floatt& out, math::Matrix* m) { magnitude2(out, m); }
+  void magnitude2OptVer2(floatt& out, math::Matrix* m) { magnitude2(out, m); }
   void sum(floatt& re, floatt& im, math::Matrix* m) { check(oaph_sum(m_api, &re, &im, c(m)), "sum"); }
   void sum(floatt& re, math::Matrix* m) {
     floatt im = 0;
@@ -191,6 +363,11 @@ class CuProceduresApi {
                        math::Matrix* a6) {
     check(oaph_calc_triangular_h(m_api, c(H), c(Q), c(R), c(a1), c(a2), c(a3), c(a4), c(a5), c(a6)),
           "calcTriangularH");
+  }
+  void calcTriangularHStep(math::Matrix* H, math::Matrix* Q, math::Matrix*, math::Matrix*,
+                           math::Matrix*, math::Matrix*, math::Matrix*, math::Matrix*,
+                           math::Matrix*) {
+    check(oaph_calc_triangular_h_step(m_api, c(H), c(Q)), "calcTriangularHStep");
   }
   oaph_api* handle() const { return m_api; }
 

@@ -57,6 +57,16 @@ typedef struct oaph_matrix {
   double* imValues;
 } oaph_matrix;
 
+/* MatrixEx (oapMatrix/MatrixEx.h:26-32): a region, columns [column,
+ * column + columns) and rows [row, row + rows).  Passed by pointer to host
+ * memory (the reference allocates it on the device). */
+typedef struct oaph_matrix_ex {
+  uint32_t column;
+  uint32_t columns;
+  uint32_t row;
+  uint32_t rows;
+} oaph_matrix_ex;
+
 /* math::MatrixInfo (oapMatrix/MatrixInfo.h:16-67) */
 typedef struct oaph_matrix_info {
   int32_t isRe;
@@ -98,6 +108,23 @@ oaph_status oaph_matrix_get_value(const oaph_matrix* m, uint32_t index, double* 
 oaph_status oaph_matrix_set_value(oaph_matrix* m, uint32_t index, double re, double im);
 /* CudaUtils::SetZeroRow (oapMatrixCuda/CudaUtils.cpp:332-359) */
 oaph_status oaph_matrix_set_zero_row(oaph_matrix* m, uint32_t row, int re, int im);
+/* CopyDeviceMatrixToDeviceMatrixDims / CopyHostMatrixToDeviceMatrixDims /
+ * CopyDeviceMatrixToHostMatrixDims (oapCudaMatrixUtils.h:102-116,
+ * oapHostMatrixUtils.h:803-892): dims[0] = {{column, columns}, {row, rows}}
+ * of dst, dims[1] the same of src (equal sizes); the block is copied plane
+ * by plane (planes present in both).  A host matrix is an oaph_matrix whose
+ * planes are host pointers. */
+oaph_status oaph_matrix_copy_d2d_dims(oaph_matrix* dst, const oaph_matrix* src,
+                                      const uint32_t dims[2][2][2]);
+oaph_status oaph_matrix_copy_h2d_dims(oaph_matrix* dst, const oaph_matrix* src_host,
+                                      const uint32_t dims[2][2][2]);
+oaph_status oaph_matrix_copy_d2h_dims(oaph_matrix* dst_host, const oaph_matrix* src,
+                                      const uint32_t dims[2][2][2]);
+/* SetMatrix / SetReMatrix / SetImMatrix(matrix, matrix1, column, row)
+ * (oapCudaMatrixUtils.h:142-146): all of src written into dst at
+ * (column, row); planes: 1 re, 2 im, 3 both (those present in both). */
+oaph_status oaph_matrix_set_submatrix(oaph_matrix* dst, const oaph_matrix* src, uint32_t column,
+                                      uint32_t row, int planes);
 
 /* ------------------------------------------------------------------ */
 /* matrix formats (host code; device matrices are copied once)         */
@@ -197,6 +224,9 @@ oaph_status oaph_set_vector(oaph_api* api, oaph_matrix* V, uint32_t column, cons
                             uint32_t length);
 oaph_status oaph_get_vector(oaph_api* api, oaph_matrix* v, uint32_t length, const oaph_matrix* V,
                             uint32_t column);
+/* magnitude2(floatt&, m) = sum re^2 + im^2 (CuProceduresApi.cpp:401-420;
+ * magnitude2Opt and magnitude2OptVer2 are the same value) */
+oaph_status oaph_magnitude2(oaph_api* api, double* out, const oaph_matrix* m);
 /* magnitude(floatt&, m) = sqrt(sum re^2 + im^2) (CuProceduresApi.cpp:342-345) */
 oaph_status oaph_magnitude(oaph_api* api, double* out, const oaph_matrix* m);
 /* Allocation-free form of oaph_magnitude; no counterpart in the reference.
@@ -223,6 +253,58 @@ oaph_status oaph_is_upper_triangular(oaph_api* api, const oaph_matrix* m, int* r
 oaph_status oaph_calc_triangular_h(oaph_api* api, oaph_matrix* H, oaph_matrix* Q, oaph_matrix* R,
                                    oaph_matrix* aux1, oaph_matrix* aux2, oaph_matrix* aux3,
                                    oaph_matrix* aux4, oaph_matrix* aux5, oaph_matrix* aux6);
+/* calcTriangularHStep (CuProceduresApi.cpp:482-490, CuTriangularH.h:61-77):
+ * one unshifted QR step H <- R Q, Q the step's factor (Q = I when H is
+ * already upper triangular; the reference leaves an aux matrix's previous
+ * contents there) */
+oaph_status oaph_calc_triangular_h_step(oaph_api* api, oaph_matrix* H, oaph_matrix* Q);
+
+/* dims forms and MatrixEx procedures */
+/* hadamardProduct / hadamardProductVec / tensorProduct (output, a, b, columns,
+ * rows) (CuProceduresApi.cpp:136-192): the given output size is checked; the
+ * operation is the one without the size (tensor: over columns x rows) */
+oaph_status oaph_hadamard_product_cr(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                     const oaph_matrix* b, uint32_t columns, uint32_t rows);
+oaph_status oaph_hadamard_product_vec_cr(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                         const oaph_matrix* b, uint32_t columns, uint32_t rows);
+oaph_status oaph_tensor_product_cr(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                   const oaph_matrix* b, uint32_t columns, uint32_t rows);
+/* add / substract (output, a, b, columns, rows) and addSubstract (out += a - b)
+ * over the columns x rows region at the top left (CuProceduresApi.cpp:309-325,
+ * CuAddSubstractionProcedures.h:48-62) */
+oaph_status oaph_add_cr(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                        const oaph_matrix* b, uint32_t columns, uint32_t rows);
+oaph_status oaph_substract_cr(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                              const oaph_matrix* b, uint32_t columns, uint32_t rows);
+oaph_status oaph_add_substract(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                               const oaph_matrix* b, uint32_t columns, uint32_t rows);
+/* addDotProduct (CuProceduresApi.cpp:119-134): the reference kernel writes
+ * output = a b (CuAddDotProductProcedures.h:26-84), not +=; kept */
+oaph_status oaph_add_dot_product(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                 const oaph_matrix* b);
+/* dotProductPeriodic (CuProceduresApi.cpp:211-214, CuDotProductPeriodicProcedures.h
+ * :65-85): with K = a.columns, out(x, y) = sum_k a(k, y % K) b(x, (y / K) K + k) */
+oaph_status oaph_dot_product_periodic(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                      const oaph_matrix* b);
+/* dotProductDimPeriodic (CuProceduresApi.cpp:216-219, CuDotProductDimPeriodic
+ * Procedures.h:27-45): W = dims[0][0], H = dims[0][1], K = dims[1][0],
+ * P = periodic_rows; for output rows y with y % P < H and x < W,
+ * out(x, y) = sum_k a(k, y % P) b(x, (y / P) K + k) */
+oaph_status oaph_dot_product_dim_periodic(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                          const oaph_matrix* b, const uint32_t dims[3][2],
+                                          uint32_t periodic_rows);
+/* dotProductEx / dotProductExOpt (output, a, b, matrixEx) (CuProceduresApi.cpp
+ * :201-209, 242-258): the product on the output region of matrixEx,
+ * out(x, y) = sum_{k < a.columns} a(k, y) b(x, k).  The reference launches a
+ * kernel its kernel file does not define; this is the region product. */
+oaph_status oaph_dot_product_ex(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                                const oaph_matrix* b, const oaph_matrix_ex* ex);
+/* transposeEx(output, a, matrixEx) (CuProceduresApi.cpp:260-266,
+ * CuTransposeProcedures.h:27-80): out(x, y) = a(y, x) for
+ * x < min(W, column + columns), y < min(H, row + rows), (W, H) the extent of
+ * the SetThreadsBlocks(columns, rows) grid the reference launches */
+oaph_status oaph_transpose_ex(oaph_api* api, oaph_matrix* out, const oaph_matrix* a,
+                              const oaph_matrix_ex* ex);
 
 /* utils::mapper::SetThreadsBlocks (oapUtils/ThreadsMapper.cpp:36-74): the
  * reference's launch-geometry map, kept bit-exact for callers that size

@@ -51,10 +51,19 @@ class oaph_arnoldi_stats(C.Structure):
                 ("fallbacks", C.c_uint32), ("mixed_declined", C.c_uint32)]
 
 
+class oaph_matrix_ex(C.Structure):
+    """MatrixEx (oapMatrix/MatrixEx.h:26-32): a region of a matrix."""
+
+    _fields_ = [("column", C.c_uint32), ("columns", C.c_uint32), ("row", C.c_uint32),
+                ("rows", C.c_uint32)]
+
+
 MP = C.POINTER(oaph_matrix)
 DP = C.POINTER(C.c_double)
 U32 = C.c_uint32
 DIMS = (C.c_uint32 * 2) * 3
+CDIMS = ((C.c_uint32 * 2) * 2) * 2
+MEXP = C.POINTER(oaph_matrix_ex)
 MULTIPLY_FN = C.CFUNCTYPE(None, MP, MP, C.c_void_p, C.c_void_p, C.c_int)
 CHECK_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_double, MP, C.c_uint32, C.c_uint32, C.c_void_p)
 
@@ -74,6 +83,10 @@ _SIGS = {
     "oaph_matrix_get_value": (C.c_int, [MP, U32, DP, DP]),
     "oaph_matrix_set_value": (C.c_int, [MP, U32, C.c_double, C.c_double]),
     "oaph_matrix_set_zero_row": (C.c_int, [MP, U32, C.c_int, C.c_int]),
+    "oaph_matrix_copy_d2d_dims": (C.c_int, [MP, MP, CDIMS]),
+    "oaph_matrix_copy_h2d_dims": (C.c_int, [MP, MP, CDIMS]),
+    "oaph_matrix_copy_d2h_dims": (C.c_int, [MP, MP, CDIMS]),
+    "oaph_matrix_set_submatrix": (C.c_int, [MP, MP, U32, U32, C.c_int]),
     "oaph_api_create": (C.c_int, [C.POINTER(C.c_void_p)]),
     "oaph_api_destroy": (C.c_int, [C.c_void_p]),
     "oaph_api_synchronize": (C.c_int, [C.c_void_p]),
@@ -96,6 +109,19 @@ _SIGS = {
     "oaph_set_vector": (C.c_int, [C.c_void_p, MP, U32, MP, U32]),
     "oaph_get_vector": (C.c_int, [C.c_void_p, MP, U32, MP, U32]),
     "oaph_magnitude": (C.c_int, [C.c_void_p, DP, MP]),
+    "oaph_magnitude2": (C.c_int, [C.c_void_p, DP, MP]),
+    "oaph_calc_triangular_h_step": (C.c_int, [C.c_void_p, MP, MP]),
+    "oaph_hadamard_product_cr": (C.c_int, [C.c_void_p, MP, MP, MP, U32, U32]),
+    "oaph_hadamard_product_vec_cr": (C.c_int, [C.c_void_p, MP, MP, MP, U32, U32]),
+    "oaph_tensor_product_cr": (C.c_int, [C.c_void_p, MP, MP, MP, U32, U32]),
+    "oaph_add_cr": (C.c_int, [C.c_void_p, MP, MP, MP, U32, U32]),
+    "oaph_substract_cr": (C.c_int, [C.c_void_p, MP, MP, MP, U32, U32]),
+    "oaph_add_substract": (C.c_int, [C.c_void_p, MP, MP, MP, U32, U32]),
+    "oaph_add_dot_product": (C.c_int, [C.c_void_p, MP, MP, MP]),
+    "oaph_dot_product_periodic": (C.c_int, [C.c_void_p, MP, MP, MP]),
+    "oaph_dot_product_dim_periodic": (C.c_int, [C.c_void_p, MP, MP, MP, DIMS, U32]),
+    "oaph_dot_product_ex": (C.c_int, [C.c_void_p, MP, MP, MP, MEXP]),
+    "oaph_transpose_ex": (C.c_int, [C.c_void_p, MP, MP, MEXP]),
     "oaph_magnitude_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, MP]),
     "oaph_sum": (C.c_int, [C.c_void_p, DP, DP, MP]),
     "oaph_qrgr": (C.c_int, [C.c_void_p, MP, MP, MP, MP, MP, MP, MP]),

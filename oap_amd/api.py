@@ -120,6 +120,42 @@ class DeviceMatrix:
     def set_value(self, index: int, re: float, im: float = 0.0) -> None:
         check(load().oaph_matrix_set_value(self._p, index, re, im), "oaph_matrix_set_value")
 
+    def set_zero_row(self, row: int, re: bool = True, im: bool = True) -> None:
+        """SetZeroRow (oapCudaMatrixUtils.h:186)"""
+        check(load().oaph_matrix_set_zero_row(self._p, row, int(re), int(im)),
+              "oaph_matrix_set_zero_row")
+
+    def copy_from(self, src: "DeviceMatrix") -> None:
+        """CopyDeviceMatrixToDeviceMatrix (oapCudaMatrixUtils.h:95)"""
+        check(load().oaph_matrix_copy_d2d(self._p, src.ptr), "oaph_matrix_copy_d2d")
+
+    def copy_from_dims(self, src: "DeviceMatrix", dims) -> None:
+        """CopyDeviceMatrixToDeviceMatrixDims: dims[0] = [[column, columns],
+        [row, rows]] of this matrix, dims[1] of src"""
+        check(load().oaph_matrix_copy_d2d_dims(self._p, src.ptr, _cdims(dims)),
+              "oaph_matrix_copy_d2d_dims")
+
+    def upload_dims(self, re, im, dims) -> None:
+        """CopyHostMatrixToDeviceMatrixDims from host planes (rows x columns)"""
+        hm, keep = _host_matrix(re, im)
+        check(load().oaph_matrix_copy_h2d_dims(self._p, C.byref(hm), _cdims(dims)),
+              "oaph_matrix_copy_h2d_dims")
+        del keep
+
+    def download_dims(self, re, im, dims) -> None:
+        """CopyDeviceMatrixToHostMatrixDims into host planes (rows x columns,
+        C-contiguous float64 arrays, written in place)"""
+        hm, keep = _host_matrix(re, im)
+        check(load().oaph_matrix_copy_d2h_dims(C.byref(hm), self._p, _cdims(dims)),
+              "oaph_matrix_copy_d2h_dims")
+        del keep
+
+    def set_matrix(self, src: "DeviceMatrix", column: int, row: int, planes: int = 3) -> None:
+        """SetMatrix (planes 3) / SetReMatrix (1) / SetImMatrix (2)
+        (oapCudaMatrixUtils.h:142-146): src written at (column, row)"""
+        check(load().oaph_matrix_set_submatrix(self._p, src.ptr, column, row, planes),
+              "oaph_matrix_set_submatrix")
+
     def free(self) -> None:
         if getattr(self, "_p", None):
             load().oaph_matrix_free(self._p)
@@ -142,6 +178,31 @@ def _dims(d) -> DIMS:
 
 def _m(x):
     return None if x is None else x.ptr
+
+
+def _cdims(d):
+    from ._lib import CDIMS
+    arr = CDIMS()
+    for i in range(2):
+        for j in range(2):
+            for k in range(2):
+                arr[i][j][k] = int(d[i][j][k])
+    return arr
+
+
+def _host_matrix(re, im):
+    """An oaph_matrix over host planes (rows x columns float64, C-contiguous;
+    either may be None); returns it with the arrays it points to."""
+    ref = re if re is not None else im
+    for a in (re, im):
+        if a is not None and (a.dtype != np.float64 or not a.flags.c_contiguous or a.ndim != 2):
+            raise ValueError("host planes must be C-contiguous 2-D float64 arrays")
+    hm = oaph_matrix()
+    hm.rows, hm.columns = ref.shape
+    hm.realRows, hm.realColumns = ref.shape
+    hm.reValues = None if re is None else re.ctypes.data
+    hm.imValues = None if im is None else im.ctypes.data
+    return hm, (re, im)
 
 
 class CuProceduresApi:
@@ -199,6 +260,44 @@ class CuProceduresApi:
         check(self._lib.oaph_hadamard_product_vec(self._h, out.ptr, a.ptr, b.ptr),
               "hadamardProductVec")
 
+    def tensorProductCR(self, out, a, b, columns: int, rows: int):
+        """tensorProduct(output, a, b, columns, rows) (CuProceduresApi.cpp:136-151)"""
+        check(self._lib.oaph_tensor_product_cr(self._h, out.ptr, a.ptr, b.ptr, columns, rows),
+              "tensorProduct(columns, rows)")
+
+    def hadamardProductCR(self, out, a, b, columns: int, rows: int):
+        check(self._lib.oaph_hadamard_product_cr(self._h, out.ptr, a.ptr, b.ptr, columns, rows),
+              "hadamardProduct(columns, rows)")
+
+    def hadamardProductVecCR(self, out, a, b, columns: int, rows: int):
+        check(self._lib.oaph_hadamard_product_vec_cr(self._h, out.ptr, a.ptr, b.ptr, columns,
+                                                     rows), "hadamardProductVec(columns, rows)")
+
+    def addDotProduct(self, out, a, b):
+        """the reference kernel writes output = a b (not +=)"""
+        check(self._lib.oaph_add_dot_product(self._h, out.ptr, a.ptr, b.ptr), "addDotProduct")
+
+    def dotProductPeriodic(self, out, a, b):
+        check(self._lib.oaph_dot_product_periodic(self._h, out.ptr, a.ptr, b.ptr),
+              "dotProductPeriodic")
+
+    def dotProductDimPeriodic(self, out, a, b, dims, periodic_rows=None):
+        if periodic_rows is None:
+            periodic_rows = a.rows  # CuProceduresApi.h:149-153
+        check(self._lib.oaph_dot_product_dim_periodic(self._h, out.ptr, a.ptr, b.ptr,
+                                                      _dims(dims), periodic_rows),
+              "dotProductDimPeriodic")
+
+    def dotProductEx(self, out, a, b, ex):
+        check(self._lib.oaph_dot_product_ex(self._h, out.ptr, a.ptr, b.ptr, C.byref(_mex(ex))),
+              "dotProductEx")
+
+    dotProductExOpt = dotProductEx
+
+    def transposeEx(self, out, a, ex):
+        check(self._lib.oaph_transpose_ex(self._h, out.ptr, a.ptr, C.byref(_mex(ex))),
+              "transposeEx")
+
     # ---- element-wise
     def transpose(self, out, a):
         check(self._lib.oaph_transpose(self._h, out.ptr, a.ptr), "transpose")
@@ -211,6 +310,21 @@ class CuProceduresApi:
 
     def substract(self, out, a, b):
         check(self._lib.oaph_substract(self._h, out.ptr, a.ptr, b.ptr), "substract")
+
+    def addCR(self, out, a, b, columns: int, rows: int):
+        """add(output, a, b, columns, rows): the columns x rows region"""
+        check(self._lib.oaph_add_cr(self._h, out.ptr, a.ptr, b.ptr, columns, rows), "add(cr)")
+
+    def substractCR(self, out, a, b, columns: int, rows: int):
+        check(self._lib.oaph_substract_cr(self._h, out.ptr, a.ptr, b.ptr, columns, rows),
+              "substract(cr)")
+
+    def addSubstract(self, out, a, b, columns=None, rows=None):
+        """output += a - b over the columns x rows region (default: all of out)"""
+        columns = out.columns if columns is None else columns
+        rows = out.rows if rows is None else rows
+        check(self._lib.oaph_add_substract(self._h, out.ptr, a.ptr, b.ptr, columns, rows),
+              "addSubstract")
 
     def multiplyReConstant(self, out, a, re: float):
         check(self._lib.oaph_multiply_re_constant(self._h, out.ptr, a.ptr, re),
@@ -240,6 +354,16 @@ class CuProceduresApi:
         check(self._lib.oaph_magnitude(self._h, C.byref(out), m.ptr), "magnitude")
         return out.value
 
+    def magnitude2(self, m) -> float:
+        out = C.c_double()
+        check(self._lib.oaph_magnitude2(self._h, C.byref(out), m.ptr), "magnitude2")
+        return out.value
+
+    magnitude2Opt = magnitude2
+    magnitude2OptVer2 = magnitude2
+    magnitudeOpt = magnitude
+    magnitudeOptVer2 = magnitude
+
     def sum(self, m) -> tuple[float, float]:
         re, im = C.c_double(), C.c_double()
         check(self._lib.oaph_sum(self._h, C.byref(re), C.byref(im), m.ptr), "sum")
@@ -259,6 +383,18 @@ class CuProceduresApi:
         a = list(aux) + [None] * (6 - len(aux))
         check(self._lib.oaph_calc_triangular_h(self._h, H.ptr, Q.ptr, _m(R), *[_m(x) for x in a]),
               "calcTriangularH")
+
+
+    def calcTriangularHStep(self, H, Q, R=None, *aux):
+        check(self._lib.oaph_calc_triangular_h_step(self._h, H.ptr, Q.ptr), "calcTriangularHStep")
+
+
+def _mex(ex):
+    """MatrixEx from a (column, columns, row, rows) sequence or struct"""
+    from ._lib import oaph_matrix_ex
+    if isinstance(ex, oaph_matrix_ex):
+        return ex
+    return oaph_matrix_ex(*[int(v) for v in ex])
 
 
 class Activation:

@@ -63,15 +63,14 @@ def build(verbose: bool = True) -> Path:
     if res.returncode != 0:
         raise RuntimeError(f"link failed:\n{res.stderr}")
     os.replace(tmp, LIB)
-    # reference-style C++ caller of the facade header (include/oap/oap.h)
-    example = PKG.parent / "tests" / "cpp" / "facade_example.cpp"
-    if example.exists():
-        cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-I", str(PKG.parent / "include"),
-               str(example), "-L", str(PKG), "-loaph", "-Wl,-rpath,$ORIGIN/../../oap_amd",
-               "-o", str(example.with_suffix(""))]
+    # reference-style C++ callers of the facade header (include/oap/oap.h)
+    for example in sorted((PKG.parent / "tests" / "cpp").glob("*.cpp")):
+        cmd = [os.environ.get("CXX", "g++"), "-std=c++17", "-O2", "-Wall", "-I",
+               str(PKG.parent / "include"), str(example), "-L", str(PKG), "-loaph",
+               "-Wl,-rpath,$ORIGIN/../../oap_amd", "-o", str(example.with_suffix(""))]
         res = subprocess.run(cmd, capture_output=True, text=True)
         if res.returncode != 0:
-            raise RuntimeError(f"facade example failed to build:\n{res.stderr}")
+            raise RuntimeError(f"{example.name} failed to build:\n{res.stderr}")
     if verbose:
         print(f"built {LIB} from {len(sources)} sources", flush=True)
     return LIB

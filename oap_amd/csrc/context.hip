@@ -235,6 +235,61 @@ oaph_status oaph_matrix_set_zero_row(oaph_matrix* m, uint32_t row, int re, int i
   return OAPH_OK;
 }
 
+// Copy of a columns x rows block between matrices (either side host or
+// device), plane by plane, for the planes both have.
+static oaph_status copy_block(oaph_matrix* dst, const oaph_matrix* src, uint32_t dcol,
+                              uint32_t drow, uint32_t scol, uint32_t srow, uint32_t columns,
+                              uint32_t rows, int planes, hipMemcpyKind kind) {
+  OAPH_CHECK((uint64_t)dcol + columns <= dst->columns && (uint64_t)drow + rows <= dst->rows &&
+                 (uint64_t)scol + columns <= src->columns && (uint64_t)srow + rows <= src->rows,
+             OAPH_ERR_SHAPE);
+  if (columns == 0 || rows == 0) return OAPH_OK;
+  OAPH_TRY(ensure_init());
+  const size_t doff = (size_t)drow * dst->columns + dcol, soff = (size_t)srow * src->columns + scol;
+  const size_t dpitch = (size_t)dst->columns * sizeof(double), spitch = (size_t)src->columns * sizeof(double);
+  OAPH_HIP(hipDeviceSynchronize());
+  if ((planes & 1) && dst->reValues && src->reValues)
+    OAPH_HIP(hipMemcpy2D(dst->reValues + doff, dpitch, src->reValues + soff, spitch,
+                         (size_t)columns * sizeof(double), rows, kind));
+  if ((planes & 2) && dst->imValues && src->imValues)
+    OAPH_HIP(hipMemcpy2D(dst->imValues + doff, dpitch, src->imValues + soff, spitch,
+                         (size_t)columns * sizeof(double), rows, kind));
+  return OAPH_OK;
+}
+
+static oaph_status copy_dims(oaph_matrix* dst, const oaph_matrix* src,
+                             const uint32_t dims[2][2][2], hipMemcpyKind kind) {
+  OAPH_CHECK(dst && src && dims, OAPH_ERR_ARG);
+  // check_CopyMatrixToMatrixDims (oapHostMatrixUtils.h:803-817): equal block
+  // sizes, same planes
+  OAPH_CHECK(dims[0][0][1] == dims[1][0][1] && dims[0][1][1] == dims[1][1][1], OAPH_ERR_SHAPE);
+  OAPH_CHECK((dst->reValues != nullptr) == (src->reValues != nullptr) &&
+                 (dst->imValues != nullptr) == (src->imValues != nullptr),
+             OAPH_ERR_PLANE);
+  return copy_block(dst, src, dims[0][0][0], dims[0][1][0], dims[1][0][0], dims[1][1][0],
+                    dims[0][0][1], dims[0][1][1], 3, kind);
+}
+
+oaph_status oaph_matrix_copy_d2d_dims(oaph_matrix* dst, const oaph_matrix* src,
+                                      const uint32_t dims[2][2][2]) {
+  return copy_dims(dst, src, dims, hipMemcpyDeviceToDevice);
+}
+oaph_status oaph_matrix_copy_h2d_dims(oaph_matrix* dst, const oaph_matrix* src_host,
+                                      const uint32_t dims[2][2][2]) {
+  return copy_dims(dst, src_host, dims, hipMemcpyHostToDevice);
+}
+oaph_status oaph_matrix_copy_d2h_dims(oaph_matrix* dst_host, const oaph_matrix* src,
+                                      const uint32_t dims[2][2][2]) {
+  return copy_dims(dst_host, src, dims, hipMemcpyDeviceToHost);
+}
+
+oaph_status oaph_matrix_set_submatrix(oaph_matrix* dst, const oaph_matrix* src, uint32_t column,
+                                      uint32_t row, int planes) {
+  OAPH_CHECK(dst && src && planes >= 1 && planes <= 3, OAPH_ERR_ARG);
+  return copy_block(dst, src, column, row, 0, 0, src->columns, src->rows, planes,
+                    hipMemcpyDeviceToDevice);
+}
+
 // ---------------------------------------------------------------- api objects
 oaph_status oaph_api_create(oaph_api** out) {
   OAPH_CHECK(out, OAPH_ERR_ARG);

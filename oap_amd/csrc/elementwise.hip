@@ -144,7 +144,138 @@ __global__ void k_vector(double* __restrict__ Vre, double* __restrict__ Vim, uin
   }
 }
 
+// ------------------------------------------------------------ region forms
+// add / substract / addSubstract over the columns x rows region at the top
+// left of the matrices (CuProceduresApi.cpp:309-325 launch their kernels on
+// a columns x rows grid; CuAddSubstractionProcedures.h:48-62: out += a - b).
+// OP 0 add, 1 substract, 4 addSubstract.
+template <int OP>
+__global__ void k_binary_region(double* __restrict__ ore, double* __restrict__ oim,
+                                const double* __restrict__ are, const double* __restrict__ aim,
+                                const double* __restrict__ bre, const double* __restrict__ bim,
+                                uint32_t ocols, uint32_t acols, uint32_t bcols, uint32_t W,
+                                uint32_t H) {
+  const size_t n = (size_t)W * H;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t y = (uint32_t)(i / W), x = (uint32_t)(i - (size_t)y * W);
+    const size_t io = (size_t)y * ocols + x, ia = (size_t)y * acols + x, ib = (size_t)y * bcols + x;
+    if (ore) {
+      const double a = are[ia], b = bre[ib];
+      ore[io] = OP == 0 ? a + b : OP == 1 ? a - b : ore[io] + (a - b);
+    }
+    if (oim) {
+      const double a = aim[ia], b = bim[ib];
+      oim[io] = OP == 0 ? a + b : OP == 1 ? a - b : oim[io] + (a - b);
+    }
+  }
+}
+
+// out(x, y) = a(y, x) for x < W, y < H (transposeEx: the region the
+// reference's launch grid and its MatrixEx bound select,
+// CuTransposeProcedures.h:27-80)
+__global__ void k_transpose_region(double* __restrict__ ore, double* __restrict__ oim,
+                                   const double* __restrict__ are, const double* __restrict__ aim,
+                                   uint32_t ocols, uint32_t acols, uint32_t W, uint32_t H) {
+  const size_t n = (size_t)W * H;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t y = (uint32_t)(i / W), x = (uint32_t)(i - (size_t)y * W);
+    const size_t io = (size_t)y * ocols + x, ia = (size_t)x * acols + y;
+    if (ore) ore[io] = are[ia];
+    if (oim) oim[io] = aim[ia];
+  }
+}
+
+// General row-mapped product for the periodic and region forms of
+// dotProduct: for x in [x0, x0 + W) and output rows y in [y0, y0 + H),
+//   out(x, y) = sum_{k < K} A(k, arow(y)) * B(x, brow(y) + k)
+// with arow(y) = (y - y0) % P + ay0 and brow(y) = ((y - y0) / P) * Q + by0
+// (a plain region: P = H, Q = 0); rows whose (y - y0) % P is not below S are
+// left alone (the periodic form's rows outside each period's block).  The k order and the per-plane
+// accumulation follow cuda_generic_dotProductRealExOffset
+// (CuDotProductGenericProcedures.h:86-104).  MODE 0: complex, 1: re only.
+template <int MODE>
+__global__ void k_dot_mapped(double* __restrict__ ore, double* __restrict__ oim,
+                             const double* __restrict__ are, const double* __restrict__ aim,
+                             const double* __restrict__ bre, const double* __restrict__ bim,
+                             uint32_t ocols, uint32_t acols, uint32_t bcols, uint32_t x0,
+                             uint32_t y0, uint32_t W, uint32_t H, uint32_t K, uint32_t P,
+                             uint32_t Q, uint32_t ay0, uint32_t by0, uint32_t S) {
+  const size_t n = (size_t)W * H;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const uint32_t dy = (uint32_t)(i / W), dx = (uint32_t)(i - (size_t)dy * W);
+    const uint32_t x = x0 + dx, y = y0 + dy;
+    if (dy % P >= S) continue;
+    const size_t ra = (size_t)(dy % P + ay0) * acols;
+    const size_t rb = (size_t)((dy / P) * Q + by0);
+    double sr = 0.0, si = 0.0;
+    for (uint32_t k = 0; k < K; ++k) {
+      const size_t ia = ra + k, ib = (rb + k) * bcols + x;
+      if (MODE == 0) {
+        sr += are[ia] * bre[ib];
+        sr -= aim[ia] * bim[ib];
+        si += are[ia] * bim[ib];
+        si += aim[ia] * bre[ib];
+      } else {
+        sr += are[ia] * bre[ib];
+      }
+    }
+    ore[(size_t)y * ocols + x] = sr;
+    if (MODE == 0) oim[(size_t)y * ocols + x] = si;
+  }
+}
+
 }  // namespace
+
+oaph_status launch_binary_region(hipStream_t s, int op, oaph_matrix* out, const oaph_matrix* a,
+                                 const oaph_matrix* b, uint32_t W, uint32_t H) {
+  const size_t n = (size_t)W * H;
+  if (n == 0) return OAPH_OK;
+  dim3 g(grid_for(n, 4)), t(kThreads);
+  double* ore = out->reValues;
+  double* oim = out->imValues;
+  if (op == 0)
+    hipLaunchKernelGGL(k_binary_region<0>, g, t, 0, s, ore, oim, a->reValues, a->imValues,
+                       b->reValues, b->imValues, out->columns, a->columns, b->columns, W, H);
+  else if (op == 1)
+    hipLaunchKernelGGL(k_binary_region<1>, g, t, 0, s, ore, oim, a->reValues, a->imValues,
+                       b->reValues, b->imValues, out->columns, a->columns, b->columns, W, H);
+  else
+    hipLaunchKernelGGL(k_binary_region<4>, g, t, 0, s, ore, oim, a->reValues, a->imValues,
+                       b->reValues, b->imValues, out->columns, a->columns, b->columns, W, H);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status launch_transpose_region(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
+                                    uint32_t W, uint32_t H) {
+  const size_t n = (size_t)W * H;
+  if (n == 0) return OAPH_OK;
+  hipLaunchKernelGGL(k_transpose_region, dim3(grid_for(n, 4)), dim3(kThreads), 0, s,
+                     out->reValues, a->imValues ? out->imValues : nullptr, a->reValues,
+                     a->imValues, out->columns, a->columns, W, H);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status launch_dot_mapped(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
+                              const oaph_matrix* b, const DotMap& mp) {
+  const size_t n = (size_t)mp.W * mp.H;
+  if (n == 0) return OAPH_OK;
+  dim3 g(grid_for(n)), t(kThreads);
+  if (out->imValues)
+    hipLaunchKernelGGL(k_dot_mapped<0>, g, t, 0, s, out->reValues, out->imValues, a->reValues,
+                       a->imValues, b->reValues, b->imValues, out->columns, a->columns,
+                       b->columns, mp.x0, mp.y0, mp.W, mp.H, mp.K, mp.P, mp.Q, mp.ay0, mp.by0, mp.S);
+  else
+    hipLaunchKernelGGL(k_dot_mapped<1>, g, t, 0, s, out->reValues, nullptr, a->reValues, nullptr,
+                       b->reValues, nullptr, out->columns, a->columns, b->columns, mp.x0, mp.y0,
+                       mp.W, mp.H, mp.K, mp.P, mp.Q, mp.ay0, mp.by0, mp.S);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
 
 oaph_status launch_transpose(hipStream_t s, oaph_matrix* out, const oaph_matrix* a, bool conj) {
   dim3 grid((out->columns + TT - 1) / TT, (out->rows + TT - 1) / TT);

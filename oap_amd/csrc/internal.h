@@ -80,6 +80,21 @@ oaph_status launch_set_vector(hipStream_t s, oaph_matrix* V, uint32_t column, co
                               uint32_t length);
 oaph_status launch_get_vector(hipStream_t s, oaph_matrix* v, uint32_t length, const oaph_matrix* V,
                               uint32_t column);
+// region forms: op 0 add, 1 substract, 4 addSubstract (out += a - b) over
+// the W x H region at the top left
+oaph_status launch_binary_region(hipStream_t s, int op, oaph_matrix* out, const oaph_matrix* a,
+                                 const oaph_matrix* b, uint32_t W, uint32_t H);
+// out(x, y) = a(y, x) for x < W, y < H
+oaph_status launch_transpose_region(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
+                                    uint32_t W, uint32_t H);
+// Row-mapped product: out(x, y) = sum_{k<K} a(k, (y-y0)%P + ay0) b(x, ((y-y0)/P) Q + by0 + k)
+// for x0 <= x < x0 + W, y0 <= y < y0 + H with (y-y0)%P < S; complex when out
+// has an im plane, else re only.
+struct DotMap {
+  uint32_t x0 = 0, y0 = 0, W = 0, H = 0, K = 0, P = 1, Q = 0, ay0 = 0, by0 = 0, S = 1;
+};
+oaph_status launch_dot_mapped(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
+                              const oaph_matrix* b, const DotMap& mp);
 // reduce.hip
 oaph_status launch_sumsq(hipStream_t s, const double* re, const double* im, size_t n,
                          double* partials, double* result);

@@ -14,6 +14,8 @@ Two kinds of fixtures:
         oapTensorProductTests_Data_1.h with the shapes their tests use -> products.json
       - the expected eigenvalues of the data1-7 callback tests
         (oapArnoldiPackageCallbackTest.cpp:153-187; operators absent)  -> callback_eigs.json
+      - the TransponseReMatrixExTest1-3 arrays of
+        oapDeviceTests/oapMatrixCudaTest.cpp:280-375                  -> transpose_ex.json
   * dense LAPACK eigenvalues (numpy) of the seeded synthetic matrices the
     oracle generates: c0_eigs.json (64 x 64 real), c1_top8.json (8192 x 8192
     complex, optional, slow: --c1).
@@ -125,6 +127,29 @@ def products_fixtures():
     return d
 
 
+def transpose_ex_fixtures():
+    """The TransponseReMatrixExTest1-3 cases of oapDeviceTests/oapMatrixCudaTest.cpp
+    :280-375: input array, output and input sizes, the MatrixEx and the expected
+    output (the device output matrix starts zeroed)."""
+    src = (REF / "oapDeviceTests" / "oapMatrixCudaTest.cpp").read_text()
+    cases = []
+    for i in (1, 2, 3):
+        start = src.index(f"TransponseReMatrixExTest{i})")
+        body = src[start:src.index("TEST_F", start + 10)]
+        arr = numbers(body[body.index("hArray[] = {") + 11:body.index("};", body.index("hArray[]"))])
+        out = numbers(body[body.index("hOutputArray[] = {") + 17:
+                           body.index("};", body.index("hOutputArray[]"))])
+        dev = re.findall(r"NewDeviceMatrix\s*\(output,\s*(\d+),\s*(\d+)\)", body)
+        ex = [int(v) for v in re.search(r"hMatrixEx\s*=\s*\{([^}]*)\}", body).group(1).split(",")]
+        eq = re.search(r"eq_output\s*=\s*oap::host::NewReMatrixCopy\((\d+),\s*(\d+)", body)
+        cases.append({"name": f"TransponseReMatrixExTest{i}", "input": arr,
+                      "output_columns": int(dev[0][0]), "output_rows": int(dev[0][1]),
+                      "input_columns": int(dev[1][0]), "input_rows": int(dev[1][1]),
+                      "matrix_ex": ex, "expected": out,
+                      "expected_columns": int(eq.group(1)), "expected_rows": int(eq.group(2))})
+    return {"cases": cases}
+
+
 def callback_eigs():
     src = (REF / "oapArnoldiDeviceTests" / "oapArnoldiPackageCallbackTest.cpp").read_text()
     vals = re.findall(r"executeArnoldiTest\s*\(\s*(" + NUM + r")\s*,\s*\"([^\"]+)\"", src)
@@ -178,6 +203,7 @@ def main():
         (HERE / "cmatrixdata.json").write_text(json.dumps(cmatrixdata_fixtures()))
         (HERE / "products.json").write_text(json.dumps(products_fixtures()))
         (HERE / "callback_eigs.json").write_text(json.dumps(callback_eigs()))
+        (HERE / "transpose_ex.json").write_text(json.dumps(transpose_ex_fixtures()))
     (HERE / "c0_eigs.json").write_text(json.dumps(c0_eigs()))
     if "--c1" in sys.argv:
         (HERE / "c1_top8.json").write_text(json.dumps(c1_top8()))
