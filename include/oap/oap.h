@@ -475,16 +475,30 @@ class EigenPair {
 };
 
 namespace ArnUtils {
-// The sort "functions" of the reference are comparator pointers; here they
-// are tags that select the same order on the device (ArnoldiUtils.cpp:24-50).
-enum SortType {
-  SortLargestValues = OAPH_SORT_LARGEST_VALUES,
-  SortLargestReValues = OAPH_SORT_LARGEST_RE,
-  SortLargestImValues = OAPH_SORT_LARGEST_IM,
-  SortSmallestValues = OAPH_SORT_SMALLEST_VALUES,
-  SortSmallestReValues = OAPH_SORT_SMALLEST_RE,
-  SortSmallestImValues = OAPH_SORT_SMALLEST_IM,
-};
+// Comparators (ArnoldiUtils.h:63-72, ArnoldiUtils.cpp:24-50): true when i
+// sorts before j.  setSortType maps these six to the same orders applied on
+// the device; any other comparator orders the Ritz values on the host.
+inline bool SortLargestValues(const EigenPair& i, const EigenPair& j) {
+  return i.re() * i.re() + i.im() * i.im() > j.re() * j.re() + j.im() * j.im();
+}
+inline bool SortLargestReValues(const EigenPair& i, const EigenPair& j) { return i.re() > j.re(); }
+inline bool SortLargestImValues(const EigenPair& i, const EigenPair& j) { return i.im() > j.im(); }
+inline bool SortSmallestValues(const EigenPair& i, const EigenPair& j) {
+  return i.re() * i.re() + i.im() * i.im() < j.re() * j.re() + j.im() * j.im();
+}
+inline bool SortSmallestReValues(const EigenPair& i, const EigenPair& j) { return i.re() < j.re(); }
+inline bool SortSmallestImValues(const EigenPair& i, const EigenPair& j) { return i.im() < j.im(); }
+typedef bool (*SortType)(const EigenPair& i, const EigenPair& j);
+// the device-side code of a built-in comparator, -1 for any other
+inline int builtinSortCode(SortType t) {
+  if (t == &SortLargestValues) return OAPH_SORT_LARGEST_VALUES;
+  if (t == &SortLargestReValues) return OAPH_SORT_LARGEST_RE;
+  if (t == &SortLargestImValues) return OAPH_SORT_LARGEST_IM;
+  if (t == &SortSmallestValues) return OAPH_SORT_SMALLEST_VALUES;
+  if (t == &SortSmallestReValues) return OAPH_SORT_SMALLEST_RE;
+  if (t == &SortSmallestImValues) return OAPH_SORT_SMALLEST_IM;
+  return -1;
+}
 enum CheckType { CHECK_INTERNAL, CHECK_FIRST_STOP, CHECK_COUNTER };
 enum TriangularHProcedureType { CALC_IN_DEVICE, CALC_IN_HOST, CALC_IN_DEVICE_STEP };
 enum Type { UNDEFINED, DEVICE, HOST };
@@ -505,7 +519,15 @@ class CuHArnoldi {
   void setVecInitType(oap::InitVVectorType t) {
     oap::check(oaph_arnoldi_set_vec_init_type(m_a, (int)t), "setVecInitType");
   }
-  void setSortType(ArnUtils::SortType t) { oap::check(oaph_arnoldi_set_sort_type(m_a, (int)t), "setSortType"); }
+  void setSortType(ArnUtils::SortType t) {
+    const int code = ArnUtils::builtinSortCode(t);
+    if (code >= 0) {
+      oap::check(oaph_arnoldi_set_sort_type(m_a, code), "setSortType");
+    } else {
+      m_sort = t;
+      oap::check(oaph_arnoldi_set_sort_fn(m_a, &CuHArnoldi::sortTrampoline, this), "setSortType");
+    }
+  }
   void setCheckType(ArnUtils::CheckType t) {
     oap::check(oaph_arnoldi_set_check_type(m_a, (int)t), "setCheckType");
   }
@@ -573,6 +595,11 @@ class CuHArnoldi {
     oap::check(oaph_arnoldi_set_check_callback(m_a, &CuHArnoldi::ctramp, this), "set_check");
   }
   oaph_arnoldi* m_a = nullptr;
+  ArnUtils::SortType m_sort = nullptr;
+  static int sortTrampoline(double ri, double ii, double rj, double ij, void* user) {
+    const CuHArnoldi* self = static_cast<const CuHArnoldi*>(user);
+    return self->m_sort(EigenPair(Complex(ri, ii), 0u), EigenPair(Complex(rj, ij), 1u)) ? 1 : 0;
+  }
   std::exception_ptr m_pending;
 
  private:

@@ -357,6 +357,9 @@ typedef struct oaph_arnoldi oaph_arnoldi;
  * must leave the result enqueued on that stream. */
 typedef void (*oaph_multiply_fn)(oaph_matrix* w, oaph_matrix* v, oaph_api* api, void* user,
                                  int type);
+/* ArnUtils::SortType (ArnoldiUtils.h:63-72) as a C comparator: nonzero when
+ * Ritz value i = (re_i, im_i) sorts before j (the wanted ones first). */
+typedef int (*oaph_sort_fn)(double re_i, double im_i, double re_j, double im_j, void* user);
 /* CuHArnoldiCallbackBase::CheckFunc (ArnoldiProceduresImpl.h:53) */
 typedef int (*oaph_check_fn)(double re, double im, oaph_matrix* vector, uint32_t index,
                              uint32_t max, void* user);
@@ -373,6 +376,12 @@ oaph_status oaph_arnoldi_set_qr_type(oaph_arnoldi* a, int qrtype);
 oaph_status oaph_arnoldi_set_vec_init_type(oaph_arnoldi* a, int init_type);
 oaph_status oaph_arnoldi_set_sort_type(oaph_arnoldi* a, int sort_type);
 oaph_status oaph_arnoldi_set_check_type(oaph_arnoldi* a, int check_type);
+/* CuHArnoldi::setSortType with a caller's comparator (ArnoldiProcedures.cpp
+ * :87-90, 370-380): the Ritz values of each restart are ordered on the host
+ * by a stable sort with `fn` (one more launch and host round trip per
+ * restart); oaph_arnoldi_set_sort_type (the built-in orders, applied on the
+ * device) replaces it, and fn = NULL removes it. */
+oaph_status oaph_arnoldi_set_sort_fn(oaph_arnoldi* a, oaph_sort_fn fn, void* user);
 oaph_status oaph_arnoldi_set_check_counts(oaph_arnoldi* a, uint32_t count);
 /* residual tolerance: a pair is converged when ||A x - l x|| <= tol * norm_a
  * (the reference stores this value but never reads it, ArnoldiProcedures.cpp:102-105) */

@@ -65,6 +65,7 @@ DIMS = (C.c_uint32 * 2) * 3
 CDIMS = ((C.c_uint32 * 2) * 2) * 2
 MEXP = C.POINTER(oaph_matrix_ex)
 MULTIPLY_FN = C.CFUNCTYPE(None, MP, MP, C.c_void_p, C.c_void_p, C.c_int)
+SORT_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_void_p)
 CHECK_FN = C.CFUNCTYPE(C.c_int, C.c_double, C.c_double, MP, C.c_uint32, C.c_uint32, C.c_void_p)
 
 # name -> (restype, argtypes)
@@ -144,6 +145,7 @@ _SIGS = {
     "oaph_arnoldi_set_output_type": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_calc_triangular_h_type": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_callback": (C.c_int, [C.c_void_p, MULTIPLY_FN, C.c_void_p]),
+    "oaph_arnoldi_set_sort_fn": (C.c_int, [C.c_void_p, SORT_FN, C.c_void_p]),
     "oaph_arnoldi_set_check_callback": (C.c_int, [C.c_void_p, CHECK_FN, C.c_void_p]),
     "oaph_arnoldi_set_matrix": (C.c_int, [C.c_void_p, MP]),
     "oaph_arnoldi_execute": (C.c_int, [C.c_void_p, U32, U32, C.POINTER(oaph_matrix_info), C.c_int]),

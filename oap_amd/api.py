@@ -540,7 +540,21 @@ class CuHArnoldiCallback:
         check(self._lib.oaph_arnoldi_set_vec_init_type(self._h, t), "setVecInitType")
 
     def setSortType(self, t):
-        check(self._lib.oaph_arnoldi_set_sort_type(self._h, t), "setSortType")
+        """a built-in order (ArnUtils.Sort*: applied on the device) or a
+        comparator before(i, j) -> bool over complex Ritz values (the
+        reference's ArnUtils::SortType, ArnoldiUtils.h:63-72: ordered on the
+        host at each restart)"""
+        if callable(t):
+            from ._lib import SORT_FN
+
+            def before(ri, ii, rj, ij, _user):
+                return 1 if t(complex(ri, ii), complex(rj, ij)) else 0
+
+            self._sort_cb = SORT_FN(before)  # kept alive with the solver
+            check(self._lib.oaph_arnoldi_set_sort_fn(self._h, self._sort_cb, None), "setSortType")
+        else:
+            check(self._lib.oaph_arnoldi_set_sort_type(self._h, t), "setSortType")
+            self._sort_cb = None
 
     def setCheckType(self, t):
         check(self._lib.oaph_arnoldi_set_check_type(self._h, t), "setCheckType")

@@ -19,7 +19,9 @@
 // then the solver restarts in FP64 from the sum of the wanted Ritz vectors
 // and polishes to the requested tolerance; every returned pair is checked
 // with an explicit FP64 residual.
+#include <algorithm>
 #include <chrono>
+#include <numeric>
 #include <vector>
 
 #include "iram.h"
@@ -63,6 +65,10 @@ struct oaph_arnoldi {
   void* muser = nullptr;
   oaph_check_fn cfn = nullptr;
   void* cuser = nullptr;
+  // a caller's ordering of Ritz values (ArnUtils::SortType as a function);
+  // NULL: the built-in order `sort`, applied on the device
+  oaph_sort_fn sort_fn = nullptr;
+  void* sort_user = nullptr;
   const oaph_matrix* A = nullptr;
   double* out_re = nullptr;
   double* out_im = nullptr;
@@ -72,7 +78,7 @@ struct oaph_arnoldi {
   uint32_t n = 0, m = 0, k = 0;
   uint32_t ky = 0;  // Ritz vectors kept in Y and X: k + guard, at most m
   bool cplx = true;
-  DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32;
+  DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32, TZ;
   size_t a32_elems = 0;
   const oaph_matrix* a32_of = nullptr;
   bool a32_exact = false;   // every operator entry is an FP32 value
@@ -108,8 +114,10 @@ oaph_status alloc(oaph_arnoldi* s) {
     OAPH_HIP(hipMemset(s->counter.ptr, 0, 256));
   }
   OAPH_TRY(s->status.reserve(sizeof(Status)));
+  if (s->sort_fn) OAPH_TRY(s->TZ.reserve(4 * m * m * sizeof(double)));
   return OAPH_OK;
 }
+
 
 // views of the solver's planes
 inline double* Vre(oaph_arnoldi* s, DevBuf& b, uint32_t c) {
@@ -261,6 +269,34 @@ bool wanted_set_changed(const oaph_arnoldi* s, const Watch& w) {
   return false;
 }
 
+// The restart with a caller's comparator: the first launch stops after the
+// Schur form, the Ritz values and their estimates; the host orders the
+// values with the comparator (a stable sort, wanted first, as the reference
+// sorts them with std::sort, ArnoldiProcedures.cpp:370-380) and hands the
+// order to the second launch, which goes on from the stored Schur form.
+oaph_status restart_with_comparator(oaph_arnoldi* s, RestartArgs ra) {
+  hipStream_t st = s->api->stream;
+  ra.TZ = s->TZ.as<double>();
+  ra.phase = 1;
+  OAPH_TRY(restart(st, ra));
+  OAPH_TRY(read_status(s));
+  const int bd = s->host_status.breakdown;
+  if (bd > 0 && ((uint32_t)bd < s->k || bd < 2)) return OAPH_OK;  // the kernel did nothing
+  const int mm = bd > 0 && (uint32_t)bd < s->m ? bd : (int)s->m;
+  const Status& hs = s->host_status;
+  std::vector<int> idx(mm);
+  std::iota(idx.begin(), idx.end(), 0);
+  std::stable_sort(idx.begin(), idx.end(), [&](int i, int j) {
+    return s->sort_fn(hs.ev_re[i], hs.ev_im[i], hs.ev_re[j], hs.ev_im[j], s->sort_user) != 0;
+  });
+  // the page-locked status buffer stays untouched until the next read
+  for (int t = 0; t < mm; ++t) s->pinned_status->order[t] = idx[t];
+  OAPH_HIP(hipMemcpyAsync(s->status.as<Status>()->order, s->pinned_status->order,
+                          mm * sizeof(int), hipMemcpyHostToDevice, st));
+  ra.phase = 2;
+  return restart(st, ra);
+}
+
 // Runs restart cycles from column `start` until convergence at `tol` (relative
 // to norm, or to max |Ritz value| when norm <= 0).  On return the status holds
 // the final ordering and Y the wanted Ritz vectors of H.  With `watch` (FP64
@@ -296,7 +332,10 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     // a breakdown seen during this factorisation limits the active block;
     // the restart kernel reads it from the status itself (no host round
     // trip before the launch)
-    OAPH_TRY(restart(st, ra));
+    if (s->sort_fn)
+      OAPH_TRY(restart_with_comparator(s, ra));
+    else
+      OAPH_TRY(restart(st, ra));
     OAPH_TRY(read_status(s));
     const int bd = s->host_status.breakdown;
     if (bd > 0 && ((uint32_t)bd < s->k || bd < 2)) {
@@ -491,7 +530,7 @@ oaph_status oaph_arnoldi_destroy(oaph_arnoldi* s) {
   if (!s) return OAPH_OK;
   if (s->api) (void)hipStreamSynchronize(s->api->stream);
   for (DevBuf* b : {&s->V, &s->V2, &s->W, &s->F, &s->H, &s->Q, &s->Y, &s->X, &s->AX, &s->coef,
-                    &s->partials, &s->counter, &s->status, &s->A32})
+                    &s->partials, &s->counter, &s->status, &s->A32, &s->TZ})
     b->release();
   if (s->pinned_status) (void)hipHostFree(s->pinned_status);
   oaph_api_destroy(s->api);
@@ -521,6 +560,13 @@ SETTER(max_restarts, max_restarts, uint32_t)
 oaph_status oaph_arnoldi_set_sort_type(oaph_arnoldi* s, int v) {
   OAPH_CHECK(s && v >= 0 && v <= 5, OAPH_ERR_ARG);
   s->sort = v;
+  s->sort_fn = nullptr;
+  return OAPH_OK;
+}
+oaph_status oaph_arnoldi_set_sort_fn(oaph_arnoldi* s, oaph_sort_fn fn, void* user) {
+  OAPH_CHECK(s, OAPH_ERR_ARG);
+  s->sort_fn = fn;
+  s->sort_user = user;
   return OAPH_OK;
 }
 oaph_status oaph_arnoldi_set_check_type(oaph_arnoldi* s, int v) {

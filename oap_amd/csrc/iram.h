@@ -53,6 +53,12 @@ struct RestartArgs {
   int do_shifts;
   double tol;
   double anorm;  // <= 0: relative to max |Ritz value|
+  // 0: the whole restart with the built-in order `which`; 1: Schur form,
+  // Ritz values and estimates only (status in natural order, T and Z stored
+  // to TZ); 2: T and Z loaded from TZ, the order taken from the status (set
+  // by the host's comparator), then the rest of the restart
+  int phase = 0;
+  double* TZ = nullptr;  // 4 m x m planes (T re, im, Z re, im), leading dimension m
 };
 
 // One Arnoldi step's CGS2 against V[:, 0..ncols): cgs_project (V^H x),

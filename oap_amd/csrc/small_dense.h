@@ -541,19 +541,33 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   __shared__ double tmax_s[1024];
 
   if (lane == 0) st->clk[0] = clock64();
-  for (int idx = lane; idx < mm * mm; idx += nthr) {
-    int r = idx / mm, c = idx - r * mm;
-    double hr = r <= c + 1 ? a.Hre[(size_t)r * a.m + c] : 0.0;
-    double hi = r <= c + 1 ? a.Him[(size_t)r * a.m + c] : 0.0;
-    T.set(r, c, {hr, hi});
-    Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
-  }
-  __syncthreads();
   __shared__ double log_c[kMaxM];
   __shared__ cd log_s[kMaxM];
   const RotLog log{log_c, log_s};
-  int fail = schur(T, Z, mm, log);
-  __syncthreads();
+  int fail = 0;
+  const size_t mm2 = (size_t)a.m * a.m;
+  if (a.phase == 2) {
+    // the Schur form of the first launch
+    for (int idx = lane; idx < mm * mm; idx += nthr) {
+      int r = idx / mm, c = idx - r * mm;
+      const size_t g = (size_t)r * a.m + c;
+      T.set(r, c, {a.TZ[g], a.TZ[mm2 + g]});
+      Z.set(r, c, {a.TZ[2 * mm2 + g], a.TZ[3 * mm2 + g]});
+    }
+    fail = st->schur_fail;
+    __syncthreads();
+  } else {
+    for (int idx = lane; idx < mm * mm; idx += nthr) {
+      int r = idx / mm, c = idx - r * mm;
+      double hr = r <= c + 1 ? a.Hre[(size_t)r * a.m + c] : 0.0;
+      double hi = r <= c + 1 ? a.Him[(size_t)r * a.m + c] : 0.0;
+      T.set(r, c, {hr, hi});
+      Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
+    }
+    __syncthreads();
+    fail = schur(T, Z, mm, log);
+    __syncthreads();
+  }
   if (lane == 0) st->clk[1] = clock64();
   for (int i = lane; i < mm; i += nthr) ev[i] = T.get(i, i);
   // ||T|| (max modulus of the upper triangle) for the trevc safeguard
@@ -610,15 +624,43 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   }
   __syncthreads();
   if (lane == 0) st->clk[2] = clock64();
-  // stable rank sort by the wanted order
-  for (int i = lane; i < mm; i += nthr) {
-    int rank = 0;
-    for (int j = 0; j < mm; ++j) {
-      bool bj = ritz_before(ev[j], ev[i], a.which);
-      bool bi = ritz_before(ev[i], ev[j], a.which);
-      if (bj || (!bi && j < i)) ++rank;
+  if (a.phase == 1) {
+    // values, estimates and the Schur form for the host's comparator
+    if (lane == 0) {
+      double maxabs = 0;
+      for (int i = 0; i < mm; ++i) maxabs = fmax(maxabs, sqrt(cabs2(ev[i])));
+      st->maxabs = maxabs;
+      st->schur_fail = fail;
     }
-    order[rank] = i;
+    for (int i = lane; i < mm; i += nthr) {
+      st->ev_re[i] = ev[i].re;
+      st->ev_im[i] = ev[i].im;
+      st->est[i] = est[i];
+      st->order[i] = i;
+    }
+    for (int idx = lane; idx < mm * mm; idx += nthr) {
+      int r = idx / mm, c = idx - r * mm;
+      const size_t g = (size_t)r * a.m + c;
+      a.TZ[g] = T.re[r * ld + c];
+      a.TZ[mm2 + g] = T.im[r * ld + c];
+      a.TZ[2 * mm2 + g] = Z.re[r * ld + c];
+      a.TZ[3 * mm2 + g] = Z.im[r * ld + c];
+    }
+    return;  // uniform: every thread returns
+  }
+  if (a.phase == 2) {
+    for (int i = lane; i < mm; i += nthr) order[i] = st->order[i];
+  } else {
+    // stable rank sort by the wanted order
+    for (int i = lane; i < mm; i += nthr) {
+      int rank = 0;
+      for (int j = 0; j < mm; ++j) {
+        bool bj = ritz_before(ev[j], ev[i], a.which);
+        bool bi = ritz_before(ev[i], ev[j], a.which);
+        if (bj || (!bi && j < i)) ++rank;
+      }
+      order[rank] = i;
+    }
   }
   __syncthreads();
   if (lane == 0) {

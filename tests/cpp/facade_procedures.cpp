@@ -5,6 +5,7 @@
 // (columns, rows) forms, addSubstract, magnitude2, calcTriangularHStep,
 // SetMatrix, the dims copies and the value accessors.  Fixed small inputs;
 // the results are printed as one JSON line for the test to check.
+#include <cmath>
 #include <cstdio>
 #include <string>
 #include <vector>
@@ -25,6 +26,12 @@ static std::vector<double> re_of(const math::Matrix* m) {
   std::vector<double> v((size_t)m->columns * m->rows);
   oap::cuda::CopyDeviceMatrixToHostArrays(v.data(), nullptr, m);
   return v;
+}
+
+// a comparator with no built-in counterpart: largest real part of
+// lambda e^{-i pi/4}
+static bool byRotatedRe(const EigenPair& i, const EigenPair& j) {
+  return (i.re() + i.im()) > (j.re() + j.im());
 }
 
 int main() {
@@ -87,10 +94,33 @@ int main() {
     oap::cuda::SetZeroRow(dM, 1);
     const std::vector<double> mset = re_of(dM);
 
-    std::printf("{\"transpose_ex\": %s, \"periodic\": %s, \"addsub\": %s, \"magnitude2\": %.17g, "
+    // CuHArnoldiDefault with the caller's comparator on a 64 x 64 complex
+    // operator built from a formula the test repeats
+    const unsigned n = 64;
+    std::vector<double> hre(n * n), him(n * n);
+    for (unsigned r = 0; r < n; ++r)
+      for (unsigned c = 0; c < n; ++c) {
+        const double off = 0.01 * std::sin(7.0 * r + 3.0 * c);
+        hre[r * n + c] = r == c ? std::cos(0.1 * r) / (1.0 + 0.05 * r) : off;
+        him[r * n + c] = r == c ? std::sin(0.1 * r) / (1.0 + 0.05 * r) : 0.5 * off;
+      }
+    math::Matrix* dOp = oap::cuda::NewDeviceMatrix(true, true, n, n);
+    oap::cuda::CopyHostArraysToDeviceMatrix(dOp, hre.data(), him.data());
+    std::vector<double> ere(3), eim(3);
+    CuHArnoldiDefault arnoldi;
+    arnoldi.setMatrix(dOp);
+    arnoldi.setSortType(byRotatedRe);
+    arnoldi.setCheckTolerance(1e-10);
+    arnoldi.setOutputType(ArnUtils::DEVICE);
+    arnoldi.setOutputsEigenvalues(ere.data(), eim.data());
+    arnoldi.execute(24, 3, math::MatrixInfo(true, true, n, n));
+    std::vector<double> eig = {ere[0], eim[0], ere[1], eim[1], ere[2], eim[2]};
+    oap::cuda::DeleteDeviceMatrix(dOp);
+
+    std::printf("{\"eig_rotated\": %s, \"transpose_ex\": %s, \"periodic\": %s, \"addsub\": %s, \"magnitude2\": %.17g, "
                 "\"magnitude\": %.17g, \"h1\": %s, \"q1\": %s, \"value21\": %.17g, "
                 "\"dims_host\": %s, \"set_matrix\": %s}\n",
-                arr(tex).c_str(), arr(per).c_str(), arr(addsub).c_str(), m2, m1, arr(h1).c_str(),
+                arr(eig).c_str(), arr(tex).c_str(), arr(per).c_str(), arr(addsub).c_str(), m2, m1, arr(h1).c_str(),
                 arr(q1).c_str(), v21, arr(host).c_str(), arr(mset).c_str());
     for (math::Matrix* m : {din, dout, dA, dB, dC, dO, dH, dQ, dM}) oap::cuda::DeleteDeviceMatrix(m);
   } catch (const oap::Error& e) {

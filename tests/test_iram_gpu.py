@@ -407,3 +407,35 @@ def test_cpp_facade_example(oracle):
     want = (np.array(fix["re"]) + 1j * np.array(fix["im"]))[:4]
     match(ev, want, 1e-10)
     assert max(res["resid"]) <= 1e-10 * fix["fro"]
+
+
+def test_custom_comparator(oracle):
+    # ArnUtils::SortType as a caller's function: the Ritz values are ordered
+    # on the host each restart.  A comparator equal to a built-in order gives
+    # the built-in result; one with no built-in counterpart (largest real
+    # part of lambda e^{-i pi/4}, an extreme direction of the spectrum) gives
+    # the dense top 4 of that order.
+    n = 600
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    dA = DeviceMatrix.from_numpy(A.real, A.imag)
+    ev = np.linalg.eigvals(A)
+    for key, before in ((lambda z: -abs(z), lambda a, b: abs(a) > abs(b)),
+                        (lambda z: -(z * np.exp(-1j * np.pi / 4)).real,
+                         lambda a, b: (a * np.exp(-1j * np.pi / 4)).real
+                         > (b * np.exp(-1j * np.pi / 4)).real)):
+        s = solver()
+        s.setMatrix(dA)
+        s.setSortType(before)
+        re, im = np.zeros(4), np.zeros(4)
+        s.setOutputsEigenvalues(re, im)
+        vecs = [DeviceMatrix(1, n) for _ in range(4)]
+        s.setOutputsEigenvectors(vecs)
+        s.execute(32, 4, MatrixInfo(True, True, n, n))
+        got = re + 1j * im
+        match(got, sorted(ev, key=key)[:4], 1e-10)
+        check_residuals(s, A, got, vecs, 4)
+    # back to a built-in order
+    s.setSortType(ArnUtils.SortLargestValues)
+    s.execute(32, 4, MatrixInfo(True, True, n, n))
+    match(re + 1j * im, top_k(ev, 4, ArnUtils.SortLargestValues), 1e-10)

@@ -282,6 +282,18 @@ def test_cpp_facade_procedures():
     out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stderr
     r = json.loads(out.stdout)
+    # CuHArnoldiDefault with the comparator byRotatedRe (largest re + im):
+    # the dense top 3 of that order
+    n = 64
+    rr, cc = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    off = 0.01 * np.sin(7.0 * rr + 3.0 * cc)
+    d = 1.0 / (1.0 + 0.05 * np.arange(n))
+    A = off + 0.5j * off
+    A[np.arange(n), np.arange(n)] = (np.cos(0.1 * np.arange(n)) + 1j * np.sin(0.1 * np.arange(n))) * d
+    want = sorted(np.linalg.eigvals(A), key=lambda z: -(z.real + z.imag))[:3]
+    got = np.array(r["eig_rotated"][0::2]) + 1j * np.array(r["eig_rotated"][1::2])
+    for w in want:
+        assert np.min(np.abs(got - w)) <= 1e-10, (got, want)
     x = np.array([(i % 7) - 2.0 for i in range(40)]).reshape(10, 4)
     tex = np.zeros((4, 10))
     tex[:2] = x[:, :2].T
