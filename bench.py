@@ -310,16 +310,17 @@ def learned_c4(count: int) -> dict:
     cond = np.array([learned.condition_estimate(p, a) for p, _, a in rng_params])
     parts = {"well_conditioned": rng_params[cond <= learned.WELL_CONDITIONED][:count],
              "ill_conditioned": rng_params[cond > learned.WELL_CONDITIONED][:max(4, count // 2)]}
-    learned.compare(model, parts["well_conditioned"][:1], mixed=True)  # warm-up
+    learned.compare(model, parts["well_conditioned"][:1], mixed=True, shifts=True)  # warm-up
     out = {"counted": "pairs with residual <= 1e-10 ||A||_F and eigenvalue within "
                       "1e-6 max|lambda| of the closed-form spectrum",
            "condition_bound": learned.WELL_CONDITIONED}
     for part, params in parts.items():
-        res = learned.compare(model, params, mixed=True)
+        res = learned.compare(model, params, mixed=True, shifts=True)
         rep = {"operators": int(len(params)),
                "pe_range": [round(float(params[:, 0].min()), 2), round(float(params[:, 0].max()), 2)]}
         for key, idx in (("random_start", 0), ("learned_start", 1),
-                         ("learned_start_fp32_operand", 2)):
+                         ("learned_start_fp32_operand", 2),
+                         ("learned_start_and_shifts_fp32_operand", 3)):
             secs, good, errmax, mvs, fbs = 0.0, 0, 0.0, [], 0
             for (pe, c, a), row in zip(params, res):
                 sol = row[idx]

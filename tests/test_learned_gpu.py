@@ -121,3 +121,43 @@ def test_wanted_set_closed_form(pe, c, alpha):
         # wanted set is still unambiguous (neighbours >= 8e-4 apart)
         assert np.max(np.abs(got.imag)) <= 1e-5 * np.max(np.abs(want))
         assert np.max(np.abs(got.real - want)) <= 1e-5 * np.max(np.abs(want)), (got, want)
+
+
+def test_shift_filter_closed_form():
+    # pe = 0 (symmetric): the eigenvectors are products of sines with known
+    # eigenvalues; the filter multiplies each by p(lambda) = T_d((lambda -
+    # c) / e) / T_d((a0 - c) / e), c and e the centre and half-width of [lo, hi]
+    prob = learned.CdrProblem()
+    pe, c0, a = 0.0, 2.0, 0.8
+    prob.set_params(pe, c0, a)
+    G, h = learned.GRID, 1.0 / (learned.GRID + 1)
+    ix = np.arange(G)
+
+    def vec(j, k):  # index ix + G iy
+        return (np.sin(k * np.pi * h * (ix + 1))[:, None] * np.sin(j * np.pi * h * (ix + 1))[None, :]).ravel()
+
+    # the operator's FP32-rounded coefficients: diagonal d, x neighbours
+    # bx, y neighbours by; (T x)_i = b x_{i-1} + d x_i + b x_{i+1} on the
+    # sine vector of mode q gives d + 2 b cos(q pi h)
+    ih2 = 1.0 / (h * h)
+    d0 = float(np.float32(2.0 * a * ih2 + 2.0 * ih2 + c0))
+    bx, by = float(np.float32(-a * ih2)), float(np.float32(-ih2))
+
+    def lam(j, k):
+        return d0 + 2 * bx * np.cos(j * np.pi * h) + 2 * by * np.cos(k * np.pi * h)
+
+    modes = [(1, 1), (2, 1), (5, 3), (20, 30)]
+    v0 = sum(vec(j, k) / np.linalg.norm(vec(j, k)) for j, k in modes)
+    lo, hi, a0, d = 60.0, learned.gershgorin_bound(pe, c0, a), lam(1, 1), 12
+    out = prob.filter_start(v0, lo, hi, a0, d)
+    cc, e = (hi + lo) / 2, (hi - lo) / 2
+
+    def T(x):
+        x = np.asarray(x, dtype=np.float64)
+        return np.where(np.abs(x) <= 1, np.cos(d * np.arccos(np.clip(x, -1, 1))),
+                        np.sign(x) ** d * np.cosh(d * np.arccosh(np.abs(x))))
+
+    want = sum(T((lam(j, k) - cc) / e) / T((a0 - cc) / e) * vec(j, k) / np.linalg.norm(vec(j, k))
+               for j, k in modes)
+    want /= np.linalg.norm(want)
+    assert np.max(np.abs(out - want)) <= 1e-9
