@@ -433,6 +433,17 @@ def main() -> int:
     check(lib.oaph_arnoldi_time_matvec(smx._h, 1, 20, C.byref(ms)), "time_matvec")
     t32 = ms.value
 
+    # config C2's BF16-split variant: the first Gram-Schmidt pass on the
+    # matrix cores from a BF16 hi + lo copy of the basis (FP32 accumulation),
+    # on the same start vectors
+    from oap_amd.api import Projection
+    sbf, rebf, imbf = make_solver(dA, Precision.MIXED_FP32)
+    sbf.setProjection(Projection.BF16X3)
+    log("mixed FP32/FP64 solver, BF16-split projection:")
+    goodbf, wallbf, stbf, medbf, missbf, errbf = solve_series(sbf, rebf, imbf, args.steps,
+                                                              args.warmup, fro, want)
+    del sbf
+
     # operator, x and y, and the m basis vectors the kernel's projection
     # epilogue reads (the first Gram-Schmidt pass of the Arnoldi step)
     bytes64 = 16 * N * N + 32 * N + 16 * N * M
@@ -488,6 +499,14 @@ def main() -> int:
                        "median_per_run": round(med64, 4), "matvecs_median": matvecs64,
                        "set_misses": miss64, "max_eigenvalue_err_vs_oracle": err64,
                        "ratio_mixed_over_fp64": round(epsmx / eps64, 3) if eps64 > 0 else None},
+        "mixed_bf16x3_projection": {
+            "value": round(goodbf / wallbf, 4), "ms_per_step": round(wallbf / args.steps * 1e3, 3),
+            "median_per_run": round(medbf, 4), "set_misses": missbf,
+            "max_eigenvalue_err_vs_oracle": errbf,
+            "matvecs_median": {"fp32": int(np.median([x["matvecs_low"] for x in stbf])),
+                               "fp64": int(np.median([x["matvecs"] - x["matvecs_low"] for x in stbf]))},
+            "note": "first Gram-Schmidt pass V^H w on v_mfma_f32_16x16x16_bf16 from a BF16 hi+lo "
+                    "basis (hi.hi + hi.lo + lo.hi, FP32 accumulation), second pass FP64"},
         "standalone_c1": standalone,
         "batched_c3": batched,
         "learned_c4": c4,

@@ -551,6 +551,7 @@ class CuHArnoldi {
   void setNormA(floatt n) { oap::check(oaph_arnoldi_set_norm_a(m_a, n), "setNormA"); }
   void setMaxRestarts(uintt n) { oap::check(oaph_arnoldi_set_max_restarts(m_a, n), "setMaxRestarts"); }
   void setPrecision(int p) { oap::check(oaph_arnoldi_set_precision(m_a, p), "setPrecision"); }
+  void setProjection(int mode) { oap::check(oaph_arnoldi_set_projection(m_a, mode), "setProjection"); }
   void setMixedTolerance(double t) {
     oap::check(oaph_arnoldi_set_mixed_tolerance(m_a, t), "setMixedTolerance");
   }

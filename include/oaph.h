@@ -350,6 +350,10 @@ typedef struct oaph_arnoldi oaph_arnoldi;
 /* matvec precision (extension, no reference counterpart) */
 #define OAPH_PREC_FP64 0
 #define OAPH_PREC_MIXED_FP32 1 /* FP32 matvec bulk, FP64 polish */
+/* first Gram-Schmidt pass of each Arnoldi step (extension) */
+#define OAPH_PROJ_FP64 0   /* FP64, in the dense operator's matvec epilogue */
+#define OAPH_PROJ_BF16X3 1 /* BF16-split (hi + lo) basis, hi.hi + hi.lo + lo.hi on
+                              the matrix cores with FP32 accumulation */
 
 /* CuHArnoldiCallbackBase::MultiplyFunc (ArnoldiProceduresImpl.h:51):
  * w = A v.  w and v are 1-column device matrices owned by the solver; the
@@ -419,6 +423,14 @@ oaph_status oaph_arnoldi_set_max_restarts(oaph_arnoldi* a, uint32_t max_restarts
  * an entry beyond the FP32 range, has no usable FP32 copy: the solve then
  * runs FP64 cycles only (stats.mixed_declined = 1). */
 oaph_status oaph_arnoldi_set_precision(oaph_arnoldi* a, int precision);
+/* The first Gram-Schmidt pass V^H w of every Arnoldi step: OAPH_PROJ_FP64
+ * (default) or OAPH_PROJ_BF16X3 -- a BF16 hi + lo copy of the basis (about
+ * 16 significant bits, kept current column by column) and
+ * v_mfma_f32_16x16x16_bf16 products hi.hi + hi.lo + lo.hi with FP32
+ * accumulation per block; the second pass (DGKS re-orthogonalisation), H,
+ * the restart and the residuals stay FP64.  Used for n % 64 == 0 and
+ * Krylov dimensions up to 64, else FP64.  No reference counterpart. */
+oaph_status oaph_arnoldi_set_projection(oaph_arnoldi* a, int mode);
 /* Staged solve (opt-in, default none): at each tolerance tols[0] > tols[1]
  * > ... (relative to the largest |Ritz value|) the cycles stop and the solve
  * restarts from the sum of the wanted (and guard) Ritz vectors (in the FP32

@@ -154,6 +154,7 @@ _SIGS = {
     "oaph_arnoldi_set_norm_a": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_max_restarts": (C.c_int, [C.c_void_p, U32]),
     "oaph_arnoldi_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "oaph_arnoldi_set_projection": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_mixed_tolerance": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_restart_guard_weight": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_start_vector": (C.c_int, [C.c_void_p, DP, DP, U32]),

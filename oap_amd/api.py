@@ -484,6 +484,11 @@ class Precision:
     FP64, MIXED_FP32 = 0, 1
 
 
+class Projection:
+    """first Gram-Schmidt pass of each Arnoldi step (OAPH_PROJ_*)"""
+    FP64, BF16X3 = 0, 1
+
+
 class MatrixInfo:
     """math::MatrixInfo (oapMatrix/MatrixInfo.h:16-67)."""
 
@@ -584,6 +589,11 @@ class CuHArnoldiCallback:
 
     def setPrecision(self, p):
         check(self._lib.oaph_arnoldi_set_precision(self._h, p), "setPrecision")
+
+    def setProjection(self, mode):
+        """Projection.FP64 (default) or Projection.BF16X3: the first
+        Gram-Schmidt pass on the matrix cores from a BF16-split basis"""
+        check(self._lib.oaph_arnoldi_set_projection(self._h, mode), "setProjection")
 
     def setRestartStages(self, tols):
         """restart from the sum of the wanted (and guard) Ritz vectors at each

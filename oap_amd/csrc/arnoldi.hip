@@ -78,7 +78,11 @@ struct oaph_arnoldi {
   uint32_t n = 0, m = 0, k = 0;
   uint32_t ky = 0;  // Ritz vectors kept in Y and X: k + guard, at most m
   bool cplx = true;
-  DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32, TZ;
+  DevBuf V, V2, W, F, H, Q, Y, X, AX, coef, partials, counter, status, A32, TZ, Vb;
+  // first Gram-Schmidt pass on the matrix cores from a BF16-split copy of V
+  // (OAPH_PROJ_BF16X3); vb_valid leading columns of the copy are current
+  int projection = OAPH_PROJ_FP64;
+  uint32_t vb_valid = 0;
   size_t a32_elems = 0;
   const oaph_matrix* a32_of = nullptr;
   bool a32_exact = false;   // every operator entry is an FP32 value
@@ -115,6 +119,9 @@ oaph_status alloc(oaph_arnoldi* s) {
   }
   OAPH_TRY(s->status.reserve(sizeof(Status)));
   if (s->sort_fn) OAPH_TRY(s->TZ.reserve(4 * m * m * sizeof(double)));
+  if (s->projection == OAPH_PROJ_BF16X3)
+    OAPH_TRY(s->Vb.reserve(4 * (m + 1) * n * sizeof(uint16_t)));
+  s->vb_valid = 0;
   return OAPH_OK;
 }
 
@@ -219,6 +226,22 @@ oaph_status arnoldi_step(oaph_arnoldi* s, uint32_t j, bool low) {
   double2* c1 = s->coef.as<double2>();
   double2* c2 = c1 + (s->m + 1);
   const uint32_t nc = j + 1;
+  if (s->projection == OAPH_PROJ_BF16X3 && proj_bf16x3_supported(s->n, nc)) {
+    // the matrix-core first pass from the BF16-split basis (columns the copy
+    // lacks converted first)
+    if (s->vb_valid < nc) {
+      OAPH_TRY(v_to_bf16x2(st, s->n, s->m, s->vb_valid, nc, Vre(s, s->V, 0), Vim(s, s->V, 0),
+                           s->Vb.as<uint16_t>()));
+      s->vb_valid = nc;
+    }
+    OAPH_TRY(apply_operator(s, Vre(s, s->V, j), Vim(s, s->V, j), wre, wim, OAPH_TYPE_WV, low));
+    uint32_t parts = 0;
+    OAPH_TRY(cgs_project_bf16x3(st, w, nc, wre, wim, s->Vb.as<uint16_t>(), &parts));
+    OAPH_TRY(cgs_update_project(st, w, nc, wre, wim, fre, fim, c1, parts));
+    const bool next = j + 1 < s->m;
+    return cgs_finish(st, w, nc, fre, fim, c1, c2, (int)j, (int)j + 1,
+                      next ? Vre(s, s->V, j + 1) : nullptr, next ? Vim(s, s->V, j + 1) : nullptr);
+  }
   // the built-in operator's matvec also forms the first pass's partial sums
   MvProj pj = cgs_matvec_projection(w, nc);
   OAPH_TRY(apply_operator(s, Vre(s, s->V, j), Vim(s, s->V, j), wre, wim, OAPH_TYPE_WV, low, &pj));
@@ -240,6 +263,7 @@ oaph_status fresh_column(oaph_arnoldi* s, uint32_t nc, uint64_t seed) {
   double2* c1 = s->coef.as<double2>();
   double2* c2 = c1 + (s->m + 1);
   OAPH_TRY(random_vector(st, s->n, seed, true, wre, wim));
+  if (s->vb_valid > nc) s->vb_valid = nc;
   // H[nc][nc-1] stays 0 (hcol = -1: no H writes, no breakdown flag)
   OAPH_TRY(cgs_project(st, w, nc, wre, wim));
   OAPH_TRY(cgs_update_project(st, w, nc, wre, wim, fre, fim, c1));
@@ -386,6 +410,7 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     OAPH_TRY(vq(st, s->n, s->m, kk + 1, Vre(s, s->V, 0), Vim(s, s->V, 0), ra.Qre, ra.Qim, s->m,
                 Vre(s, s->V2, 0), Vim(s, s->V2, 0), (int)kk, fre, fim, ra.st));
     std::swap(s->V, s->V2);
+    s->vb_valid = 0;
     Work w = work_of(s);
     OAPH_TRY(residual(st, w, 0, fre, fim, nullptr, (int)kk - 1, (int)kk));
     OAPH_TRY(normalize(st, w, fre, fim, Vre(s, s->V, kk), Vim(s, s->V, kk)));
@@ -434,6 +459,7 @@ oaph_status start_vector(oaph_arnoldi* s) {
 // normalise V[:, 0] in place
 oaph_status normalize_v0(oaph_arnoldi* s) {
   hipStream_t st = s->api->stream;
+  s->vb_valid = 0;
   Work w = work_of(s);
   double* fre = s->F.as<double>();
   double* fim = fre + s->n;
@@ -530,7 +556,7 @@ oaph_status oaph_arnoldi_destroy(oaph_arnoldi* s) {
   if (!s) return OAPH_OK;
   if (s->api) (void)hipStreamSynchronize(s->api->stream);
   for (DevBuf* b : {&s->V, &s->V2, &s->W, &s->F, &s->H, &s->Q, &s->Y, &s->X, &s->AX, &s->coef,
-                    &s->partials, &s->counter, &s->status, &s->A32, &s->TZ})
+                    &s->partials, &s->counter, &s->status, &s->A32, &s->TZ, &s->Vb})
     b->release();
   if (s->pinned_status) (void)hipHostFree(s->pinned_status);
   oaph_api_destroy(s->api);
@@ -610,6 +636,12 @@ oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* s, double w) {
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* s, double tol_low) {
   OAPH_CHECK(s && tol_low > 0 && tol_low < 1, OAPH_ERR_ARG);
   s->tol_low = tol_low;
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_set_projection(oaph_arnoldi* s, int mode) {
+  OAPH_CHECK(s && (mode == OAPH_PROJ_FP64 || mode == OAPH_PROJ_BF16X3), OAPH_ERR_ARG);
+  s->projection = mode;
   return OAPH_OK;
 }
 

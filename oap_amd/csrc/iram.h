@@ -85,6 +85,15 @@ size_t orth_partials_bytes(uint32_t n);
 oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const double* fim,
                       double* vre, double* vim);
 oaph_status restart(hipStream_t s, const RestartArgs& a);
+// The BF16-split basis copy Vb (4 planes of (m + 1) x n: re hi, re lo, im hi,
+// im lo) of columns [c0, c1), and the first Gram-Schmidt pass on the matrix
+// cores from it (partial sums into the P1 area that cgs_update_project sums;
+// *parts = their count).  Supported for n % 64 == 0 and ncols <= 64.
+oaph_status v_to_bf16x2(hipStream_t s, uint32_t n, uint32_t m, uint32_t c0, uint32_t c1,
+                       const double* Vre, const double* Vim, uint16_t* Vb);
+bool proj_bf16x3_supported(uint32_t n, uint32_t ncols);
+oaph_status cgs_project_bf16x3(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
+                               const double* xim, const uint16_t* Vb, uint32_t* parts);
 oaph_status vq(hipStream_t s, uint32_t n, uint32_t mrows, uint32_t ncols, const double* Vre,
                const double* Vim, const double* Mre, const double* Mim, uint32_t ldm,
                double* Ore, double* Oim, int fcol, double* fre, double* fim, const Status* st);

@@ -321,6 +321,111 @@ __global__ __launch_bounds__(kOrthThreads) void k_cgs(CgsArgs a) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// BF16-split projection (first Gram-Schmidt pass on the matrix cores).
+// Each value v of the basis is kept as two BF16 numbers, hi = bf16(v) and
+// lo = bf16(v - hi) (v to about 16 significant bits); V^H w is formed as
+// hi.hi + hi.lo + lo.hi on v_mfma_f32_16x16x16_bf16 with FP32 accumulation
+// per block and the per-block partial sums in FP64, the second pass in FP64.
+
+__device__ __forceinline__ uint16_t bf16_rne_bits(float f) {
+  uint32_t u = __float_as_uint(f);
+  if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)((u >> 16) | ((u & 0xffffu) ? 0x40u : 0u));
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf16_value(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+__device__ __forceinline__ void bf16_split(double v, uint16_t& hi, uint16_t& lo) {
+  hi = bf16_rne_bits((float)v);
+  lo = bf16_rne_bits((float)(v - (double)bf16_value(hi)));
+}
+
+// Vb planes (each (m + 1) x n, column c at c * n): re hi, re lo, im hi, im lo
+__global__ void k_v_to_bf16x2(uint32_t n, uint32_t c0, uint32_t c1, size_t plane,
+                              const double* __restrict__ Vre, const double* __restrict__ Vim,
+                              uint16_t* __restrict__ Vb) {
+  const size_t total = (size_t)(c1 - c0) * n;
+  for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < total;
+       t += (size_t)gridDim.x * blockDim.x) {
+    const size_t i = (size_t)c0 * n + t;
+    uint16_t h, l;
+    bf16_split(Vre[i], h, l);
+    Vb[i] = h;
+    Vb[plane + i] = l;
+    bf16_split(Vim[i], h, l);
+    Vb[2 * plane + i] = h;
+    Vb[3 * plane + i] = l;
+  }
+}
+
+typedef short bs4 __attribute__((ext_vector_type(4)));
+typedef float bf4 __attribute__((ext_vector_type(4)));
+
+// part[b][c] = sum over rows r of block b of conj(V[c][r]) w[r], c < ncols <= 64,
+// as one real product over the rows of [Vre | Vim] with the two right-hand
+// columns [wre; wim] (re) and [wim; -wre] (im).  Wave q owns basis columns
+// 16 q .. 16 q + 15 (the A rows of its 16 x 16 tiles); MFMA operand layout:
+// lane l holds A(l % 16, 4 (l / 16) .. + 3) and B(4 (l / 16) .. + 3, l % 16),
+// and C(4 (l / 16) + i, l % 16) for i < 4.  n and rpb are multiples of 64.
+__global__ __launch_bounds__(256) void k_proj_bf16x3(uint32_t n, uint32_t ncols, uint32_t rpb,
+                                                     size_t plane, const uint16_t* __restrict__ Vb,
+                                                     const double* __restrict__ wre,
+                                                     const double* __restrict__ wim,
+                                                     double2* __restrict__ part) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t c = 16 * wave + (lane & 15);  // basis column of this lane's A row
+  const int j = lane & 15;                     // B column of this lane
+  const uint32_t kq = 4 * (lane >> 4);
+  const uint32_t r0 = blockIdx.x * rpb;
+  const uint32_t r1 = r0 + rpb < n ? r0 + rpb : n;
+  bf4 acc = {0.f, 0.f, 0.f, 0.f};
+  for (uint32_t k = r0; k < r1; k += 16) {
+    const uint32_t r = k + kq;
+    // w rows r .. r + 3, split
+    double xr[4], xi[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      xr[e] = wre[r + e];
+      xi[e] = wim[r + e];
+    }
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {
+      // A = Vre rows (half 0) or Vim rows (half 1) of column c
+      bs4 ahi = {0, 0, 0, 0}, alo = {0, 0, 0, 0};
+      if (c < ncols) {
+        const size_t base = (size_t)(2 * half) * plane + (size_t)c * n + r;
+        const uint2 h = *reinterpret_cast<const uint2*>(Vb + base);
+        const uint2 l = *reinterpret_cast<const uint2*>(Vb + plane + base);
+        ahi = bs4{(short)(h.x & 0xffff), (short)(h.x >> 16), (short)(h.y & 0xffff), (short)(h.y >> 16)};
+        alo = bs4{(short)(l.x & 0xffff), (short)(l.x >> 16), (short)(l.y & 0xffff), (short)(l.y >> 16)};
+      }
+      // B column 0: re part of conj(v) w, column 1: im part
+      bs4 bhi = {0, 0, 0, 0}, blo = {0, 0, 0, 0};
+      if (j < 2) {
+        uint16_t hh[4], ll[4];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const double v = half == 0 ? (j == 0 ? xr[e] : xi[e]) : (j == 0 ? xi[e] : -xr[e]);
+          bf16_split(v, hh[e], ll[e]);
+        }
+        bhi = bs4{(short)hh[0], (short)hh[1], (short)hh[2], (short)hh[3]};
+        blo = bs4{(short)ll[0], (short)ll[1], (short)ll[2], (short)ll[3]};
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(alo, bhi, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, blo, acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ahi, bhi, acc, 0, 0, 0);
+    }
+  }
+  // lane l: rows 4 (l / 16) + i (basis columns 16 wave + ...), B column l % 16
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float re = acc[i];
+    const float im = __shfl_down(acc[i], 1, 64);  // the neighbour holds B column 1
+    const uint32_t cc = 16 * wave + 4 * (lane >> 4) + i;
+    if (j == 0 && cc < ncols) part[(size_t)blockIdx.x * kCgsStride + cc] = make_double2(re, im);
+  }
+}
+
 __global__ void k_normalize(uint32_t n, const double* fre, const double* fim, double* vre,
                             double* vim, const Status* st) {
   const double beta = st->beta;
@@ -626,6 +731,33 @@ oaph_status cgs_finish(hipStream_t s, const Work& w, uint32_t ncols, double* fre
   a.hrow = hrow;
   hipLaunchKernelGGL(k_cgs<2>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
   OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status v_to_bf16x2(hipStream_t s, uint32_t n, uint32_t m, uint32_t c0, uint32_t c1,
+                       const double* Vre, const double* Vim, uint16_t* Vb) {
+  if (c1 <= c0) return OAPH_OK;
+  const size_t total = (size_t)(c1 - c0) * n;
+  size_t blocks = (total + 255) / 256;
+  if (blocks > 4096) blocks = 4096;
+  hipLaunchKernelGGL(k_v_to_bf16x2, dim3((unsigned)blocks), dim3(256), 0, s, n, c0, c1,
+                     (size_t)(m + 1) * n, Vre, Vim, Vb);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+bool proj_bf16x3_supported(uint32_t n, uint32_t ncols) {
+  return n % 64 == 0 && ncols >= 1 && ncols <= 64;
+}
+
+oaph_status cgs_project_bf16x3(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
+                               const double* xim, const uint16_t* Vb, uint32_t* parts) {
+  OAPH_CHECK(proj_bf16x3_supported(w.n, ncols), OAPH_ERR_UNSUPPORTED);
+  const CgsArgs a = cgs_args(w, ncols);
+  hipLaunchKernelGGL(k_proj_bf16x3, dim3(a.nparts), dim3(256), 0, s, w.n, ncols, a.rpb,
+                     (size_t)(w.m + 1) * w.n, Vb, xre, xim, cgs_p1(w));
+  OAPH_HIP(hipGetLastError());
+  *parts = a.nparts;
   return OAPH_OK;
 }
 

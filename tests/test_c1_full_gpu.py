@@ -53,8 +53,10 @@ def test_device_matrix_is_the_fixture_matrix(oracle, c1):
     assert np.max(np.abs(re - ar)) <= 1e-14 and np.max(np.abs(im - ai)) <= 1e-14
 
 
-@pytest.mark.parametrize("precision", [Precision.FP64, Precision.MIXED_FP32])
-def test_c1_top8_against_fixture(c1, precision):
+@pytest.mark.parametrize("precision,projection", [(Precision.FP64, 0), (Precision.MIXED_FP32, 0),
+                                                  (Precision.MIXED_FP32, 1)],
+                         ids=["fp64", "mixed", "mixed_bf16x3_projection"])
+def test_c1_top8_against_fixture(c1, precision, projection):
     s = CuHArnoldiCallback()
     s.setQRType(QRType.QRGR)
     s.setSortType(ArnUtils.SortLargestValues)
@@ -62,6 +64,7 @@ def test_c1_top8_against_fixture(c1, precision):
     s.setOutputType(ArnUtils.DEVICE)
     s.setMaxRestarts(2000)
     s.setPrecision(precision)
+    s.setProjection(projection)  # 1: the first Gram-Schmidt pass from the BF16-split basis
     s.setMatrix(c1)
     re, im = np.zeros(K), np.zeros(K)
     s.setOutputsEigenvalues(re, im)

@@ -439,3 +439,29 @@ def test_custom_comparator(oracle):
     s.setSortType(ArnUtils.SortLargestValues)
     s.execute(32, 4, MatrixInfo(True, True, n, n))
     match(re + 1j * im, top_k(ev, 4, ArnUtils.SortLargestValues), 1e-10)
+
+
+@pytest.mark.parametrize("precision", [Precision.FP64, Precision.MIXED_FP32])
+def test_bf16x3_projection(oracle, precision):
+    # the first Gram-Schmidt pass on the matrix cores from the BF16-split
+    # basis (about 16 significant bits), the second pass in FP64: the same
+    # eigenpairs to 1e-10 as the dense oracle, residuals to 1e-10 ||A||_F
+    from oap_amd.api import Projection
+    n = 1024
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    dA = DeviceMatrix.from_numpy(A.real, A.imag)
+    s = solver()
+    s.setMatrix(dA)
+    s.setPrecision(precision)
+    s.setProjection(Projection.BF16X3)
+    re, im = np.zeros(8), np.zeros(8)
+    s.setOutputsEigenvalues(re, im)
+    vecs = [DeviceMatrix(1, n) for _ in range(8)]
+    s.setOutputsEigenvectors(vecs)
+    s.execute(48, 8, MatrixInfo(True, True, n, n))
+    ev = re + 1j * im
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+    with pytest.raises(OapError):
+        s.setProjection(7)
