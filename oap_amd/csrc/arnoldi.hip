@@ -118,7 +118,9 @@ oaph_status alloc(oaph_arnoldi* s) {
     OAPH_HIP(hipMemset(s->counter.ptr, 0, 256));
   }
   OAPH_TRY(s->status.reserve(sizeof(Status)));
-  if (s->sort_fn) OAPH_TRY(s->TZ.reserve(4 * m * m * sizeof(double)));
+  // T and Z between the comparator's two restart launches, and the general
+  // restart kernel's Z (and T where it does not fit in LDS)
+  if (s->sort_fn || m > (size_t)kMaxM) OAPH_TRY(s->TZ.reserve(4 * m * m * sizeof(double)));
   if (s->projection == OAPH_PROJ_BF16X3)
     OAPH_TRY(s->Vb.reserve(4 * (m + 1) * n * sizeof(uint16_t)));
   s->vb_valid = 0;
@@ -351,6 +353,7 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     ra.ky = (int)s->ky;
     ra.which = s->sort;
     ra.do_shifts = 1;
+    ra.TZ = s->TZ.as<double>();
     ra.tol = tol;
     ra.anorm = norm;
     // a breakdown seen during this factorisation limits the active block;
@@ -702,7 +705,7 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
   (void)type;
   // begin() preconditions (ArnoldiProcedures.cpp:146-152)
   OAPH_CHECK(wanted >= 1 && hdim > wanted, OAPH_ERR_ARG);
-  OAPH_CHECK(hdim <= (uint32_t)kMaxM, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(hdim <= (uint32_t)kMaxMBig, OAPH_ERR_UNSUPPORTED);
   OAPH_CHECK(info->rows >= hdim, OAPH_ERR_SHAPE);
   OAPH_CHECK(s->A || s->mfn, OAPH_ERR_STATE);
   if (s->A) OAPH_CHECK(s->A->rows == info->rows, OAPH_ERR_SHAPE);

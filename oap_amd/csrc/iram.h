@@ -6,7 +6,8 @@
 namespace oaph {
 namespace iram {
 
-constexpr int kMaxM = 64;  // Krylov dimension limit of the one-wave restart kernel
+constexpr int kMaxM = 64;      // Krylov dimension limit of the LDS-resident restart kernel
+constexpr int kMaxMBig = 256;  // Krylov dimension limit of the general restart kernel
 
 // Device-resident solver status; the host reads it once per restart cycle.
 struct Status {
@@ -18,8 +19,8 @@ struct Status {
   double beta_k_re, beta_k_im, sigma_re, sigma_im;
   double maxabs;   // max |Ritz value|
   double tolabs;   // absolute residual tolerance used for nconv
-  double ev_re[kMaxM], ev_im[kMaxM], est[kMaxM];  // Ritz values in wanted order, estimates
-  int order[kMaxM];
+  double ev_re[kMaxMBig], ev_im[kMaxMBig], est[kMaxMBig];  // Ritz values in wanted order, estimates
+  int order[kMaxMBig];
   long long clk[8];  // shader-clock stamps of the restart phases (profiling)
 };
 
@@ -59,6 +60,7 @@ struct RestartArgs {
   // by the host's comparator), then the rest of the restart
   int phase = 0;
   double* TZ = nullptr;  // 4 m x m planes (T re, im, Z re, im), leading dimension m
+                         // (always given when mact > kMaxM: the general kernel's T and Z)
 };
 
 // One Arnoldi step's CGS2 against V[:, 0..ncols): cgs_project (V^H x),

@@ -126,12 +126,13 @@ __global__ __launch_bounds__(kOrthThreads) void k_residual(
 // flagged or tiny either way.  Saves the norm launch that would otherwise
 // separate forming f from scaling it.
 constexpr int kCgsMaxBlocks = 256;
-constexpr int kCgsStride = kMaxM + 1;  // double2 entries per block in P1, P2
+constexpr int kCgsStride = kMaxMBig + 1;  // double2 entries per block in P1, P2
 
-// coef[c] = sum_b part[b][c] for c < ncols, into LDS; every thread calls.
+// coef[c] = sum_b part[b][c] for c < ncols (<= NC), into LDS; every thread calls.
+template <int NC>
 __device__ void sum_partials(const double2* __restrict__ part, uint32_t nparts, uint32_t ncols,
                              double2* coef) {
-  __shared__ double2 acc[4][kMaxM + 1];
+  __shared__ double2 acc[4][NC];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   for (uint32_t c = lane; c < ncols; c += 64) {
     double sr = 0.0, si = 0.0;
@@ -182,7 +183,7 @@ __global__ __launch_bounds__(kOrthThreads) void k_cgs(CgsArgs a) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const uint32_t ncols = a.ncols, n = a.n;
   if (MODE >= 1) {
-    sum_partials(a.part_in, a.nin, ncols, coef);
+    sum_partials<kMaxM + 1>(a.part_in, a.nin, ncols, coef);
     if (blockIdx.x == 0)
       for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) a.coef_out[c] = coef[c];
   }
@@ -321,6 +322,112 @@ __global__ __launch_bounds__(kOrthThreads) void k_cgs(CgsArgs a) {
   }
 }
 
+// The same three launches for more than kMaxM + 1 basis columns (Krylov
+// dimension above kMaxM): the columns are walked in loops instead of
+// per-column registers.  MODE >= 1: thread per row of the block forms
+// f = x - V coef; MODE <= 1: wave w forms the block's partial sums of
+// columns w, w + 4, ... (V^H x, or V^H f read back after the block barrier).
+template <int MODE>
+__global__ __launch_bounds__(kOrthThreads) void k_cgs_general(CgsArgs a) {
+  __shared__ double2 coef[kMaxMBig + 1];
+  __shared__ double inv_s;
+  __shared__ double nrm_s[kOrthThreads / 64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const uint32_t ncols = a.ncols, n = a.n;
+  const uint32_t r0 = blockIdx.x * a.rpb;
+  const uint32_t r1 = r0 + a.rpb < n ? r0 + a.rpb : n;
+  if (MODE >= 1) {
+    sum_partials<kMaxMBig + 1>(a.part_in, a.nin, ncols, coef);
+    if (blockIdx.x == 0)
+      for (uint32_t c = threadIdx.x; c < ncols; c += blockDim.x) a.coef_out[c] = coef[c];
+  }
+  if (MODE == 2) {
+    if (threadIdx.x < 64) {
+      double f1n = 0, ss = 0, hs = 0;
+      for (uint32_t b = lane; b < a.nparts; b += 64) f1n += a.nrm[b];
+      for (uint32_t c = lane; c < ncols; c += 64) {
+        const double2 s = coef[c];
+        ss += s.x * s.x + s.y * s.y;
+        if (a.hcol >= 0) {
+          const double2 h = make_double2(a.h[c].x + s.x, a.h[c].y + s.y);
+          hs += h.x * h.x + h.y * h.y;
+          if (blockIdx.x == 0) {
+            a.Hre[(size_t)c * a.m + a.hcol] = h.x;
+            a.Him[(size_t)c * a.m + a.hcol] = h.y;
+          }
+        }
+      }
+      f1n = wave_sum(f1n);
+      ss = wave_sum(ss);
+      hs = wave_sum(hs);
+      if (lane == 0) {
+        const double b2 = f1n - ss;
+        double beta = b2 > 0.0 ? sqrt(b2) : 0.0;
+        const bool broke = a.hcol >= 0 && ncols > 0 && beta <= 1e-13 * sqrt(hs + beta * beta);
+        if (broke) beta = 0.0;
+        inv_s = beta > 0.0 ? 1.0 / beta : 0.0;
+        if (blockIdx.x == 0) {
+          if (broke && a.st->breakdown == 0) a.st->breakdown = a.hrow;
+          a.st->beta = beta;
+          if (a.hcol >= 0 && a.hrow >= 0 && (uint32_t)a.hrow < a.m) {
+            a.Hre[(size_t)a.hrow * a.m + a.hcol] = beta;
+            a.Him[(size_t)a.hrow * a.m + a.hcol] = 0.0;
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+  double nrm = 0.0;
+  if (MODE >= 1) {
+    const double inv = MODE == 2 ? inv_s : 0.0;
+    for (uint32_t i = r0 + threadIdx.x; i < r1; i += blockDim.x) {
+      double pr = 0.0, pi = 0.0;
+      for (uint32_t c = 0; c < ncols; ++c) {
+        const double vr = a.Vre[(size_t)c * n + i], vi = a.Vim[(size_t)c * n + i];
+        const double2 s = coef[c];
+        pr += vr * s.x - vi * s.y;
+        pi += vr * s.y + vi * s.x;
+      }
+      const double xr = a.xre[i] - pr, xi = a.xim[i] - pi;
+      a.fre[i] = xr;
+      a.fim[i] = xi;
+      if (MODE == 2 && a.vre) {
+        a.vre[i] = xr * inv;
+        a.vim[i] = xi * inv;
+      }
+      nrm += xr * xr + xi * xi;
+    }
+  }
+  if (MODE <= 1) {
+    if (MODE == 1) __syncthreads();  // the block's rows of f are written
+    const double* xre = MODE == 0 ? a.xre : a.fre;
+    const double* xim = MODE == 0 ? a.xim : a.fim;
+    for (uint32_t c = wave; c < ncols; c += kOrthThreads / 64) {
+      double sr = 0.0, si = 0.0;
+      for (uint32_t i = r0 + lane; i < r1; i += 64) {
+        const double vr = a.Vre[(size_t)c * n + i], vi = a.Vim[(size_t)c * n + i];
+        const double xr = xre[i], xi = xim[i];
+        sr += vr * xr + vi * xi;  // conj(v) x
+        si += vr * xi - vi * xr;
+      }
+      sr = wave_sum(sr);
+      si = wave_sum(si);
+      if (lane == 0) a.part_out[(size_t)blockIdx.x * kCgsStride + c] = make_double2(sr, si);
+    }
+  }
+  if (MODE == 1) {
+    nrm = wave_sum(nrm);
+    if (lane == 0) nrm_s[wave] = nrm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double t = 0.0;
+      for (int w = 0; w < kOrthThreads / 64; ++w) t += nrm_s[w];
+      a.nrm[blockIdx.x] = t;
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // BF16-split projection (first Gram-Schmidt pass on the matrix cores).
 // Each value v of the basis is kept as two BF16 numbers, hi = bf16(v) and
@@ -443,10 +550,19 @@ __global__ __launch_bounds__(512) void k_restart(RestartArgs a) {
   restart_body(a, smem);
 }
 
+// Krylov dimension above kMaxM: T in LDS while it fits, Z (and T beyond
+// that) in the TZ work; single-shift sweeps by all threads.
+__global__ __launch_bounds__(kGeneralThreads) void k_restart_general(RestartArgs a) {
+  extern __shared__ double smem[];
+  restart_body_t<kMaxMBig, true>(a, smem);
+}
+
 // out[:, c] = sum_l V[:, l] M[l][c], c < ncols (out may not alias V).
 // When fcol >= 0, column fcol is not stored; instead
 // f <- (V M)[:, fcol] * beta_k + f * sigma (coefficients from the status).
 constexpr int kVqChunk = 4;
+constexpr size_t kVqLdsMax = 64 * 1024;  // M staged in LDS up to this size, else read from L2
+template <bool LDSM>
 __global__ __launch_bounds__(256) void k_vq(uint32_t n, uint32_t mrows, uint32_t ncols,
                                             const double* __restrict__ Vre,
                                             const double* __restrict__ Vim,
@@ -456,12 +572,14 @@ __global__ __launch_bounds__(256) void k_vq(uint32_t n, uint32_t mrows, uint32_t
                                             int fcol, double* fre, double* fim, const Status* st) {
   extern __shared__ double qs[];  // mrows x ncols complex: re block then im block
   const uint32_t total = mrows * ncols;
-  for (uint32_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
-    uint32_t l = idx / ncols, c = idx - l * ncols;
-    qs[idx] = Mre[(size_t)l * ldm + c];
-    qs[total + idx] = Mim[(size_t)l * ldm + c];
+  if (LDSM) {
+    for (uint32_t idx = threadIdx.x; idx < total; idx += blockDim.x) {
+      uint32_t l = idx / ncols, c = idx - l * ncols;
+      qs[idx] = Mre[(size_t)l * ldm + c];
+      qs[total + idx] = Mim[(size_t)l * ldm + c];
+    }
+    __syncthreads();
   }
-  __syncthreads();
   const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   {
@@ -476,7 +594,8 @@ __global__ __launch_bounds__(256) void k_vq(uint32_t n, uint32_t mrows, uint32_t
       for (int q = 0; q < kVqChunk; ++q) {
         uint32_t c = c0 + q;
         if (c < ncols) {
-          double mr = qs[l * ncols + c], mi = qs[total + l * ncols + c];
+          double mr = LDSM ? qs[l * ncols + c] : Mre[(size_t)l * ldm + c];
+          double mi = LDSM ? qs[total + l * ncols + c] : Mim[(size_t)l * ldm + c];
           ar[q] += vr * mr - vi * mi;
           ai[q] += vr * mi + vi * mr;
         }
@@ -670,12 +789,15 @@ size_t orth_partials_bytes(uint32_t n) {
 
 oaph_status cgs_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
                         const double* xim) {
-  OAPH_CHECK(ncols >= 1 && ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(ncols >= 1 && ncols <= kMaxMBig + 1, OAPH_ERR_UNSUPPORTED);
   CgsArgs a = cgs_args(w, ncols);
   a.xre = xre;
   a.xim = xim;
   a.part_out = cgs_p1(w);
-  hipLaunchKernelGGL(k_cgs<0>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  if (ncols > kMaxM + 1)
+    hipLaunchKernelGGL(k_cgs_general<0>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_cgs<0>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
@@ -695,7 +817,7 @@ MvProj cgs_matvec_projection(const Work& w, uint32_t ncols) {
 oaph_status cgs_update_project(hipStream_t s, const Work& w, uint32_t ncols, const double* xre,
                                const double* xim, double* fre, double* fim, double2* h,
                                uint32_t p1_parts) {
-  OAPH_CHECK(ncols >= 1 && ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(ncols >= 1 && ncols <= kMaxMBig + 1, OAPH_ERR_UNSUPPORTED);
   OAPH_CHECK(p1_parts <= kCgsMaxBlocks, OAPH_ERR_ARG);
   CgsArgs a = cgs_args(w, ncols);
   if (p1_parts) a.nin = p1_parts;
@@ -707,7 +829,10 @@ oaph_status cgs_update_project(hipStream_t s, const Work& w, uint32_t ncols, con
   a.part_out = cgs_p2(w);
   a.nrm = cgs_nrm(w);
   a.coef_out = h;
-  hipLaunchKernelGGL(k_cgs<1>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  if (ncols > kMaxM + 1)
+    hipLaunchKernelGGL(k_cgs_general<1>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_cgs<1>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
@@ -715,7 +840,7 @@ oaph_status cgs_update_project(hipStream_t s, const Work& w, uint32_t ncols, con
 oaph_status cgs_finish(hipStream_t s, const Work& w, uint32_t ncols, double* fre, double* fim,
                        const double2* h, double2* sc, int hcol, int hrow, double* vre,
                        double* vim) {
-  OAPH_CHECK(ncols >= 1 && ncols <= kMaxM + 1, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(ncols >= 1 && ncols <= kMaxMBig + 1, OAPH_ERR_UNSUPPORTED);
   CgsArgs a = cgs_args(w, ncols);
   a.xre = fre;
   a.xim = fim;
@@ -729,7 +854,10 @@ oaph_status cgs_finish(hipStream_t s, const Work& w, uint32_t ncols, double* fre
   a.vim = vim;
   a.hcol = hcol;
   a.hrow = hrow;
-  hipLaunchKernelGGL(k_cgs<2>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  if (ncols > kMaxM + 1)
+    hipLaunchKernelGGL(k_cgs_general<2>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_cgs<2>, dim3(a.nparts), dim3(kOrthThreads), 0, s, a);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
@@ -771,7 +899,20 @@ oaph_status normalize(hipStream_t s, const Work& w, const double* fre, const dou
 }
 
 oaph_status restart(hipStream_t s, const RestartArgs& a) {
-  OAPH_CHECK(a.mact >= 2 && a.mact <= kMaxM, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(a.mact >= 2 && a.mact <= kMaxMBig, OAPH_ERR_UNSUPPORTED);
+  if (a.mact > kMaxM) {
+    OAPH_CHECK(a.TZ != nullptr, OAPH_ERR_ARG);
+    static bool general_attr_set = false;
+    if (!general_attr_set) {
+      OAPH_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(k_restart_general),
+                                   hipFuncAttributeMaxDynamicSharedMemorySize, kGeneralLdsMax));
+      general_attr_set = true;
+    }
+    hipLaunchKernelGGL(k_restart_general, dim3(1), dim3(kGeneralThreads),
+                       (size_t)general_lds_bytes(a.mact), s, a);
+    OAPH_HIP(hipGetLastError());
+    return OAPH_OK;
+  }
   const size_t lds = (size_t)restart_lds_bytes(a.mact);
   static bool attr_set = false;
   if (!attr_set) {
@@ -789,12 +930,15 @@ oaph_status restart(hipStream_t s, const RestartArgs& a) {
 oaph_status vq(hipStream_t s, uint32_t n, uint32_t mrows, uint32_t ncols, const double* Vre,
                const double* Vim, const double* Mre, const double* Mim, uint32_t ldm,
                double* Ore, double* Oim, int fcol, double* fre, double* fim, const Status* st) {
-  size_t lds = (size_t)2 * mrows * ncols * sizeof(double);
-  OAPH_CHECK(lds <= 64 * 1024, OAPH_ERR_UNSUPPORTED);
+  const size_t lds = (size_t)2 * mrows * ncols * sizeof(double);
   // rows x column chunks: enough workgroups to spread over the CUs
   const dim3 grid((n + 63) / 64, (ncols + kVqChunk - 1) / kVqChunk);
-  hipLaunchKernelGGL(k_vq, grid, dim3(64), lds, s, n, mrows, ncols, Vre, Vim,
-                     Mre, Mim, ldm, Ore, Oim, fcol, fre, fim, st);
+  if (lds <= kVqLdsMax)
+    hipLaunchKernelGGL(k_vq<true>, grid, dim3(64), lds, s, n, mrows, ncols, Vre, Vim,
+                       Mre, Mim, ldm, Ore, Oim, fcol, fre, fim, st);
+  else
+    hipLaunchKernelGGL(k_vq<false>, grid, dim3(64), 0, s, n, mrows, ncols, Vre, Vim,
+                       Mre, Mim, ldm, Ore, Oim, fcol, fre, fim, st);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

@@ -481,6 +481,156 @@ __device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
 }
 
 __host__ __device__ constexpr int lds_ld(int m) { return m | 1; }
+
+// ---------------------------------------------------------------------------
+// General Krylov dimension (kMaxM < m <= kMaxMBig).  T lives in LDS while it
+// fits (general_t_in_lds), else in global memory like Z (the TZ work, L2
+// resident); one implicit single-shift sweep at a time by all threads.  Each
+// rotation is a row phase (threads over the columns of the row pair) and a
+// column phase (threads over the rows of T's column pair) between block
+// barriers; the threads that form the next rotation's (f, g) hand them over
+// in LDS, and the sweep's logged rotations are applied to Z after the sweep,
+// one row of Z per thread.  Same iteration as shifted_sweep in
+// oracle/oracle.c (single shifts, no bulge trains).
+constexpr int kGeneralThreads = 256;
+constexpr int kGeneralLdsMax = 128 * 1024;
+__host__ __device__ constexpr bool general_t_in_lds(int m) {
+  return 2 * m * lds_ld(m) * (int)sizeof(double) <= kGeneralLdsMax;
+}
+__host__ __device__ constexpr int general_lds_bytes(int m) {
+  return general_t_in_lds(m) ? 2 * m * lds_ld(m) * (int)sizeof(double) : 0;
+}
+
+// |t_{i,i-1}| <= eps (|t_{i-1,i-1}| + |t_{i,i}|) (the test of negligible_mask)
+__device__ __forceinline__ bool sub_negligible(Mat T, int i) {
+  const double eps = 2.220446049250313e-16;
+  const cd a = T.get(i - 1, i - 1), b = T.get(i, i), sd = T.get(i, i - 1);
+  const double d = fabs(a.re) + fabs(a.im) + fabs(b.re) + fabs(b.im);
+  const double sub = fabs(sd.re) + fabs(sd.im);
+  return sub <= eps * d || sub < 1e-300;
+}
+
+// Z[:, lo..hi] <- Z[:, lo..hi] G_lo^H ... G_{hi-1}^H from the rotation log
+__device__ void apply_log_to_z(Mat Z, int kdim, int lo, int hi, RotLog log) {
+  for (int r = threadIdx.x; r < kdim; r += blockDim.x) {
+    cd a = Z.get(r, lo);
+#pragma unroll 4
+    for (int kk = lo; kk < hi; ++kk) {
+      const cd b = Z.get(r, kk + 1);
+      const double c = log.c[kk];
+      const cd s = log.s[kk];
+      Z.set(r, kk, gcol_left(a, b, c, s));
+      a = gcol_right(a, b, c, s);
+    }
+    Z.set(r, hi, a);
+  }
+}
+
+// One implicit single-shift QR sweep with shift mu over rows/columns
+// [lo, hi] of T (the same rotations as chase), accumulating Z; fg: two LDS
+// entries.  Every thread calls; ends with a barrier.
+__device__ void sweep_general(Mat T, Mat Z, int kdim, int lo, int hi, cd mu, RotLog log,
+                              cd* fg) {
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  if (tid == 0) {
+    fg[0] = csub(T.get(lo, lo), mu);
+    fg[1] = T.get(lo + 1, lo);
+  }
+  __syncthreads();
+  for (int kk = lo; kk < hi; ++kk) {
+    double c;
+    cd s;
+    cgivens(fg[0], fg[1], c, s);
+    if (tid == 0) {
+      log.c[kk] = c;
+      log.s[kk] = s;
+    }
+    const int j0 = kk > lo ? kk - 1 : kk;
+    for (int j = j0 + tid; j < kdim; j += nthr) {
+      const cd a = T.get(kk, j), b = T.get(kk + 1, j);
+      T.set(kk, j, grow_top(a, b, c, s));
+      T.set(kk + 1, j, (kk > lo && j == kk - 1) ? cd{0.0, 0.0} : grow_bot(a, b, c, s));
+    }
+    __syncthreads();
+    const int rmax = kk + 2 < hi ? kk + 2 : hi;
+    for (int r = tid; r <= rmax; r += nthr) {
+      const cd x = T.get(r, kk), y = T.get(r, kk + 1);
+      const cd nx = gcol_left(x, y, c, s);
+      T.set(r, kk, nx);
+      T.set(r, kk + 1, gcol_right(x, y, c, s));
+      if (r == kk + 1) fg[0] = nx;
+      if (r == kk + 2) fg[1] = nx;
+    }
+    __syncthreads();
+  }
+  apply_log_to_z(Z, kdim, lo, hi, log);
+  __syncthreads();
+}
+
+// Complex Schur form of T (mm x mm), accumulating Z: deflation at the
+// bottom, Wilkinson shifts (exceptional shifts every tenth iteration), as
+// schur() with one sweep per iteration.  scr: one LDS int.
+__device__ int schur_general(Mat T, Mat Z, int mm, RotLog log, cd* fg, int* scr) {
+  int ihi = mm - 1, iter = 0, total = 0;
+  while (ihi >= 0) {
+    if (threadIdx.x == 0) scr[0] = 0;
+    __syncthreads();
+    // largest l <= ihi with a negligible subdiagonal (0 if none)
+    for (int i = 1 + (int)threadIdx.x; i <= ihi; i += blockDim.x)
+      if (sub_negligible(T, i)) atomicMax(scr, i);
+    __syncthreads();
+    const int l = scr[0];
+    if (l > 0 && threadIdx.x == 0) T.set(l, l - 1, {0, 0});
+    __syncthreads();
+    if (l == ihi) {
+      --ihi;
+      iter = 0;
+      continue;
+    }
+    if (++total > 100 * mm) return -1;
+    cd mu;
+    if (iter > 0 && iter % 10 == 0) {
+      const cd d = T.get(ihi, ihi), sd = T.get(ihi, ihi - 1);
+      mu = {d.re + 0.75 * fabs(sd.re), d.im};
+    } else {
+      const cd a = T.get(ihi - 1, ihi - 1), b = T.get(ihi - 1, ihi), c = T.get(ihi, ihi - 1),
+               d = T.get(ihi, ihi);
+      const cd h = {0.5 * (a.re - d.re), 0.5 * (a.im - d.im)};
+      const cd q = cadd(cmul(h, h), cmul(b, c));
+      const double mod = hypot(q.re, q.im);
+      double sr = sqrt(0.5 * (mod + q.re)), si = sqrt(0.5 * (mod - q.re));
+      if (q.im < 0) si = -si;
+      const cd t1 = {h.re + sr, h.im + si}, t2 = {h.re - sr, h.im - si};
+      mu = cabs2(t1) < cabs2(t2) ? cadd(d, t1) : cadd(d, t2);
+    }
+    ++iter;
+    sweep_general(T, Z, mm, l, ihi, mu, log, fg);
+  }
+  return 0;
+}
+
+// The exact-shift sweeps of the general restart: for each shift, T is split
+// at its negligible subdiagonals (set to zero) and every unreduced block
+// [lo, hi] gets one sweep.  negl: kMaxMBig LDS flags.
+__device__ void shifts_general(Mat T, Mat Z, int mm, int nshift, const cd* mus, RotLog log,
+                               cd* fg, unsigned char* negl) {
+  for (int t = 0; t < nshift; ++t) {
+    for (int i = 1 + (int)threadIdx.x; i < mm; i += blockDim.x) {
+      const bool ng = sub_negligible(T, i);
+      negl[i] = ng ? 1 : 0;
+      if (ng) T.set(i, i - 1, {0, 0});
+    }
+    __syncthreads();
+    int lo = 0;
+    while (lo < mm) {
+      int hi = lo;
+      while (hi + 1 < mm && !negl[hi + 1]) ++hi;
+      if (hi > lo) sweep_general(T, Z, mm, lo, hi, mus[t], log, fg);
+      lo = hi + 1;
+    }
+    __syncthreads();  // the flags are read before the next shift writes them
+  }
+}
 // Dynamic LDS of the restart kernel: T and Z, and the eigenvector work X
 // beside them when the three fit (m <= 54); larger m keep X in global memory.
 constexpr int kRestartLdsMax = 144 * 1024;
@@ -503,17 +653,21 @@ __device__ bool ritz_before(cd a, cd b, int which) {
   }
 }
 
-// The restart of one problem by one workgroup (any block size; k_restart
-// launches 8 waves).  smem: T and Z, mact x
-// mact complex each with the odd leading dimension lds_ld(mact), so that a
-// column walk (one row per lane) spreads over the LDS banks
-// (4 mact lds_ld(mact) doubles), then X, the eigenvectors of T (column i for
-// T[i][i]; 2 mact lds_ld(mact) doubles) when x_in_lds(mact), else in the
-// global work a.X (m x m complex).  Outputs: status (nconv, kk, sorted
-// Ritz values, estimates), Y (the first ky Ritz vectors of H in wanted
-// order), and with a.do_shifts the compressed H and the accumulated Q of the
-// exact shifts.
-__device__ void restart_body(const RestartArgs& a, double* smem) {
+// The restart of one problem by one workgroup.  GENERAL = false (k_restart,
+// 8 waves, mact <= kMaxM): T and Z in LDS, mact x mact complex each with the
+// odd leading dimension lds_ld(mact), so that a column walk (one row per
+// lane) spreads over the LDS banks (4 mact lds_ld(mact) doubles), then X,
+// the eigenvectors of T (column i for T[i][i]; 2 mact lds_ld(mact) doubles)
+// when x_in_lds(mact), else in the global work a.X (m x m complex); the
+// Schur form and the exact shifts run as bulge trains.  GENERAL = true
+// (k_restart_general, mact <= kMaxMBig): Z in the global work a.TZ, T in LDS
+// when general_t_in_lds(mact) else in a.TZ, X in a.X, single-shift sweeps
+// (sweep_general).  NM: capacity of the per-value LDS arrays.  Outputs:
+// status (nconv, kk, sorted Ritz values, estimates), Y (the first ky Ritz
+// vectors of H in wanted order), and with a.do_shifts the compressed H and
+// the accumulated Q of the exact shifts.
+template <int NM, bool GENERAL>
+__device__ void restart_body_t(const RestartArgs& a, double* smem) {
   // A breakdown flagged during this factorisation (the same value for every
   // thread: the orthogonalisation kernels have finished) limits the active
   // block to the invariant leading block, with no shifts; one below k or 2
@@ -528,31 +682,39 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
       do_shifts = false;
     }
   }
+  const size_t mm2 = (size_t)a.m * a.m;
   const int ld = lds_ld(mm);
-  Mat T{smem, smem + mm * ld, ld};
-  Mat Z{smem + 2 * mm * ld, smem + 3 * mm * ld, ld};
-  __shared__ cd ev[kMaxM];
-  __shared__ double est[kMaxM];
-  __shared__ int order[kMaxM];
+  // where T and Z live: LDS, or the TZ planes (leading dimension a.m); the
+  // launch sized the LDS for a.mact (>= mm)
+  const bool t_lds = !GENERAL || general_t_in_lds(a.mact);
+  const bool z_lds = !GENERAL;
+  Mat T = t_lds ? Mat{smem, smem + mm * ld, ld} : Mat{a.TZ, a.TZ + mm2, a.m};
+  Mat Z = z_lds ? Mat{smem + 2 * mm * ld, smem + 3 * mm * ld, ld}
+                : Mat{a.TZ + 2 * mm2, a.TZ + 3 * mm2, a.m};
+  __shared__ cd ev[NM];
+  __shared__ double est[NM];
+  __shared__ int order[NM];
   __shared__ double tnorm_s;
   __shared__ int kk_s;  // kept size, broadcast through LDS (uniform loop bound)
+  __shared__ cd fg_s[2];
+  __shared__ int scr_s[1];
+  __shared__ unsigned char negl_s[GENERAL ? NM : 1];
   Status* st = a.st;
   const int lane = threadIdx.x, nthr = blockDim.x;
   __shared__ double tmax_s[1024];
 
   if (lane == 0) st->clk[0] = clock64();
-  __shared__ double log_c[kMaxM];
-  __shared__ cd log_s[kMaxM];
+  __shared__ double log_c[NM];
+  __shared__ cd log_s[NM];
   const RotLog log{log_c, log_s};
   int fail = 0;
-  const size_t mm2 = (size_t)a.m * a.m;
   if (a.phase == 2) {
-    // the Schur form of the first launch
+    // the Schur form of the first launch (what of it lives in LDS)
     for (int idx = lane; idx < mm * mm; idx += nthr) {
       int r = idx / mm, c = idx - r * mm;
       const size_t g = (size_t)r * a.m + c;
-      T.set(r, c, {a.TZ[g], a.TZ[mm2 + g]});
-      Z.set(r, c, {a.TZ[2 * mm2 + g], a.TZ[3 * mm2 + g]});
+      if (t_lds) T.set(r, c, {a.TZ[g], a.TZ[mm2 + g]});
+      if (z_lds) Z.set(r, c, {a.TZ[2 * mm2 + g], a.TZ[3 * mm2 + g]});
     }
     fail = st->schur_fail;
     __syncthreads();
@@ -565,7 +727,10 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
       Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
     }
     __syncthreads();
-    fail = schur(T, Z, mm, log);
+    if (GENERAL)
+      fail = schur_general(T, Z, mm, log, fg_s, scr_s);
+    else
+      fail = schur(T, Z, mm, log);
     __syncthreads();
   }
   if (lane == 0) st->clk[1] = clock64();
@@ -574,7 +739,10 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   double tm = 0;
   for (int idx = lane; idx < mm * mm; idx += nthr) {
     int r = idx / mm, c = idx - r * mm;
-    if (c >= r) tm = fmax(tm, hypot(T.re[r * ld + c], T.im[r * ld + c]));
+    if (c >= r) {
+      const cd t = T.get(r, c);
+      tm = fmax(tm, hypot(t.re, t.im));
+    }
   }
   tmax_s[lane] = tm;
   __syncthreads();
@@ -589,7 +757,7 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   // X in LDS when it fits in what the launch allocated for a.mact (each
   // lane's dependent back substitution then reads LDS, not L2), else the
   // global work a.X
-  const bool xl = 6 * mm * ld * (int)sizeof(double) <= restart_lds_bytes(a.mact);
+  const bool xl = !GENERAL && 6 * mm * ld * (int)sizeof(double) <= restart_lds_bytes(a.mact);
   double* Xre = xl ? smem + 4 * mm * ld : a.Xre;
   double* Xim = xl ? smem + 5 * mm * ld : a.Xim;
   const size_t xld = xl ? (size_t)ld : (size_t)a.m;
@@ -641,10 +809,16 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
     for (int idx = lane; idx < mm * mm; idx += nthr) {
       int r = idx / mm, c = idx - r * mm;
       const size_t g = (size_t)r * a.m + c;
-      a.TZ[g] = T.re[r * ld + c];
-      a.TZ[mm2 + g] = T.im[r * ld + c];
-      a.TZ[2 * mm2 + g] = Z.re[r * ld + c];
-      a.TZ[3 * mm2 + g] = Z.im[r * ld + c];
+      if (t_lds) {
+        const cd t = T.get(r, c);
+        a.TZ[g] = t.re;
+        a.TZ[mm2 + g] = t.im;
+      }
+      if (z_lds) {
+        const cd z = Z.get(r, c);
+        a.TZ[2 * mm2 + g] = z.re;
+        a.TZ[3 * mm2 + g] = z.im;
+      }
     }
     return;  // uniform: every thread returns
   }
@@ -725,10 +899,12 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   __syncthreads();
   // the p = mm - kk unwanted values, in trains of up to one per wave
   const int nw = nthr / 64;
-  __shared__ cd mus_s[kMaxM];
+  __shared__ cd mus_s[NM];
   for (int t = kk + lane; t < mm; t += nthr) mus_s[t - kk] = ev[order[t]];
   __syncthreads();
-  if (nw >= 2) {
+  if (GENERAL) {
+    shifts_general(T, Z, mm, mm - kk, mus_s, log, fg_s, negl_s);
+  } else if (nw >= 2) {
     for (int t0 = 0; t0 < mm - kk; t0 += nw) {
       const int ns = mm - kk - t0 < nw ? mm - kk - t0 : nw;
       shifted_train(T, Z, mm, ns, mus_s + t0);
@@ -740,21 +916,26 @@ __device__ void restart_body(const RestartArgs& a, double* smem) {
   if (lane == 0) st->clk[5] = clock64();
   for (int idx = lane; idx < mm * mm; idx += nthr) {
     int r = idx / mm, c = idx - r * mm;
-    a.Qre[(size_t)r * a.m + c] = Z.re[r * ld + c];
-    a.Qim[(size_t)r * a.m + c] = Z.im[r * ld + c];
+    const cd z = Z.get(r, c), t = T.get(r, c);
+    a.Qre[(size_t)r * a.m + c] = z.re;
+    a.Qim[(size_t)r * a.m + c] = z.im;
     bool keep = r < kk && c < kk;
-    a.Hre[(size_t)r * a.m + c] = keep ? T.re[r * ld + c] : 0.0;
-    a.Him[(size_t)r * a.m + c] = keep ? T.im[r * ld + c] : 0.0;
+    a.Hre[(size_t)r * a.m + c] = keep ? t.re : 0.0;
+    a.Him[(size_t)r * a.m + c] = keep ? t.im : 0.0;
   }
   if (lane == 0) {
-    st->beta_k_re = T.re[kk * ld + kk - 1];
-    st->beta_k_im = T.im[kk * ld + kk - 1];
-    st->sigma_re = Z.re[(mm - 1) * ld + kk - 1];
-    st->sigma_im = Z.im[(mm - 1) * ld + kk - 1];
+    const cd bk = T.get(kk, kk - 1), sg = Z.get(mm - 1, kk - 1);
+    st->beta_k_re = bk.re;
+    st->beta_k_im = bk.im;
+    st->sigma_re = sg.re;
+    st->sigma_im = sg.im;
   }
   __syncthreads();
 }
 
+__device__ __forceinline__ void restart_body(const RestartArgs& a, double* smem) {
+  restart_body_t<kMaxM, false>(a, smem);
+}
 
 }  // namespace
 }  // namespace iram

@@ -387,7 +387,7 @@ def test_argument_errors():
     s.setMatrix(dA)
     with pytest.raises(OapError):  # hdim must exceed wanted
         s.execute(4, 4, MatrixInfo(True, False, 64, 64))
-    with pytest.raises(OapError):  # Krylov dimension above the one-wave limit
+    with pytest.raises(OapError):  # Krylov dimension above the operator's order
         s.execute(65, 4, MatrixInfo(True, False, 64, 64))
 
 
