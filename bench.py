@@ -24,8 +24,9 @@ beside it (`plain_fp64`).  `batched_c3` is the 512-problem covariance batch
 vector on held-out convection-diffusion-reaction operators (n = 4096, 6
 smallest-real eigenpairs).
 `standalone_c1` times the C1 standalone procedures: the complex dotProduct of
-a 4096 x 4096 matrix with a vector, and magnitude / sum over 2^26 complex
-elements.
+a 4096 x 4096 matrix with a vector, the 4096 x 4096 times 4096 x 4096 complex
+product on the FP64 matrix cores (fraction of the 78.6 TF/s FP64 peak), and
+magnitude / sum over 2^26 complex elements.
 `roofline` is for the dominant kernel of the reported solve (the FP32-operand
 matvec), timed with HIP events on the solver's stream; `cpu_baseline` runs the
 oracle's FP64 IRAM (threaded dense matvec) for one full solve of the same
@@ -54,6 +55,7 @@ SEED = 0x5EED0001
 EPS_OFF, DECAY = 1e-2, 500.0
 TOL = 1e-10
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 (matrix cores and vector FMA alike)
 
 
 def log(msg: str) -> None:
@@ -193,7 +195,17 @@ def standalone_c1() -> dict:
     out["dot_product_4096"] = {"us": round(ms.value * 1e3, 2), "bytes": b,
                                "gbs": round(b / (ms.value * 1e-3) / 1e9, 1),
                                "frac": round(b / (ms.value * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
-    del A
+    # the GEMM reading: complex 4096 x 4096 times 4096 x 4096 (8 n^3 flops)
+    # on the FP64 matrix cores
+    B = DeviceMatrix(n, n, True, True)
+    Cm = DeviceMatrix(n, n, True, True)
+    check(lib.oaph_synth_matrix(B.ptr, 1, SEED + 1, EPS_OFF, DECAY), "synth")
+    check(lib.oaph_time_dot_product(api._h, Cm.ptr, A.ptr, B.ptr, 5, C.byref(ms)), "time_gemm")
+    tf = 8 * n ** 3 / (ms.value * 1e-3) / 1e12
+    out["gemm_4096"] = {"ms": round(ms.value, 3), "flops": 8 * n ** 3, "tflops": round(tf, 2),
+                        "peak_tflops": FP64_PEAK_TFLOPS, "frac": round(tf / FP64_PEAK_TFLOPS, 4),
+                        "kernel": "k_zgemm_mfma (v_mfma_f64_16x16x4_f64)"}
+    del A, B, Cm
     Z = DeviceMatrix(1 << 13, 1 << 13, True, True)  # 2^26 complex elements
     check(lib.oaph_synth_matrix(Z.ptr, 1, SEED, EPS_OFF, DECAY), "synth")
     b = 16 << 26

@@ -504,6 +504,13 @@ oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, d
  * taken one after another); process-wide.  No reference counterpart (tuning
  * knob). */
 oaph_status oaph_set_matvec_shape(int fp32, int variant);
+/* extension (no reference counterpart): which kernel forms dotProduct.
+ * 1 (default): products with W, H >= 256 and K >= 64 on the FP64 matrix
+ * cores (v_mfma_f64_16x16x4_f64, 128 x 128 tiles), smaller ones on the
+ * vector-FMA tiles; 2: every product on the matrix cores; 0: every product on
+ * the vector-FMA tiles.  Both are FP64 throughout and agree to rounding (they
+ * sum the K terms in different orders). */
+oaph_status oaph_set_gemm_path(int mfma);
 oaph_status oaph_matvec_shape_info(int fp32, int variant, int* rows_per_wave,
                                    int* loads_in_flight, int* row_groups);
 /* Average duration of `iters` back-to-back dotProduct calls (HIP events on

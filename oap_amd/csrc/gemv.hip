@@ -439,6 +439,9 @@ constexpr Shape kCShapes[] = {{4, 3, 2}, {4, 3, 1}, {4, 2, 2}, {4, 4, 2}, {2, 2,
 constexpr int kNumZ = sizeof(kZShapes) / sizeof(kZShapes[0]);
 constexpr int kNumC = sizeof(kCShapes) / sizeof(kCShapes[0]);
 int g_zshape = 0, g_cshape = 0;
+// dotProducts on the matrix cores: 1 large ones (gemm_mfma_eligible), 2 all,
+// 0 none (the vector-FMA tiles)
+int g_gemm_mfma = 1;
 
 using ZFn = void (*)(uint32_t, uint32_t, uint32_t, const double*, const double*, const double*,
                      const double*, double*, double*, MvProj);
@@ -579,6 +582,10 @@ oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
                         b->reValues, mode == 0 ? b->imValues : nullptr, out->reValues,
                         mode == 0 ? out->imValues : nullptr);
   }
+  // large products on the matrix cores (gemm.hip)
+  if (g_gemm_mfma == 2 || (g_gemm_mfma == 1 && gemm_mfma_eligible(W, H, K)))
+    return launch_gemm_mfma(s, mode, W, H, K, a->reValues, a->imValues, a->columns, b->reValues,
+                            b->imValues, b->columns, out->reValues, out->imValues, out->columns);
   dim3 grid((W + GT - 1) / GT, (H + GT - 1) / GT);
   OAPH_CHECK(grid.y <= 65535, OAPH_ERR_UNSUPPORTED);
   if (mode == 0)
@@ -618,6 +625,12 @@ oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, co
 }  // namespace oaph
 
 extern "C" {
+
+oaph_status oaph_set_gemm_path(int mfma) {
+  OAPH_CHECK(mfma >= 0 && mfma <= 2, OAPH_ERR_ARG);
+  oaph::g_gemm_mfma = mfma;
+  return OAPH_OK;
+}
 
 oaph_status oaph_set_matvec_shape(int fp32, int variant) {
   using namespace oaph;

@@ -123,6 +123,12 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
                          const double* are, const double* aim, const double* xre,
                          const double* xim, double* yre, double* yim, MvProj* pj = nullptr);
 // out(x, y) = sum_k A[y][k] B[x][k] (A * B^T, B read in place), W x H, K terms
+// the matrix-core complex product (gemm.hip): 128 x 128 tiles, for W, H >= 256
+bool gemm_mfma_eligible(uint32_t W, uint32_t H, uint32_t K);
+oaph_status launch_gemm_mfma(hipStream_t s, int mode, uint32_t W, uint32_t H, uint32_t K,
+                             const double* are, const double* aim, uint32_t lda,
+                             const double* bre, const double* bim, uint32_t ldb, double* ore,
+                             double* oim, uint32_t ldo);
 oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, const double* are,
                            const double* aim, uint32_t lda, const double* bre, const double* bim,
                            uint32_t ldb, double* ore, double* oim, uint32_t ldo);
