@@ -406,26 +406,55 @@ def compare(model: LearnedStart, params: np.ndarray, seed: int = 7, mixed: bool 
 
 
 def main(argv=None) -> int:
-    """python -m oap_amd.learned train --solves 1024 --out model.npz"""
+    """python -m oap_amd.learned train --solves 1024 --out model.npz
+    python -m oap_amd.learned generate --total 20000 --chunk 0 --chunk-size 5000 --out d0.npz
+    python -m oap_amd.learned train --data d0.npz d1.npz ... --out model.npz
+
+    `generate` solves one slice of the `total` operators drawn from the
+    sampling box (the same draw for every chunk, so chunks partition it) and
+    stores (params, targets, matvecs); `train --data` fits on the union of
+    such files instead of generating."""
     import argparse
     import sys
     ap = argparse.ArgumentParser()
-    ap.add_argument("cmd", choices=["train"])
+    ap.add_argument("cmd", choices=["train", "generate"])
     ap.add_argument("--solves", type=int, default=1024)
+    ap.add_argument("--total", type=int, default=20000)
+    ap.add_argument("--chunk", type=int, default=0)
+    ap.add_argument("--chunk-size", type=int, default=5000)
+    ap.add_argument("--data", nargs="*", default=None)
     ap.add_argument("--epochs", type=int, default=600)
+    ap.add_argument("--batch", type=int, default=64)
+    ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--seed", type=int, default=0x5EED0004)
     ap.add_argument("--out", default=str(DEFAULT_MODEL))
     ap.add_argument("--eval", type=int, default=32)
     a = ap.parse_args(argv)
     log = lambda s: print(s, file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    params = sample_params(a.solves, a.seed)
-    P, T, mv = generate(params, log=log)
-    log(f"generated {len(P)}/{a.solves} converged solves in {time.perf_counter() - t0:.1f} s, "
-        f"mean matvecs {mv.mean():.0f}")
+    if a.cmd == "generate":
+        lo = a.chunk * a.chunk_size
+        params = sample_params(a.total, a.seed)[lo:lo + a.chunk_size]
+        P, T, mv = generate(params, seed=1 + lo, log=log)
+        np.savez(a.out, params=P, targets=T, matvecs=mv)
+        log(f"chunk {a.chunk}: {len(P)}/{len(params)} converged solves in "
+            f"{time.perf_counter() - t0:.1f} s, mean matvecs {mv.mean():.0f}")
+        return 0
+    if a.data:
+        parts = [np.load(f, allow_pickle=False) for f in a.data]
+        P = np.concatenate([d["params"] for d in parts])
+        T = np.concatenate([d["targets"] for d in parts])
+        mv = np.concatenate([d["matvecs"] for d in parts])
+        log(f"loaded {len(P)} converged solves from {len(a.data)} files, mean matvecs "
+            f"{mv.mean():.0f}")
+    else:
+        params = sample_params(a.solves, a.seed)
+        P, T, mv = generate(params, log=log)
+        log(f"generated {len(P)}/{a.solves} converged solves in {time.perf_counter() - t0:.1f} s, "
+            f"mean matvecs {mv.mean():.0f}")
     model = LearnedStart()
     t1 = time.perf_counter()
-    losses = model.fit(P, T, epochs=a.epochs)
+    losses = model.fit(P, T, epochs=a.epochs, batch=a.batch, lr=a.lr)
     log(f"trained {a.epochs} epochs in {time.perf_counter() - t1:.1f} s, loss "
         f"{losses[0]:.4g} -> {losses[-1]:.4g}")
     Path(a.out).parent.mkdir(parents=True, exist_ok=True)
