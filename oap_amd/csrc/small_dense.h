@@ -485,13 +485,12 @@ __host__ __device__ constexpr int lds_ld(int m) { return m | 1; }
 // ---------------------------------------------------------------------------
 // General Krylov dimension (kMaxM < m <= kMaxMBig).  T lives in LDS while it
 // fits (general_t_in_lds), else in global memory like Z (the TZ work, L2
-// resident); one implicit single-shift sweep at a time by all threads.  Each
-// rotation is a row phase (threads over the columns of the row pair) and a
-// column phase (threads over the rows of T's column pair) between block
-// barriers; the threads that form the next rotation's (f, g) hand them over
-// in LDS, and the sweep's logged rotations are applied to Z after the sweep,
-// one row of Z per thread.  Same iteration as shifted_sweep in
-// oracle/oracle.c (single shifts, no bulge trains).
+// resident); the sweeps run as bulge trains (chase_train_general) of two
+// Wilkinson shifts in the Schur phase and up to one exact shift per wave,
+// with the lanes walking 64-wide chunks of the rows and columns and Z
+// updated after each train from the rotation logs, one row of Z per thread
+// (the logs live in the eigenvector work a.X, idle in both phases).  Same
+// iteration as shifted_sweep in oracle/oracle.c.
 constexpr int kGeneralThreads = 256;
 constexpr int kGeneralLdsMax = 128 * 1024;
 __host__ __device__ constexpr bool general_t_in_lds(int m) {
@@ -526,51 +525,74 @@ __device__ void apply_log_to_z(Mat Z, int kdim, int lo, int hi, RotLog log) {
   }
 }
 
-// One implicit single-shift QR sweep with shift mu over rows/columns
-// [lo, hi] of T (the same rotations as chase), accumulating Z; fg: two LDS
-// entries.  Every thread calls; ends with a barrier.
-__device__ void sweep_general(Mat T, Mat Z, int kdim, int lo, int hi, cd mu, RotLog log,
-                              cd* fg) {
-  const int tid = threadIdx.x, nthr = blockDim.x;
-  if (tid == 0) {
-    fg[0] = csub(T.get(lo, lo), mu);
-    fg[1] = T.get(lo + 1, lo);
-  }
-  __syncthreads();
-  for (int kk = lo; kk < hi; ++kk) {
-    double c;
-    cd s;
-    cgivens(fg[0], fg[1], c, s);
-    if (tid == 0) {
-      log.c[kk] = c;
-      log.s[kk] = s;
-    }
-    const int j0 = kk > lo ? kk - 1 : kk;
-    for (int j = j0 + tid; j < kdim; j += nthr) {
-      const cd a = T.get(kk, j), b = T.get(kk + 1, j);
-      T.set(kk, j, grow_top(a, b, c, s));
-      T.set(kk + 1, j, (kk > lo && j == kk - 1) ? cd{0.0, 0.0} : grow_bot(a, b, c, s));
+// A train of ns (<= waves) implicit single-shift sweeps over rows/columns
+// [lo, hi] of T for the general restart (kdim <= kMaxMBig): the steps of
+// chase_train (bulge w chased by wave w three rows behind bulge w - 1, a row
+// phase and a column phase between block barriers, each bulge's next (f, g)
+// kept in registers) with the lanes walking the columns and rows in chunks
+// of 64.  Z takes the column rotations after the train, sweep by sweep from
+// the rotation logs (lgc, lgs: ns logs of kMaxMBig entries in global
+// memory): the train applies Z's column pairs in an order that differs from
+// sweep by sweep only by swapping rotations of disjoint column pairs.
+__device__ void chase_train_general(Mat T, Mat Z, int kdim, int lo, int hi, int ns, cd mu,
+                                    double* lgc, cd* lgs) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const int L = hi - lo;
+  const int steps = L + 3 * (ns - 1);
+  cd fn = {0.0, 0.0}, gn = {0.0, 0.0};
+  for (int t = 0; t < steps; ++t) {
+    const int p = t - 3 * wave;
+    const bool act = wave < ns && p >= 0 && p < L;
+    const int k = lo + p;
+    double c = 1.0;
+    cd s = {0.0, 0.0};
+    if (act) {
+      if (k == lo) {
+        fn = csub(T.get(lo, lo), mu);
+        gn = T.get(lo + 1, lo);
+      }
+      cgivens(fn, gn, c, s);
+      if (lane == 0) {
+        lgc[wave * kMaxMBig + k] = c;
+        lgs[wave * kMaxMBig + k] = s;
+      }
+      const int j0 = k > lo ? k - 1 : k;
+      for (int j = j0 + lane; j < kdim; j += 64) {
+        const cd a = T.get(k, j), b = T.get(k + 1, j);
+        T.set(k, j, grow_top(a, b, c, s));
+        T.set(k + 1, j, (k > lo && j == k - 1) ? cd{0.0, 0.0} : grow_bot(a, b, c, s));
+      }
     }
     __syncthreads();
-    const int rmax = kk + 2 < hi ? kk + 2 : hi;
-    for (int r = tid; r <= rmax; r += nthr) {
-      const cd x = T.get(r, kk), y = T.get(r, kk + 1);
-      const cd nx = gcol_left(x, y, c, s);
-      T.set(r, kk, nx);
-      T.set(r, kk + 1, gcol_right(x, y, c, s));
-      if (r == kk + 1) fg[0] = nx;
-      if (r == kk + 2) fg[1] = nx;
+    if (act) {
+      const int rmax = k + 2 < hi ? k + 2 : hi;
+      cd f1 = {0.0, 0.0}, g1 = {0.0, 0.0};
+      for (int r = lane; r <= rmax; r += 64) {
+        const cd x = T.get(r, k), y = T.get(r, k + 1);
+        const cd nx = gcol_left(x, y, c, s);
+        T.set(r, k, nx);
+        T.set(r, k + 1, gcol_right(x, y, c, s));
+        if (r == k + 1) f1 = nx;
+        if (r == k + 2) g1 = nx;
+      }
+      if (k + 1 < hi) {
+        fn = lane_read(f1, (k + 1) & 63);
+        gn = lane_read(g1, (k + 2) & 63);
+      }
     }
     __syncthreads();
   }
-  apply_log_to_z(Z, kdim, lo, hi, log);
+  for (int w = 0; w < ns; ++w)
+    apply_log_to_z(Z, kdim, lo, hi, RotLog{lgc + w * kMaxMBig, lgs + w * kMaxMBig});
   __syncthreads();
 }
 
 // Complex Schur form of T (mm x mm), accumulating Z: deflation at the
-// bottom, Wilkinson shifts (exceptional shifts every tenth iteration), as
-// schur() with one sweep per iteration.  scr: one LDS int.
-__device__ int schur_general(Mat T, Mat Z, int mm, RotLog log, cd* fg, int* scr) {
+// bottom, both eigenvalues of the trailing 2 x 2 as a train of two (one
+// shift when the window is below 4; exceptional shifts every tenth
+// iteration), as schur().  scr: one LDS int; lgc, lgs: the train logs.
+__device__ int schur_general(Mat T, Mat Z, int mm, int* scr, double* lgc, cd* lgs) {
   int ihi = mm - 1, iter = 0, total = 0;
   while (ihi >= 0) {
     if (threadIdx.x == 0) scr[0] = 0;
@@ -588,7 +610,8 @@ __device__ int schur_general(Mat T, Mat Z, int mm, RotLog log, cd* fg, int* scr)
       continue;
     }
     if (++total > 100 * mm) return -1;
-    cd mu;
+    cd mu, mu2 = {0.0, 0.0};
+    bool two = false;
     if (iter > 0 && iter % 10 == 0) {
       const cd d = T.get(ihi, ihi), sd = T.get(ihi, ihi - 1);
       mu = {d.re + 0.75 * fabs(sd.re), d.im};
@@ -601,36 +624,47 @@ __device__ int schur_general(Mat T, Mat Z, int mm, RotLog log, cd* fg, int* scr)
       double sr = sqrt(0.5 * (mod + q.re)), si = sqrt(0.5 * (mod - q.re));
       if (q.im < 0) si = -si;
       const cd t1 = {h.re + sr, h.im + si}, t2 = {h.re - sr, h.im - si};
-      mu = cabs2(t1) < cabs2(t2) ? cadd(d, t1) : cadd(d, t2);
+      const bool near1 = cabs2(t1) < cabs2(t2);
+      mu = near1 ? cadd(d, t1) : cadd(d, t2);
+      mu2 = near1 ? cadd(d, t2) : cadd(d, t1);
+      two = ihi - l >= 3 && blockDim.x >= 128;
     }
     ++iter;
-    sweep_general(T, Z, mm, l, ihi, mu, log, fg);
+    __syncthreads();  // every wave has read the window before the train writes it
+    chase_train_general(T, Z, mm, l, ihi, two ? 2 : 1, (threadIdx.x >> 6) == 1 ? mu2 : mu, lgc,
+                        lgs);
   }
   return 0;
 }
 
-// The exact-shift sweeps of the general restart: for each shift, T is split
-// at its negligible subdiagonals (set to zero) and every unreduced block
-// [lo, hi] gets one sweep.  negl: kMaxMBig LDS flags.
-__device__ void shifts_general(Mat T, Mat Z, int mm, int nshift, const cd* mus, RotLog log,
-                               cd* fg, unsigned char* negl) {
-  for (int t = 0; t < nshift; ++t) {
+// The exact-shift sweeps of the general restart, in trains of up to one
+// shift per wave: before each train T is split at its negligible
+// subdiagonals (set to zero) and every unreduced block [lo, hi] gets the
+// train.  negl: kMaxMBig LDS flags.
+__device__ void shifts_general(Mat T, Mat Z, int mm, int nshift, const cd* mus,
+                               unsigned char* negl, double* lgc, cd* lgs) {
+  const int nw = blockDim.x / 64;
+  for (int t0 = 0; t0 < nshift; t0 += nw) {
+    const int ns = nshift - t0 < nw ? nshift - t0 : nw;
     for (int i = 1 + (int)threadIdx.x; i < mm; i += blockDim.x) {
       const bool ng = sub_negligible(T, i);
       negl[i] = ng ? 1 : 0;
       if (ng) T.set(i, i - 1, {0, 0});
     }
     __syncthreads();
+    const int wave = threadIdx.x >> 6;
+    const cd mu = mus[t0 + (wave < ns ? wave : 0)];
     int lo = 0;
     while (lo < mm) {
       int hi = lo;
       while (hi + 1 < mm && !negl[hi + 1]) ++hi;
-      if (hi > lo) sweep_general(T, Z, mm, lo, hi, mus[t], log, fg);
+      if (hi > lo) chase_train_general(T, Z, mm, lo, hi, ns, mu, lgc, lgs);
       lo = hi + 1;
     }
-    __syncthreads();  // the flags are read before the next shift writes them
+    __syncthreads();  // the flags are read before the next train writes them
   }
 }
+
 // Dynamic LDS of the restart kernel: T and Z, and the eigenvector work X
 // beside them when the three fit (m <= 54); larger m keep X in global memory.
 constexpr int kRestartLdsMax = 144 * 1024;
@@ -662,7 +696,7 @@ __device__ bool ritz_before(cd a, cd b, int which) {
 // Schur form and the exact shifts run as bulge trains.  GENERAL = true
 // (k_restart_general, mact <= kMaxMBig): Z in the global work a.TZ, T in LDS
 // when general_t_in_lds(mact) else in a.TZ, X in a.X, single-shift sweeps
-// (sweep_general).  NM: capacity of the per-value LDS arrays.  Outputs:
+// (chase_train_general).  NM: capacity of the per-value LDS arrays.  Outputs:
 // status (nconv, kk, sorted Ritz values, estimates), Y (the first ky Ritz
 // vectors of H in wanted order), and with a.do_shifts the compressed H and
 // the accumulated Q of the exact shifts.
@@ -696,7 +730,6 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
   __shared__ int order[NM];
   __shared__ double tnorm_s;
   __shared__ int kk_s;  // kept size, broadcast through LDS (uniform loop bound)
-  __shared__ cd fg_s[2];
   __shared__ int scr_s[1];
   __shared__ unsigned char negl_s[GENERAL ? NM : 1];
   Status* st = a.st;
@@ -728,7 +761,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
     }
     __syncthreads();
     if (GENERAL)
-      fail = schur_general(T, Z, mm, log, fg_s, scr_s);
+      fail = schur_general(T, Z, mm, scr_s, a.Xre, reinterpret_cast<cd*>(a.Xre + 4 * kMaxMBig));
     else
       fail = schur(T, Z, mm, log);
     __syncthreads();
@@ -903,7 +936,8 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
   for (int t = kk + lane; t < mm; t += nthr) mus_s[t - kk] = ev[order[t]];
   __syncthreads();
   if (GENERAL) {
-    shifts_general(T, Z, mm, mm - kk, mus_s, log, fg_s, negl_s);
+    shifts_general(T, Z, mm, mm - kk, mus_s, negl_s, a.Xre,
+                   reinterpret_cast<cd*>(a.Xre + 4 * kMaxMBig));
   } else if (nw >= 2) {
     for (int t0 = 0; t0 < mm - kk; t0 += nw) {
       const int ns = mm - kk - t0 < nw ? mm - kk - t0 : nw;
