@@ -377,8 +377,10 @@ def profiled_traffic(kernel: str):
 
 def traffic_probe() -> int:
     """Launches for the rocprofv3 counter runs: 5 FP32-operand matvecs, 5 FP64
-    matvecs, and 5 multiply_re_constant copies of a 2^27-double plane (1 GiB
-    read + 1 GiB write) as the calibration kernel of known size."""
+    matvecs, 5 multiply_re_constant copies of a 2^27-double plane (1 GiB
+    read + 1 GiB write) as the calibration kernel of known size, then 3
+    complex 4096^3 products on the FP64 matrix cores and 3 magnitude and 3
+    sum reductions over 2^26 complex elements."""
     import ctypes as C
     from oap_amd._lib import check, load
     from oap_amd.api import CuProceduresApi, DeviceMatrix, Precision
@@ -394,6 +396,21 @@ def traffic_probe() -> int:
     o = DeviceMatrix(1 << 14, 1 << 13, True, False)
     for _ in range(5):
         api.multiplyReConstant(o, a, 2.0)
+    api.synchronize()
+    del a, o, s, dA
+    n = 4096
+    a4, b4, c4 = DeviceMatrix(n, n), DeviceMatrix(n, n), DeviceMatrix(n, n)
+    check(lib.oaph_synth_matrix(a4.ptr, 1, SEED, EPS_OFF, DECAY), "synth")
+    check(lib.oaph_synth_matrix(b4.ptr, 1, SEED + 1, EPS_OFF, DECAY), "synth")
+    for _ in range(3):
+        api.dotProduct(c4, a4, b4)
+    api.synchronize()
+    del a4, b4, c4
+    z = DeviceMatrix(1 << 13, 1 << 13)
+    check(lib.oaph_synth_matrix(z.ptr, 1, SEED, EPS_OFF, DECAY), "synth")
+    for _ in range(3):
+        api.magnitude(z)
+        api.sum(z)
     api.synchronize()
     print("traffic probe done", flush=True)
     return 0
