@@ -566,6 +566,10 @@ class CuHArnoldi {
   void setRestartGuardWeight(double w) {
     oap::check(oaph_arnoldi_set_restart_guard_weight(m_a, w), "setRestartGuardWeight");
   }
+  /* extension: 1 perfect-shift Schur form where possible (default), 0 Wilkinson */
+  void setSchurMethod(int method) {
+    oap::check(oaph_arnoldi_set_schur_method(m_a, method), "setSchurMethod");
+  }
   oaph_arnoldi_stats stats() const {
     oaph_arnoldi_stats s;
     oap::check(oaph_arnoldi_get_stats(m_a, &s), "stats");

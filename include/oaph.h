@@ -449,6 +449,15 @@ oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* a, uint32_t guard);
 /* Weight of the guard Ritz vectors in a Ritz-sum restart (0..1, default
  * 1e-3; the wanted ones have weight 1).  No reference counterpart. */
 oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* a, double weight);
+/* How the restart forms the Schur form of H (extension, no reference
+ * counterpart; the reference triangularises H by unshifted QR steps,
+ * CuTriangularH.h).  1 (default): for hdim <= 54 the eigenvalues of H come
+ * first (Ehrlich-Aberth iteration on det(H - zI) by Hyman's recurrence) and
+ * serve as perfect shifts, eight per bulge train, smallest modulus first;
+ * where that does not converge or a split does not happen the Wilkinson
+ * iteration takes over.  0: Wilkinson shifts only.  Same Schur form to
+ * rounding either way. */
+oaph_status oaph_arnoldi_set_schur_method(oaph_arnoldi* a, int method);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
  * the largest Ritz value (default 1e-6) */
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* a, double tol_low);
@@ -469,6 +478,8 @@ typedef struct oaph_arnoldi_stats {
                               FP32 copy is unusable (row length not a multiple
                               of 4, or an entry beyond the FP32 range); the
                               solve ran FP64 cycles only */
+  uint32_t restarts_perfect_shift; /* restarts whose Schur form came from the
+                                      perfect shifts (oaph_arnoldi_set_schur_method) */
 } oaph_arnoldi_stats;
 oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
 

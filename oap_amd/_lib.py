@@ -48,7 +48,8 @@ class oaph_matrix_info(C.Structure):
 class oaph_arnoldi_stats(C.Structure):
     _fields_ = [("restarts", C.c_uint32), ("matvecs", C.c_uint32), ("matvecs_low", C.c_uint32),
                 ("nconv", C.c_uint32), ("seconds", C.c_double), ("max_resid", C.c_double),
-                ("fallbacks", C.c_uint32), ("mixed_declined", C.c_uint32)]
+                ("fallbacks", C.c_uint32), ("mixed_declined", C.c_uint32),
+                ("restarts_perfect_shift", C.c_uint32)]
 
 
 class oaph_matrix_ex(C.Structure):
@@ -157,6 +158,7 @@ _SIGS = {
     "oaph_arnoldi_set_projection": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_mixed_tolerance": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_restart_guard_weight": (C.c_int, [C.c_void_p, C.c_double]),
+    "oaph_arnoldi_set_schur_method": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_start_vector": (C.c_int, [C.c_void_p, DP, DP, U32]),
     "oaph_arnoldi_get_stats": (C.c_int, [C.c_void_p, C.POINTER(oaph_arnoldi_stats)]),
     "oaph_arnoldi_time_matvec": (C.c_int, [C.c_void_p, C.c_int, U32, DP]),

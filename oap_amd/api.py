@@ -613,6 +613,11 @@ class CuHArnoldiCallback:
         check(self._lib.oaph_arnoldi_set_restart_guard_weight(self._h, weight),
               "setRestartGuardWeight")
 
+    def setSchurMethod(self, method):
+        """1 (default): the restart's Schur form from perfect shifts (the
+        Aberth eigenvalues of H) where possible; 0: Wilkinson shifts only"""
+        check(self._lib.oaph_arnoldi_set_schur_method(self._h, method), "setSchurMethod")
+
     def setMixedTolerance(self, tol_low):
         check(self._lib.oaph_arnoldi_set_mixed_tolerance(self._h, tol_low), "set_mixed_tolerance")
 

@@ -22,6 +22,7 @@ struct Status {
   double ev_re[kMaxMBig], ev_im[kMaxMBig], est[kMaxMBig];  // Ritz values in wanted order, estimates
   int order[kMaxMBig];
   long long clk[8];  // shader-clock stamps of the restart phases (profiling)
+  int schur_path;    // 1: the last restart's Schur form came from perfect shifts
 };
 
 // Device buffers the orthogonalisation kernels use.
@@ -59,6 +60,7 @@ struct RestartArgs {
   // to TZ); 2: T and Z loaded from TZ, the order taken from the status (set
   // by the host's comparator), then the rest of the restart
   int phase = 0;
+  int schur_method = 1;  // 1: perfect shifts from the Aberth eigenvalues where possible
   double* TZ = nullptr;  // 4 m x m planes (T re, im, Z re, im), leading dimension m
                          // (always given when mact > kMaxM: the general kernel's T and Z)
 };

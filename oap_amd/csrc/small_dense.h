@@ -434,15 +434,18 @@ __device__ void shifted_train(Mat T, Mat Q, int kdim, int ns, const cd* mus) {
   }
 }
 
-__device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
-  int ihi = mm - 1, iter = 0, total = 0;
-  while (ihi >= 0) {
-    // largest l <= ihi with a negligible subdiagonal (0 if none)
+// The Schur iteration on the window [lo, ihi] of T (the rest of T already
+// reduced around it): deflation at the bottom, Wilkinson shifts.
+__device__ int schur_window(Mat T, Mat Z, int mm, int lo, int ihi, RotLog log) {
+  int iter = 0, total = 0;
+  while (ihi >= lo) {
+    // largest l in (lo, ihi] with a negligible subdiagonal (lo if none)
     const unsigned long long upto = ihi >= 63 ? ~0ull : ((2ull << ihi) - 1ull);
-    const unsigned long long neg = negligible_mask(T, mm) & upto;
-    const int l = neg ? 63 - __builtin_clzll(neg) : 0;
+    const unsigned long long above = lo >= 63 ? 0ull : ~((2ull << lo) - 1ull);
+    const unsigned long long neg = negligible_mask(T, mm) & upto & above;
+    const int l = neg ? 63 - __builtin_clzll(neg) : lo;
     bsync();
-    if (l > 0 && threadIdx.x == 0) T.set(l, l - 1, {0, 0});
+    if (l > lo && threadIdx.x == 0) T.set(l, l - 1, {0, 0});
     bsync();
     if (l == ihi) {
       --ihi;
@@ -479,6 +482,293 @@ __device__ int schur(Mat T, Mat Z, int mm, RotLog log) {
   }
   return 0;
 }
+
+__device__ __forceinline__ int schur(Mat T, Mat Z, int mm, RotLog log) {
+  return schur_window(T, Z, mm, 0, mm - 1, log);
+}
+
+// ---------------------------------------------------------------------------
+// Perfect-shift Schur form (m <= kAberthMaxM): the eigenvalues of the
+// Hessenberg H first, by the Ehrlich-Aberth simultaneous iteration on
+// det(H - zI) (Hyman's recurrence: with x_{m-1} = 1, row r of (H - zI) x = 0
+// gives x_{r-1}, and row 0 leaves q(z), det(H - zI) = c q(z); the
+// z-derivative runs beside it, and q'/q is all the iteration needs).  Then
+// the Schur iteration uses them as perfect shifts, smallest modulus first: a
+// train of eight of them over the active window splits its trailing 8 x 8
+// block off (|T[b][b-1]| at the rounding level of ||H||), whose eight
+// eigenvalues then deflate one per single perfect-shift sweep at the
+// bottom.  Where a split or a deflation does not happen (forward
+// instability) the Wilkinson iteration takes over that window; the last
+// window of up to twelve goes by single perfect shifts.  CPU emulation on
+// Arnoldi H of order 48 (C1 construction, first and later cycles): 430-655
+// train steps against 2,250-2,690 with Wilkinson pairs, eigenvalues within
+// 1e-14 of LAPACK; 11-15 Aberth iterations.
+constexpr int kAberthMaxM = 64;   // eight register slots of x, x' per wave
+constexpr int kAberthMaxIter = 64;
+
+// a / b with b scaled by its larger component (Smith's form: no overflow of
+// |b|^2 for the large values Hyman's recurrence can reach)
+__device__ __forceinline__ cd cdiv(cd a, cd b) {
+  if (fabs(b.re) >= fabs(b.im)) {
+    const double r = b.im / b.re, d = 1.0 / (b.re + b.im * r);
+    return {(a.re + a.im * r) * d, (a.im - a.re * r) * d};
+  }
+  const double r = b.re / b.im, d = 1.0 / (b.re * r + b.im);
+  return {(a.re * r + a.im) * d, (a.im * r - a.re) * d};
+}
+
+// |t_{i,i-1}| <= 8 eps ||H||_F (a backward-stable split) or the local test
+__device__ __forceinline__ bool split_ok(Mat T, int i, double hnorm) {
+  const double eps = 2.220446049250313e-16;
+  const cd sd = T.get(i, i - 1);
+  if (fabs(sd.re) + fabs(sd.im) <= 8.0 * eps * hnorm) return true;
+  const cd a = T.get(i - 1, i - 1), b = T.get(i, i);
+  const double d = fabs(a.re) + fabs(a.im) + fabs(b.re) + fabs(b.im);
+  const double sub = fabs(sd.re) + fabs(sd.im);
+  return sub <= eps * d || sub < 1e-300;
+}
+
+// All mm eigenvalues of H (in T, unreduced: every subdiagonal nonzero) into
+// z (LDS, mm entries), by all 8 waves of the block; lane i iterates root i.
+// Wave w owns the columns j = w + 8 q of the recurrence vectors and keeps
+// x_j, x'_j in registers (slot q); for each row r every wave adds its
+// columns' terms, the partial sums meet in LDS (part: 2 x 8 x 4 x 64
+// doubles, double-buffered by row parity) and the owner of column r - 1
+// forms x_{r-1}, x'_{r-1}.  Wave 0 then takes the Aberth step for every
+// root.  Returns (the same in every thread) whether all roots converged,
+// are finite, and sum to the trace.
+__device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, int* flags,
+                             cd* inv_sub) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const bool act = lane < mm;
+  // starting points: a circle around trace / mm with the RMS radius of the
+  // spectrum bound ||H - cI||_F / sqrt(mm) (wave 0)
+  cd tr = {0.0, 0.0};
+  for (int i = 0; i < mm; ++i) tr = cadd(tr, T.get(i, i));
+  const cd c0 = {tr.re / mm, tr.im / mm};
+  if (wave == 0) {
+    double f2 = 0.0;
+    for (int i = lane; i < mm * mm; i += 64) {
+      const int r = i / mm, cc = i - r * mm;
+      if (r <= cc + 1) {
+        cd t = T.get(r, cc);
+        if (r == cc) t = csub(t, c0);
+        f2 += cabs2(t);
+      }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) f2 += __shfl_xor(f2, off, 64);
+    const double rad = sqrt(f2 / mm) + 1e-300;
+    if (act) {
+      const double ang = 6.283185307179586 * lane / mm + 0.4;
+      z[lane] = {c0.re + rad * cos(ang), c0.im + rad * sin(ang)};
+    }
+    if (lane == 0) flags[0] = 0;  // 1: every root converged
+    // the reciprocal subdiagonals 1 / t_{r, r-1} (the recurrence divides by
+    // them every row)
+    for (int r = 1 + lane; r < mm; r += 64) inv_sub[r] = cdiv({1.0, 0.0}, T.get(r, r - 1));
+  }
+  __syncthreads();
+  bool done = !act;  // wave 0's view
+  double prev = 1e300;
+  for (int it = 0; it < kAberthMaxIter; ++it) {
+    // no early exit from the loop: the iterations left once every root has
+    // converged skip their work (an early exit here miscompiled for gfx950)
+    const bool all_done = flags[0] != 0;
+    const cd zi = act ? z[lane] : cd{0.0, 0.0};
+    double xr[8], xi[8], dr[8], di[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) { xr[q] = 0.0; xi[q] = 0.0; dr[q] = 0.0; di[q] = 0.0; }
+    if (!all_done && (mm - 1) % 8 == wave) {
+#pragma unroll
+      for (int q = 0; q < 8; ++q)
+        if (wave + 8 * q == mm - 1) xr[q] = 1.0;
+    }
+    cd q0 = {0.0, 0.0}, qd0 = {0.0, 0.0};
+    for (int r = mm - 1; r >= 0 && !all_done; --r) {
+      // this wave's columns of row r of (H - zI) against x and x'
+      cd s = {0.0, 0.0}, sd = {0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = wave + 8 * q;
+        if (j >= r && j < mm) {  // the same for the whole wave
+          cd h = T.get(r, j);
+          if (j == r) {
+            h = csub(h, zi);
+            sd = csub(sd, {xr[q], xi[q]});
+          }
+          s = cadd(s, cmul(h, {xr[q], xi[q]}));
+          sd = cadd(sd, cmul(h, {dr[q], di[q]}));
+        }
+      }
+      double* pb = part + (size_t)(r & 1) * (8 * 4 * 64) + (size_t)wave * (4 * 64);
+      pb[lane] = s.re;
+      pb[64 + lane] = s.im;
+      pb[128 + lane] = sd.re;
+      pb[192 + lane] = sd.im;
+      __syncthreads();
+      const int owner = r > 0 ? (r - 1) % 8 : 0;
+      if (wave == owner) {
+        const double* pr = part + (size_t)(r & 1) * (8 * 4 * 64);
+        cd S = {0.0, 0.0}, SD = {0.0, 0.0};
+#pragma unroll
+        for (int w = 0; w < 8; ++w) {
+          S = cadd(S, {pr[w * 256 + lane], pr[w * 256 + 64 + lane]});
+          SD = cadd(SD, {pr[w * 256 + 128 + lane], pr[w * 256 + 192 + lane]});
+        }
+        if (r == 0) {
+          q0 = S;
+          qd0 = SD;
+        } else {
+          const cd ih = inv_sub[r];
+          const cd xn = cmul(S, ih), dn = cmul(SD, ih);
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            if (wave + 8 * q == r - 1) {
+              xr[q] = -xn.re;
+              xi[q] = -xn.im;
+              dr[q] = -dn.re;
+              di[q] = -dn.im;
+            }
+          }
+        }
+      }
+    }
+    // wave 0: the Aberth step w = 1 / (q'/q - sum_{j != i} 1 / (z_i - z_j))
+    if (wave == 0 && !all_done) {
+      cd w = {0.0, 0.0};
+      bool hit = false;
+      if (!done) {
+        if (q0.re == 0.0 && q0.im == 0.0) {
+          hit = true;  // z is a root
+        } else {
+          const cd ratio = cdiv(qd0, q0);
+          cd sum = {0.0, 0.0};
+          for (int j = 0; j < mm; ++j) {
+            if (j == lane) continue;
+            // 1 / (z_i - z_j) = conj(d) / |d|^2 (the differences stay far
+            // from the overflow range)
+            const cd d = csub(zi, z[j]);
+            const double inv = 1.0 / cabs2(d);
+            sum = cadd(sum, {d.re * inv, -d.im * inv});
+          }
+          w = cdiv({1.0, 0.0}, csub(ratio, sum));
+        }
+      }
+      wsync();  // every lane has read z before any writes
+      if (!done) {
+        if (hit) {
+          done = true;
+        } else {
+          const cd zn = csub(zi, w);
+          z[lane] = zn;
+          const double rel = sqrt(cabs2(w)) / fmax(sqrt(cabs2(zn)), 1e-300 + 1e-16 * hnorm);
+          // converged, or stalled at the rounding level
+          if (rel <= 16.0 * 2.220446049250313e-16 || (rel < 1e-12 && rel > 0.5 * prev)) done = true;
+          prev = rel;
+        }
+      }
+      const bool left = __ballot(!done) != 0ull;
+      if (lane == 0 && !left) flags[0] = 1;
+    }
+    __syncthreads();
+  }
+  // every root converged, finite, and the roots sum to the trace (wave 0)
+  if (wave == 0) {
+    const cd zi = act ? z[lane] : cd{0.0, 0.0};
+    double sre = act ? zi.re : 0.0, sim = act ? zi.im : 0.0, sab = act ? sqrt(cabs2(zi)) : 0.0;
+    bool bad = !done || (act && !(fabs(zi.re) < 1e300 && fabs(zi.im) < 1e300));
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+      sre += __shfl_xor(sre, off, 64);
+      sim += __shfl_xor(sim, off, 64);
+      sab += __shfl_xor(sab, off, 64);
+    }
+    const bool trace_ok = hypot(sre - tr.re, sim - tr.im) <= 1e-12 * (sab + hnorm);
+    const bool ok = __ballot(bad) == 0ull && trace_ok;
+    if (lane == 0) flags[1] = ok ? 1 : 0;
+  }
+  __syncthreads();
+  return flags[1] != 0;
+}
+
+// Window [lo, hi] whose eigenvalues are the cnt (<= 16) values seeds: one
+// single perfect-shift sweep per eigenvalue (the unused seed nearest the
+// trailing diagonal entry), each deflating at the bottom; the Wilkinson
+// iteration finishes the window when a sweep does not deflate.
+__device__ int perfect_block(Mat T, Mat Z, int mm, int lo, int hi, const cd* seeds, int cnt,
+                             double hnorm, RotLog log) {
+  unsigned used = 0u;
+  while (hi > lo) {
+    bsync();
+    if (split_ok(T, hi, hnorm)) {
+      bsync();
+      if (threadIdx.x == 0) T.set(hi, hi - 1, {0.0, 0.0});
+      --hi;
+      continue;
+    }
+    const cd t = T.get(hi, hi);
+    int best = -1;
+    double bd = 0.0;
+    for (int i = 0; i < cnt; ++i) {
+      if ((used >> i) & 1u) continue;
+      const double d = cabs2(csub(seeds[i], t));
+      if (best < 0 || d < bd) { best = i; bd = d; }
+    }
+    if (best < 0) return schur_window(T, Z, mm, lo, hi, log);
+    bsync();
+    chase(T, Z, mm, lo, hi, seeds[best], log);
+    bsync();
+    if (split_ok(T, hi, hnorm)) {
+      used |= 1u << best;
+      bsync();
+      if (threadIdx.x == 0) T.set(hi, hi - 1, {0.0, 0.0});
+      --hi;
+    } else {
+      return schur_window(T, Z, mm, lo, hi, log);
+    }
+  }
+  bsync();
+  return 0;
+}
+
+// The Schur form from the perfect shifts seeds (all mm eigenvalues, ascending
+// modulus); block of >= 8 waves.
+__device__ int schur_peel(Mat T, Mat Z, int mm, const cd* seeds, double hnorm, RotLog log,
+                          int* peels = nullptr) {
+  const int wave = threadIdx.x >> 6;
+  int ihi = mm - 1, next = 0;
+  while (ihi > 0) {
+    bsync();
+    if (split_ok(T, ihi, hnorm)) {
+      bsync();
+      if (threadIdx.x == 0) T.set(ihi, ihi - 1, {0.0, 0.0});
+      --ihi;
+      continue;
+    }
+    // the unused seeds are the window's eigenvalues only while every
+    // deflation so far consumed one
+    const int remaining = mm - next;
+    if (remaining != ihi + 1) return schur_window(T, Z, mm, 0, ihi, log);
+    if (remaining <= 12) return perfect_block(T, Z, mm, 0, ihi, seeds + next, remaining, hnorm, log);
+    bsync();
+    chase_train(T, Z, mm, 0, ihi, 8, seeds[next + (wave < 8 ? wave : 0)]);
+    const int b = ihi - 7;
+    bsync();
+    if (!split_ok(T, b, hnorm)) return schur_window(T, Z, mm, 0, ihi, log);
+    bsync();
+    if (threadIdx.x == 0) T.set(b, b - 1, {0.0, 0.0});
+    const int f = perfect_block(T, Z, mm, b, ihi, seeds + next, 8, hnorm, log);
+    if (peels && threadIdx.x == 0) ++*peels;
+    if (f) return f;
+    next += 8;
+    ihi = b - 1;
+  }
+  bsync();
+  return 0;
+}
+
 
 __host__ __device__ constexpr int lds_ld(int m) { return m | 1; }
 
@@ -671,8 +961,15 @@ constexpr int kRestartLdsMax = 144 * 1024;
 __host__ __device__ constexpr bool x_in_lds(int m) {
   return 6 * m * lds_ld(m) * (int)sizeof(double) <= kRestartLdsMax;
 }
+// The Aberth iteration's partial sums (2 x 8 x 4 x 64 doubles) follow T
+// (in the Z area, set to the identity afterwards); for small m the launch
+// allocates beyond the restart's own need to hold them.
+constexpr int kAberthPartDoubles = 2 * 8 * 4 * 64;
 __host__ __device__ constexpr int restart_lds_bytes(int m) {
-  return (x_in_lds(m) ? 6 : 4) * m * lds_ld(m) * (int)sizeof(double);
+  return ((x_in_lds(m) ? 6 : 4) * m * lds_ld(m) > 2 * m * lds_ld(m) + kAberthPartDoubles
+              ? (x_in_lds(m) ? 6 : 4) * m * lds_ld(m)
+              : 2 * m * lds_ld(m) + kAberthPartDoubles) *
+         (int)sizeof(double);
 }
 
 __device__ bool ritz_before(cd a, cd b, int which) {
@@ -757,13 +1054,61 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
       double hr = r <= c + 1 ? a.Hre[(size_t)r * a.m + c] : 0.0;
       double hi = r <= c + 1 ? a.Him[(size_t)r * a.m + c] : 0.0;
       T.set(r, c, {hr, hi});
-      Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
+      if (GENERAL) Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
     }
     __syncthreads();
-    if (GENERAL)
+    if (GENERAL) {
       fail = schur_general(T, Z, mm, scr_s, a.Xre, reinterpret_cast<cd*>(a.Xre + 4 * kMaxMBig));
-    else
-      fail = schur(T, Z, mm, log);
+      if (lane == 0) st->schur_path = 0;
+    } else {
+      // perfect shifts where possible: the eigenvalues of the unreduced H by
+      // the Aberth iteration on wave 0, its scratch in the Z and X areas
+      // (Z is set to the identity afterwards)
+      __shared__ cd seeds_s[NM];
+      int peel = 0;
+      double hnorm = 0.0;
+      const bool try_peel = a.schur_method == 1 && mm <= kAberthMaxM && mm >= 3 && nthr >= 512;
+      if (try_peel) {
+        double f2 = 0.0;
+        for (int idx = lane; idx < mm * mm; idx += nthr) {
+          const int r = idx / mm, c = idx - r * mm;
+          if (r <= c + 1) f2 += cabs2(T.get(r, c));
+        }
+        tmax_s[lane] = f2;
+        __syncthreads();
+        if (lane == 0) {
+          double v = 0.0;
+          for (int t = 0; t < nthr; ++t) v += tmax_s[t];
+          tnorm_s = sqrt(v);
+        }
+        __syncthreads();
+        hnorm = tnorm_s;
+        if (negligible_mask(T, mm) == 0ull) {  // the same in every wave
+          __shared__ int aberth_flags[2];
+          peel = aberth_roots(T, mm, hnorm, ev, smem + 2 * mm * ld, aberth_flags, seeds_s) ? 1 : 0;
+        }
+        if (peel) {
+          // smallest modulus first (stable rank sort)
+          for (int i = lane; i < mm; i += nthr) {
+            const double mi = cabs2(ev[i]);
+            int rank = 0;
+            for (int j = 0; j < mm; ++j) {
+              const double mj = cabs2(ev[j]);
+              if (mj < mi || (mj == mi && j < i)) ++rank;
+            }
+            seeds_s[rank] = ev[i];
+          }
+        }
+        __syncthreads();
+      }
+      for (int idx = lane; idx < mm * mm; idx += nthr) {
+        int r = idx / mm, c = idx - r * mm;
+        Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
+      }
+      __syncthreads();
+      fail = peel ? schur_peel(T, Z, mm, seeds_s, hnorm, log) : schur(T, Z, mm, log);
+      if (lane == 0) st->schur_path = peel;
+    }
     __syncthreads();
   }
   if (lane == 0) st->clk[1] = clock64();

@@ -465,3 +465,48 @@ def test_bf16x3_projection(oracle, precision):
     check_residuals(s, A, ev, vecs, 8)
     with pytest.raises(OapError):
         s.setProjection(7)
+
+
+def _similarity_operator(n, seed):
+    # real nonsymmetric, eigenvalues (1..4)^2 spread by a near-identity
+    # similarity: a well-separated spectrum (the wanted set is never in doubt)
+    rng = np.random.default_rng(seed)
+    X = np.eye(n) + 0.05 * rng.standard_normal((n, n)) / np.sqrt(n)
+    return X @ np.diag(np.linspace(1.0, 4.0, n) ** 2) @ np.linalg.inv(X)
+
+
+@pytest.mark.parametrize("case,m,k", [("c1", 48, 8), ("similarity", 24, 6), ("c1", 54, 4)])
+def test_perfect_shift_schur_matches_wilkinson(oracle, case, m, k):
+    # the restart's Schur form from perfect shifts (the Aberth eigenvalues of
+    # H, eight per train, smallest modulus first) against Wilkinson shifts
+    # only: the same eigenpairs (LAPACK to 1e-10, residuals to 1e-10
+    # ||A||_F), and the perfect-shift path is the one that ran
+    if case == "c1":
+        n = 1024 if m == 48 else 600
+        ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+        A = ar + 1j * ai
+    else:
+        n = 800
+        A = _similarity_operator(n, 11).astype(complex)
+    want = top_k(np.linalg.eigvals(A), k, ArnUtils.SortLargestValues)
+    dA = DeviceMatrix.from_numpy(np.ascontiguousarray(A.real), np.ascontiguousarray(A.imag))
+    for method in (1, 0):
+        s = solver()
+        s.setMatrix(dA)
+        s.setSchurMethod(method)
+        re, im = np.zeros(k), np.zeros(k)
+        s.setOutputsEigenvalues(re, im)
+        vecs = [DeviceMatrix(1, n) for _ in range(k)]
+        s.setOutputsEigenvectors(vecs)
+        s.execute(m, k, MatrixInfo(True, True, n, n))
+        ev = re + 1j * im
+        print(f"method {method}: {s.stats()}", flush=True)
+        match(ev, want, 1e-10)
+        check_residuals(s, A, ev, vecs, k)
+        st = s.stats()
+        if method == 1:
+            assert st["restarts_perfect_shift"] >= 1, st
+        else:
+            assert st["restarts_perfect_shift"] == 0, st
+    with pytest.raises(OapError):
+        s.setSchurMethod(2)

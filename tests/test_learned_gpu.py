@@ -154,8 +154,9 @@ def test_shift_filter_closed_form():
 
     def T(x):
         x = np.asarray(x, dtype=np.float64)
+        # both branches are evaluated: each on arguments clipped to its domain
         return np.where(np.abs(x) <= 1, np.cos(d * np.arccos(np.clip(x, -1, 1))),
-                        np.sign(x) ** d * np.cosh(d * np.arccosh(np.abs(x))))
+                        np.sign(x) ** d * np.cosh(d * np.arccosh(np.maximum(np.abs(x), 1))))
 
     want = sum(T((lam(j, k) - cc) / e) / T((a0 - cc) / e) * vec(j, k) / np.linalg.norm(vec(j, k))
                for j, k in modes)
