@@ -355,6 +355,7 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     ra.which = s->sort;
     ra.do_shifts = 1;
     ra.schur_method = s->schur_method;
+    ra.warm = cycle > 0 ? 1 : 0;
     ra.TZ = s->TZ.as<double>();
     ra.tol = tol;
     ra.anorm = norm;

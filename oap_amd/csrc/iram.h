@@ -23,6 +23,7 @@ struct Status {
   int order[kMaxMBig];
   long long clk[8];  // shader-clock stamps of the restart phases (profiling)
   int schur_path;    // 1: the last restart's Schur form came from perfect shifts
+  int ev_count;      // Ritz values the last restart left in ev_re / ev_im
 };
 
 // Device buffers the orthogonalisation kernels use.
@@ -61,6 +62,7 @@ struct RestartArgs {
   // by the host's comparator), then the rest of the restart
   int phase = 0;
   int schur_method = 1;  // 1: perfect shifts from the Aberth eigenvalues where possible
+  int warm = 0;          // 1: an earlier restart of this factorisation left its Ritz values
   double* TZ = nullptr;  // 4 m x m planes (T re, im, Z re, im), leading dimension m
                          // (always given when mact > kMaxM: the general kernel's T and Z)
 };

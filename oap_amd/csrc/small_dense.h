@@ -538,7 +538,8 @@ __device__ __forceinline__ bool split_ok(Mat T, int i, double hnorm) {
 // root.  Returns (the same in every thread) whether all roots converged,
 // are finite, and sum to the trace.
 __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, int* flags,
-                             cd* inv_sub) {
+                             cd* inv_sub, const double* warm_re = nullptr,
+                             const double* warm_im = nullptr) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const bool act = lane < mm;
@@ -562,7 +563,13 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
     const double rad = sqrt(f2 / mm) + 1e-300;
     if (act) {
       const double ang = 6.283185307179586 * lane / mm + 0.4;
-      z[lane] = {c0.re + rad * cos(ang), c0.im + rad * sin(ang)};
+      if (warm_re) {
+        // the previous restart's Ritz values (6-8 iterations instead of
+        // 9-13 in emulation), kept apart by a tiny turn
+        z[lane] = {warm_re[lane] + 1e-8 * rad * cos(ang), warm_im[lane] + 1e-8 * rad * sin(ang)};
+      } else {
+        z[lane] = {c0.re + rad * cos(ang), c0.im + rad * sin(ang)};
+      }
     }
     if (lane == 0) flags[0] = 0;  // 1: every root converged
     // the reciprocal subdiagonals 1 / t_{r, r-1} (the recurrence divides by
@@ -572,6 +579,17 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
   __syncthreads();
   bool done = !act;  // wave 0's view
   double prev = 1e300;
+  // this wave's columns of H, one row per lane (H[r][j] for j = wave + 8 q in
+  // lane r, read back by readlane: the same value for every root)
+  double hre[8], him[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) {
+    const int j = wave + 8 * q;
+    const bool in = j < mm && lane < mm && lane <= j + 1;
+    const cd h = in ? T.get(lane, j) : cd{0.0, 0.0};
+    hre[q] = h.re;
+    him[q] = h.im;
+  }
   for (int it = 0; it < kAberthMaxIter; ++it) {
     // no early exit from the loop: the iterations left once every root has
     // converged skip their work (an early exit here miscompiled for gfx950)
@@ -593,7 +611,7 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
       for (int q = 0; q < 8; ++q) {
         const int j = wave + 8 * q;
         if (j >= r && j < mm) {  // the same for the whole wave
-          cd h = T.get(r, j);
+          cd h = {lane_read(hre[q], r), lane_read(him[q], r)};
           if (j == r) {
             h = csub(h, zi);
             sd = csub(sd, {xr[q], xi[q]});
@@ -1085,7 +1103,13 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
         hnorm = tnorm_s;
         if (negligible_mask(T, mm) == 0ull) {  // the same in every wave
           __shared__ int aberth_flags[2];
-          peel = aberth_roots(T, mm, hnorm, ev, smem + 2 * mm * ld, aberth_flags, seeds_s) ? 1 : 0;
+          // the previous restart of this factorisation left its Ritz values
+          // in the status: warm starting points
+          const bool warm = a.warm && st->ev_count == mm;
+          peel = aberth_roots(T, mm, hnorm, ev, smem + 2 * mm * ld, aberth_flags, seeds_s,
+                              warm ? st->ev_re : nullptr, warm ? st->ev_im : nullptr)
+                     ? 1
+                     : 0;
         }
         if (peel) {
           // smallest modulus first (stable rank sort)
@@ -1177,6 +1201,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
       for (int i = 0; i < mm; ++i) maxabs = fmax(maxabs, sqrt(cabs2(ev[i])));
       st->maxabs = maxabs;
       st->schur_fail = fail;
+      st->ev_count = mm;
     }
     for (int i = lane; i < mm; i += nthr) {
       st->ev_re[i] = ev[i].re;
@@ -1243,6 +1268,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
       st->est[t] = est[order[t]];
       st->order[t] = order[t];
     }
+    st->ev_count = mm;
   }
   __syncthreads();
   if (lane == 0) st->clk[3] = clock64();
