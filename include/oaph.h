@@ -449,14 +449,19 @@ oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* a, uint32_t guard);
 /* Weight of the guard Ritz vectors in a Ritz-sum restart (0..1, default
  * 1e-3; the wanted ones have weight 1).  No reference counterpart. */
 oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* a, double weight);
-/* How the restart forms the Schur form of H (extension, no reference
+/* How the restart finds the eigenpairs of H (extension, no reference
  * counterpart; the reference triangularises H by unshifted QR steps,
- * CuTriangularH.h).  1 (default): for hdim <= 54 the eigenvalues of H come
- * first (Ehrlich-Aberth iteration on det(H - zI) by Hyman's recurrence) and
- * serve as perfect shifts, eight per bulge train, smallest modulus first;
- * where that does not converge or a split does not happen the Wilkinson
- * iteration takes over.  0: Wilkinson shifts only.  Same Schur form to
- * rounding either way. */
+ * CuTriangularH.h).  For hdim <= 64 the eigenvalues of H come first
+ * (Ehrlich-Aberth iteration on det(H - zI) by Hyman's recurrence).
+ * 2 (default): the wanted eigenvectors of H by inverse iteration (LU of
+ * H - lambda I with partial pivoting, LAPACK zlaein's method), each accepted
+ * on its growth and an FP64 residual <= 64 hdim eps ||H||_F; no Schur form is
+ * formed.  1: the Aberth values serve as perfect shifts for the Schur form,
+ * eight per bulge train, smallest modulus first.  0: Wilkinson shifts only.
+ * Where a method does not succeed (no convergence, a vector not accepted, a
+ * split that does not happen) the next lower one takes over; with a
+ * comparator (oaph_arnoldi_set_sort_fn) method 2 runs as 1.  Same Ritz pairs
+ * to rounding either way. */
 oaph_status oaph_arnoldi_set_schur_method(oaph_arnoldi* a, int method);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
  * the largest Ritz value (default 1e-6) */
@@ -480,6 +485,9 @@ typedef struct oaph_arnoldi_stats {
                               solve ran FP64 cycles only */
   uint32_t restarts_perfect_shift; /* restarts whose Schur form came from the
                                       perfect shifts (oaph_arnoldi_set_schur_method) */
+  uint32_t restarts_inverse_iteration; /* restarts that took the Aberth eigenvalues
+                                          and inverse-iteration eigenvectors of H
+                                          (method 2) without a Schur form */
 } oaph_arnoldi_stats;
 oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
 

@@ -49,7 +49,8 @@ class oaph_arnoldi_stats(C.Structure):
     _fields_ = [("restarts", C.c_uint32), ("matvecs", C.c_uint32), ("matvecs_low", C.c_uint32),
                 ("nconv", C.c_uint32), ("seconds", C.c_double), ("max_resid", C.c_double),
                 ("fallbacks", C.c_uint32), ("mixed_declined", C.c_uint32),
-                ("restarts_perfect_shift", C.c_uint32)]
+                ("restarts_perfect_shift", C.c_uint32),
+                ("restarts_inverse_iteration", C.c_uint32)]
 
 
 class oaph_matrix_ex(C.Structure):

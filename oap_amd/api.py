@@ -614,8 +614,11 @@ class CuHArnoldiCallback:
               "setRestartGuardWeight")
 
     def setSchurMethod(self, method):
-        """1 (default): the restart's Schur form from perfect shifts (the
-        Aberth eigenvalues of H) where possible; 0: Wilkinson shifts only"""
+        """How the restart finds the eigenpairs of H.  2 (default): the
+        Aberth eigenvalues and the wanted eigenvectors by inverse iteration,
+        no Schur form where every vector is accepted; 1: the Schur form from
+        perfect shifts (the Aberth eigenvalues); 0: Wilkinson shifts only.
+        Each falls back to the next lower method where it does not succeed."""
         check(self._lib.oaph_arnoldi_set_schur_method(self._h, method), "setSchurMethod")
 
     def setMixedTolerance(self, tol_low):

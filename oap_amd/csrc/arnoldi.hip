@@ -61,7 +61,7 @@ struct oaph_arnoldi {
   // matvecs per solve at n = 8192 (weight 1: 220; 0.1: 182; 0.01: 177).
   uint32_t guard = 4;
   double guard_weight = 1e-3;
-  int schur_method = 1;  // restart's Schur form: 1 perfect shifts where possible, 0 Wilkinson
+  int schur_method = 2;  // restart: 2 Aberth values + inverse iteration, 1 perfect-shift Schur, 0 Wilkinson
   oaph_multiply_fn mfn = nullptr;
   void* muser = nullptr;
   oaph_check_fn cfn = nullptr;
@@ -380,6 +380,7 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
     s->stats.restarts++;
     const Status& hs = s->host_status;
     if (hs.schur_path == 1) s->stats.restarts_perfect_shift++;
+    if (hs.schur_path == 2) s->stats.restarts_inverse_iteration++;
     if (hs.schur_fail) {
       oaph::set_last_message("Hessenberg QR did not converge");
       return OAPH_ERR_NOT_CONVERGED;
@@ -635,7 +636,7 @@ oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* s, uint32_t guard) {
 }
 
 oaph_status oaph_arnoldi_set_schur_method(oaph_arnoldi* s, int method) {
-  OAPH_CHECK(s && (method == 0 || method == 1), OAPH_ERR_ARG);
+  OAPH_CHECK(s && method >= 0 && method <= 2, OAPH_ERR_ARG);
   s->schur_method = method;
   return OAPH_OK;
 }

@@ -22,7 +22,7 @@ struct Status {
   double ev_re[kMaxMBig], ev_im[kMaxMBig], est[kMaxMBig];  // Ritz values in wanted order, estimates
   int order[kMaxMBig];
   long long clk[8];  // shader-clock stamps of the restart phases (profiling)
-  int schur_path;    // 1: the last restart's Schur form came from perfect shifts
+  int schur_path;    // last restart: 2 Aberth values + inverse iteration, 1 perfect-shift Schur form, 0 Wilkinson
   int ev_count;      // Ritz values the last restart left in ev_re / ev_im
 };
 
@@ -61,7 +61,7 @@ struct RestartArgs {
   // to TZ); 2: T and Z loaded from TZ, the order taken from the status (set
   // by the host's comparator), then the rest of the restart
   int phase = 0;
-  int schur_method = 1;  // 1: perfect shifts from the Aberth eigenvalues where possible
+  int schur_method = 2;  // 2: Aberth eigenvalues + inverse iteration; 1: perfect-shift Schur form
   int warm = 0;          // 1: an earlier restart of this factorisation left its Ritz values
   double* TZ = nullptr;  // 4 m x m planes (T re, im, Z re, im), leading dimension m
                          // (always given when mact > kMaxM: the general kernel's T and Z)

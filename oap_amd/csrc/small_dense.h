@@ -711,6 +711,123 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
   return flags[1] != 0;
 }
 
+// Right eigenvectors of H (in T, unreduced upper Hessenberg, mm <= 64) for
+// the eigenvalues ev[order[t]], t < cnt, by inverse iteration (the method of
+// LAPACK's zlaein): Gaussian elimination of H - mu I with partial pivoting
+// between the carried row and the next row of H (a zero pivot becomes
+// eps3 = eps ||H||_F), then U x = v by back substitution with v = eps3 (1,
+// ..., 1) and, where x did not grow by zlaein's factor (||x||_1 >= 0.1 /
+// sqrt(mm)), two more of zlaein's starting vectors.  Wave w < nwe takes t =
+// w, w + nwe, ...: the lanes hold the columns of the carried row, the pivot
+// rows go to the wave's packed U in LDS (ubuf: mm (mm + 1) / 2 entries per
+// wave, row j from column j on) and the back substitution runs by columns
+// (lane r accumulates row r).  A vector is accepted when it grew and its
+// residual ||(H - mu I) y||_2 <= 64 mm eps ||H||_F.  Writes y / ||y|| to
+// column t of Y (leading dimension ky, rows < mm) and the Ritz estimate
+// beta |y_{mm-1}| to est[order[t]].  Returns (the same in every thread)
+// whether every vector was accepted.
+__device__ bool inverse_vectors(Mat T, int mm, double hnorm, const cd* ev, const int* order,
+                                int cnt, cd* ubuf, int nwe, double* Yre, double* Yim, int ky,
+                                double beta, double* est, int* flag) {
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const double eps = 2.220446049250313e-16;
+  const double eps3 = fmax(eps * hnorm, 1e-300);
+  const double rootn = sqrt((double)mm);
+  if (threadIdx.x == 0) *flag = 0;
+  __syncthreads();
+  const bool act = lane < mm;
+  cd* U = ubuf + (size_t)wave * (mm * (mm + 1) / 2);
+  const int roff = lane * mm - lane * (lane - 1) / 2;  // packed start of row `lane`
+  bool bad = false;
+  for (int t = wave; wave < nwe && t < cnt; t += nwe) {
+    const cd mu = ev[order[t]];
+    // carried row: row 0 of H - mu I
+    cd w = act ? T.get(0, lane) : cd{0.0, 0.0};
+    if (lane == 0) w = csub(w, mu);
+    cd invd = {0.0, 0.0};  // lane j: 1 / U[j][j]
+    int off = 0;           // packed start of row j
+    for (int j = 0; j < mm - 1; ++j) {
+      cd h = (act && lane >= j) ? T.get(j + 1, lane) : cd{0.0, 0.0};
+      if (lane == j + 1) h = csub(h, mu);
+      const cd p = lane_read(w, j);
+      const cd q = T.get(j + 1, j);
+      const bool swap = cabs2(q) > cabs2(p);  // the same in every lane
+      cd piv = swap ? q : p;
+      if (piv.re == 0.0 && piv.im == 0.0) piv = {eps3, 0.0};
+      const cd ip = cdiv({1.0, 0.0}, piv);
+      const cd l = cmul(swap ? p : q, ip);
+      const cd prow = swap ? h : w;  // pivot row
+      const cd orow = swap ? w : h;
+      if (act && lane >= j) U[off + lane - j] = lane == j ? piv : prow;
+      if (lane == j) invd = ip;
+      w = csub(orow, cmul(l, prow));
+      off += mm - j;
+    }
+    {
+      cd piv = lane_read(w, mm - 1);
+      if (piv.re == 0.0 && piv.im == 0.0) piv = {eps3, 0.0};
+      if (lane == mm - 1) {
+        U[off] = piv;
+        invd = cdiv({1.0, 0.0}, piv);
+      }
+    }
+    wsync();
+    cd x = {0.0, 0.0};
+    bool grew = false;
+    for (int its = 0; its < 3 && !grew; ++its) {
+      // zlaein's starting vectors: eps3 (1, ..., 1), then eps3 / (sqrt(mm) + 1)
+      // everywhere but eps3 first, minus eps3 sqrt(mm) at position mm - its
+      double v = eps3;
+      if (its > 0) {
+        v = lane == 0 ? eps3 : eps3 / (rootn + 1.0);
+        if (lane == mm - its) v -= eps3 * rootn;
+      }
+      cd acc = {act ? v : 0.0, 0.0};
+      for (int i = mm - 1; i >= 0; --i) {
+        const cd xi = lane_read(cmul(acc, invd), i);
+        if (lane == i) x = xi;
+        if (lane < i) acc = csub(acc, cmul(U[roff + i - lane], xi));
+      }
+      double n1 = act ? fabs(x.re) + fabs(x.im) : 0.0;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o, 64);
+      n1 = lane_read(n1, 0);  // lanes may differ in the last bit: lane 0 decides
+      grew = n1 >= 0.1 / rootn && n1 < 1e300;
+    }
+    double n2 = act ? cabs2(x) : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
+    n2 = lane_read(n2, 0);
+    const double inv = n2 > 0.0 ? 1.0 / sqrt(n2) : 0.0;
+    const cd y = {x.re * inv, x.im * inv};
+    // residual (H - mu I) y, y broadcast through the wave's U area
+    wsync();
+    if (act) U[lane] = y;
+    wsync();
+    cd rr = {0.0, 0.0};
+    if (act) {
+      for (int c = lane > 0 ? lane - 1 : 0; c < mm; ++c) rr = cadd(rr, cmul(T.get(lane, c), U[c]));
+      rr = csub(rr, cmul(mu, y));
+    }
+    double r2 = act ? cabs2(rr) : 0.0;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) r2 += __shfl_xor(r2, o, 64);
+    r2 = lane_read(r2, 0);
+    const double rtol = 64.0 * mm * eps * hnorm;
+    if (!grew || !(r2 <= rtol * rtol) || !(n2 > 0.0)) bad = true;
+    if (act) {
+      Yre[(size_t)lane * ky + t] = y.re;
+      Yim[(size_t)lane * ky + t] = y.im;
+    }
+    if (lane == mm - 1) est[order[t]] = beta * sqrt(cabs2(y));
+    wsync();  // the residual's reads of U before the next vector's stores
+  }
+  if (bad && lane == 0) *flag = 1;
+  __syncthreads();
+  return *flag == 0;
+}
+
 // Window [lo, hi] whose eigenvalues are the cnt (<= 16) values seeds: one
 // single perfect-shift sweep per eigenvalue (the unused seed nearest the
 // trailing diagonal entry), each deflating at the bottom; the Wilkinson
@@ -1056,6 +1173,9 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
   __shared__ cd log_s[NM];
   const RotLog log{log_c, log_s};
   int fail = 0;
+  // method 2: the Aberth eigenvalues and inverse-iteration eigenvectors took
+  // the place of the Schur form (the same in every thread)
+  bool inv = false;
   if (a.phase == 2) {
     // the Schur form of the first launch (what of it lives in LDS)
     for (int idx = lane; idx < mm * mm; idx += nthr) {
@@ -1085,7 +1205,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
       __shared__ cd seeds_s[NM];
       int peel = 0;
       double hnorm = 0.0;
-      const bool try_peel = a.schur_method == 1 && mm <= kAberthMaxM && mm >= 3 && nthr >= 512;
+      const bool try_peel = a.schur_method >= 1 && mm <= kAberthMaxM && mm >= 3 && nthr >= 512;
       if (try_peel) {
         double f2 = 0.0;
         for (int idx = lane; idx < mm * mm; idx += nthr) {
@@ -1125,36 +1245,43 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
         }
         __syncthreads();
       }
-      for (int idx = lane; idx < mm * mm; idx += nthr) {
-        int r = idx / mm, c = idx - r * mm;
-        Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
+      if (peel && a.schur_method == 2 && a.phase == 0) {
+        // the Aberth values in wanted order, then the wanted eigenvectors of
+        // H by inverse iteration, U factors in the Z and X areas; the Schur
+        // form is formed only where a vector is not accepted
+        for (int i = lane; i < mm; i += nthr) {
+          int rank = 0;
+          for (int j = 0; j < mm; ++j) {
+            const bool bj = ritz_before(ev[j], ev[i], a.which);
+            const bool bi = ritz_before(ev[i], ev[j], a.which);
+            if (bj || (!bi && j < i)) ++rank;
+          }
+          order[rank] = i;
+          est[i] = INFINITY;  // estimates are formed for the ky wanted ones only
+        }
+        __syncthreads();
+        const int avail = restart_lds_bytes(a.mact) / (int)sizeof(double) - 2 * mm * ld;
+        int nwe = avail / (mm * (mm + 1));
+        if (nwe > nthr / 64) nwe = nthr / 64;
+        const int cnt = a.ky < mm ? a.ky : mm;
+        if (nwe >= 1)
+          inv = inverse_vectors(T, mm, hnorm, ev, order, cnt,
+                                reinterpret_cast<cd*>(smem + 2 * mm * ld), nwe, a.Yre, a.Yim,
+                                a.ky, st->beta, est, scr_s);
       }
-      __syncthreads();
-      fail = peel ? schur_peel(T, Z, mm, seeds_s, hnorm, log) : schur(T, Z, mm, log);
-      if (lane == 0) st->schur_path = peel;
+      if (!inv) {
+        for (int idx = lane; idx < mm * mm; idx += nthr) {
+          int r = idx / mm, c = idx - r * mm;
+          Z.set(r, c, {r == c ? 1.0 : 0.0, 0.0});
+        }
+        __syncthreads();
+        fail = peel ? schur_peel(T, Z, mm, seeds_s, hnorm, log) : schur(T, Z, mm, log);
+      }
+      if (lane == 0) st->schur_path = inv ? 2 : peel;
     }
     __syncthreads();
   }
   if (lane == 0) st->clk[1] = clock64();
-  for (int i = lane; i < mm; i += nthr) ev[i] = T.get(i, i);
-  // ||T|| (max modulus of the upper triangle) for the trevc safeguard
-  double tm = 0;
-  for (int idx = lane; idx < mm * mm; idx += nthr) {
-    int r = idx / mm, c = idx - r * mm;
-    if (c >= r) {
-      const cd t = T.get(r, c);
-      tm = fmax(tm, hypot(t.re, t.im));
-    }
-  }
-  tmax_s[lane] = tm;
-  __syncthreads();
-  if (lane == 0) {
-    double v = 0;
-    for (int t = 0; t < nthr; ++t) v = fmax(v, tmax_s[t]);
-    tnorm_s = v;
-  }
-  __syncthreads();
-  const double small = fmax(tnorm_s * 2.220446049250313e-16, 1e-300);
   // eigenvectors of T by back substitution: lane i owns column i of X
   // X in LDS when it fits in what the launch allocated for a.mact (each
   // lane's dependent back substitution then reads LDS, not L2), else the
@@ -1163,34 +1290,55 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
   double* Xre = xl ? smem + 4 * mm * ld : a.Xre;
   double* Xim = xl ? smem + 5 * mm * ld : a.Xim;
   const size_t xld = xl ? (size_t)ld : (size_t)a.m;
-  const double beta = st->beta;
-  for (int i = lane; i < mm; i += nthr) {
-    cd li = ev[i];
-    for (int j = 0; j < mm; ++j) {
-      Xre[j * xld + i] = j == i ? 1.0 : 0.0;
-      Xim[j * xld + i] = 0.0;
-    }
-    for (int j = i - 1; j >= 0; --j) {
-      cd s = {0, 0};
-      for (int l = j + 1; l <= i; ++l) {
-        cd x = {Xre[l * xld + i], Xim[l * xld + i]};
-        s = cadd(s, cmul(T.get(j, l), x));
+  if (!inv) {
+    for (int i = lane; i < mm; i += nthr) ev[i] = T.get(i, i);
+    // ||T|| (max modulus of the upper triangle) for the trevc safeguard
+    double tm = 0;
+    for (int idx = lane; idx < mm * mm; idx += nthr) {
+      int r = idx / mm, c = idx - r * mm;
+      if (c >= r) {
+        const cd t = T.get(r, c);
+        tm = fmax(tm, hypot(t.re, t.im));
       }
-      cd d = csub(T.get(j, j), li);
-      if (hypot(d.re, d.im) < small) d = {small, 0.0};
-      double den = cabs2(d);
-      Xre[j * xld + i] = -(s.re * d.re + s.im * d.im) / den;
-      Xim[j * xld + i] = -(s.im * d.re - s.re * d.im) / den;
     }
-    // y = Z x: ||y|| = ||x|| (Z unitary); residual estimate beta |y[mm-1]| / ||y||
-    double nx = 0;
-    cd ylast = {0, 0};
-    for (int l = 0; l <= i; ++l) {
-      cd x = {Xre[l * xld + i], Xim[l * xld + i]};
-      nx += cabs2(x);
-      ylast = cadd(ylast, cmul(Z.get(mm - 1, l), x));
+    tmax_s[lane] = tm;
+    __syncthreads();
+    if (lane == 0) {
+      double v = 0;
+      for (int t = 0; t < nthr; ++t) v = fmax(v, tmax_s[t]);
+      tnorm_s = v;
     }
-    est[i] = beta * hypot(ylast.re, ylast.im) / sqrt(nx);
+    __syncthreads();
+    const double small = fmax(tnorm_s * 2.220446049250313e-16, 1e-300);
+    const double beta = st->beta;
+    for (int i = lane; i < mm; i += nthr) {
+      cd li = ev[i];
+      for (int j = 0; j < mm; ++j) {
+        Xre[j * xld + i] = j == i ? 1.0 : 0.0;
+        Xim[j * xld + i] = 0.0;
+      }
+      for (int j = i - 1; j >= 0; --j) {
+        cd s = {0, 0};
+        for (int l = j + 1; l <= i; ++l) {
+          cd x = {Xre[l * xld + i], Xim[l * xld + i]};
+          s = cadd(s, cmul(T.get(j, l), x));
+        }
+        cd d = csub(T.get(j, j), li);
+        if (hypot(d.re, d.im) < small) d = {small, 0.0};
+        double den = cabs2(d);
+        Xre[j * xld + i] = -(s.re * d.re + s.im * d.im) / den;
+        Xim[j * xld + i] = -(s.im * d.re - s.re * d.im) / den;
+      }
+      // y = Z x: ||y|| = ||x|| (Z unitary); residual estimate beta |y[mm-1]| / ||y||
+      double nx = 0;
+      cd ylast = {0, 0};
+      for (int l = 0; l <= i; ++l) {
+        cd x = {Xre[l * xld + i], Xim[l * xld + i]};
+        nx += cabs2(x);
+        ylast = cadd(ylast, cmul(Z.get(mm - 1, l), x));
+      }
+      est[i] = beta * hypot(ylast.re, ylast.im) / sqrt(nx);
+    }
   }
   __syncthreads();
   if (lane == 0) st->clk[2] = clock64();
@@ -1227,7 +1375,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
   }
   if (a.phase == 2) {
     for (int i = lane; i < mm; i += nthr) order[i] = st->order[i];
-  } else {
+  } else if (!inv) {
     // stable rank sort by the wanted order
     for (int i = lane; i < mm; i += nthr) {
       int rank = 0;
@@ -1273,7 +1421,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
   __syncthreads();
   if (lane == 0) st->clk[3] = clock64();
   // Ritz vectors of H in wanted order: Y[:, t] = Z x_{order[t]} / ||x||, t < ky
-  for (int t = 0; t < a.ky && t < mm; ++t) {
+  for (int t = 0; !inv && t < a.ky && t < mm; ++t) {
     int i = order[t];
     double nx = 0;
     for (int l = 0; l <= i; ++l)

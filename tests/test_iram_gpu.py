@@ -475,12 +475,14 @@ def _similarity_operator(n, seed):
     return X @ np.diag(np.linspace(1.0, 4.0, n) ** 2) @ np.linalg.inv(X)
 
 
-@pytest.mark.parametrize("case,m,k", [("c1", 48, 8), ("similarity", 24, 6), ("c1", 54, 4)])
-def test_perfect_shift_schur_matches_wilkinson(oracle, case, m, k):
-    # the restart's Schur form from perfect shifts (the Aberth eigenvalues of
-    # H, eight per train, smallest modulus first) against Wilkinson shifts
-    # only: the same eigenpairs (LAPACK to 1e-10, residuals to 1e-10
-    # ||A||_F), and the perfect-shift path is the one that ran
+@pytest.mark.parametrize("case,m,k", [("c1", 48, 8), ("similarity", 24, 6), ("c1", 54, 4),
+                                      ("c1", 64, 12)])
+def test_restart_methods_agree(oracle, case, m, k):
+    # the three ways the restart finds the eigenpairs of H: the Aberth
+    # eigenvalues with inverse-iteration eigenvectors (2, default), the Schur
+    # form from perfect shifts (1), Wilkinson shifts only (0): the same
+    # eigenpairs (LAPACK to 1e-10, residuals to 1e-10 ||A||_F), and the path
+    # asked for is the one that ran
     if case == "c1":
         n = 1024 if m == 48 else 600
         ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
@@ -490,7 +492,7 @@ def test_perfect_shift_schur_matches_wilkinson(oracle, case, m, k):
         A = _similarity_operator(n, 11).astype(complex)
     want = top_k(np.linalg.eigvals(A), k, ArnUtils.SortLargestValues)
     dA = DeviceMatrix.from_numpy(np.ascontiguousarray(A.real), np.ascontiguousarray(A.imag))
-    for method in (1, 0):
+    for method in (2, 1, 0):
         s = solver()
         s.setMatrix(dA)
         s.setSchurMethod(method)
@@ -504,9 +506,14 @@ def test_perfect_shift_schur_matches_wilkinson(oracle, case, m, k):
         match(ev, want, 1e-10)
         check_residuals(s, A, ev, vecs, k)
         st = s.stats()
-        if method == 1:
+        if method == 2:
+            # every restart but ones whose vectors were not accepted
+            assert st["restarts_inverse_iteration"] >= st["restarts"] - 1, st
+        elif method == 1:
             assert st["restarts_perfect_shift"] >= 1, st
+            assert st["restarts_inverse_iteration"] == 0, st
         else:
             assert st["restarts_perfect_shift"] == 0, st
+            assert st["restarts_inverse_iteration"] == 0, st
     with pytest.raises(OapError):
-        s.setSchurMethod(2)
+        s.setSchurMethod(3)
