@@ -711,9 +711,11 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
   return flags[1] != 0;
 }
 
-// Right eigenvectors of H (in T, unreduced upper Hessenberg, mm <= 64) for
-// the eigenvalues ev[order[t]], t < cnt, by inverse iteration (the method of
-// LAPACK's zlaein): Gaussian elimination of H - mu I with partial pivoting
+// Right eigenvectors of H (in T, upper Hessenberg, mm <= 64) for the
+// eigenvalues ev[order[t]], t < cnt, by inverse iteration (the method of
+// LAPACK's zlaein, on the leading rows and columns 0..bend[i] of H, bend[i]
+// the last row of the unreduced block that holds eigenvalue i, as zhsein
+// does; the vector is zero below): Gaussian elimination of H - mu I with partial pivoting
 // between the carried row and the next row of H (a zero pivot becomes
 // eps3 = eps ||H||_F), then U x = v by back substitution with v = eps3 (1,
 // ..., 1) and, where x did not grow by zlaein's factor (||x||_1 >= 0.1 /
@@ -727,28 +729,30 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
 // beta |y_{mm-1}| to est[order[t]].  Returns (the same in every thread)
 // whether every vector was accepted.
 __device__ bool inverse_vectors(Mat T, int mm, double hnorm, const cd* ev, const int* order,
-                                int cnt, cd* ubuf, int nwe, double* Yre, double* Yim, int ky,
-                                double beta, double* est, int* flag) {
+                                const int* bend, int cnt, cd* ubuf, int nwe, double* Yre,
+                                double* Yim, int ky, double beta, double* est, int* flag) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const double eps = 2.220446049250313e-16;
   const double eps3 = fmax(eps * hnorm, 1e-300);
-  const double rootn = sqrt((double)mm);
   if (threadIdx.x == 0) *flag = 0;
   __syncthreads();
   const bool act = lane < mm;
   cd* U = ubuf + (size_t)wave * (mm * (mm + 1) / 2);
-  const int roff = lane * mm - lane * (lane - 1) / 2;  // packed start of row `lane`
   bool bad = false;
   for (int t = wave; wave < nwe && t < cnt; t += nwe) {
     const cd mu = ev[order[t]];
+    const int ms = __builtin_amdgcn_readfirstlane(bend[order[t]] + 1);  // leading block
+    const bool in = lane < ms;
+    const double rootn = sqrt((double)ms);
+    const int roff = lane * ms - lane * (lane - 1) / 2;  // packed start of row `lane`
     // carried row: row 0 of H - mu I
-    cd w = act ? T.get(0, lane) : cd{0.0, 0.0};
+    cd w = in ? T.get(0, lane) : cd{0.0, 0.0};
     if (lane == 0) w = csub(w, mu);
     cd invd = {0.0, 0.0};  // lane j: 1 / U[j][j]
     int off = 0;           // packed start of row j
-    for (int j = 0; j < mm - 1; ++j) {
-      cd h = (act && lane >= j) ? T.get(j + 1, lane) : cd{0.0, 0.0};
+    for (int j = 0; j < ms - 1; ++j) {
+      cd h = (in && lane >= j) ? T.get(j + 1, lane) : cd{0.0, 0.0};
       if (lane == j + 1) h = csub(h, mu);
       const cd p = lane_read(w, j);
       const cd q = T.get(j + 1, j);
@@ -759,15 +763,15 @@ __device__ bool inverse_vectors(Mat T, int mm, double hnorm, const cd* ev, const
       const cd l = cmul(swap ? p : q, ip);
       const cd prow = swap ? h : w;  // pivot row
       const cd orow = swap ? w : h;
-      if (act && lane >= j) U[off + lane - j] = lane == j ? piv : prow;
+      if (in && lane >= j) U[off + lane - j] = lane == j ? piv : prow;
       if (lane == j) invd = ip;
       w = csub(orow, cmul(l, prow));
-      off += mm - j;
+      off += ms - j;
     }
     {
-      cd piv = lane_read(w, mm - 1);
+      cd piv = lane_read(w, ms - 1);
       if (piv.re == 0.0 && piv.im == 0.0) piv = {eps3, 0.0};
-      if (lane == mm - 1) {
+      if (lane == ms - 1) {
         U[off] = piv;
         invd = cdiv({1.0, 0.0}, piv);
       }
@@ -776,26 +780,26 @@ __device__ bool inverse_vectors(Mat T, int mm, double hnorm, const cd* ev, const
     cd x = {0.0, 0.0};
     bool grew = false;
     for (int its = 0; its < 3 && !grew; ++its) {
-      // zlaein's starting vectors: eps3 (1, ..., 1), then eps3 / (sqrt(mm) + 1)
-      // everywhere but eps3 first, minus eps3 sqrt(mm) at position mm - its
+      // zlaein's starting vectors: eps3 (1, ..., 1), then eps3 / (sqrt(ms) + 1)
+      // everywhere but eps3 first, minus eps3 sqrt(ms) at position ms - its
       double v = eps3;
       if (its > 0) {
         v = lane == 0 ? eps3 : eps3 / (rootn + 1.0);
-        if (lane == mm - its) v -= eps3 * rootn;
+        if (lane == ms - its) v -= eps3 * rootn;
       }
-      cd acc = {act ? v : 0.0, 0.0};
-      for (int i = mm - 1; i >= 0; --i) {
+      cd acc = {in ? v : 0.0, 0.0};
+      for (int i = ms - 1; i >= 0; --i) {
         const cd xi = lane_read(cmul(acc, invd), i);
         if (lane == i) x = xi;
         if (lane < i) acc = csub(acc, cmul(U[roff + i - lane], xi));
       }
-      double n1 = act ? fabs(x.re) + fabs(x.im) : 0.0;
+      double n1 = in ? fabs(x.re) + fabs(x.im) : 0.0;
 #pragma unroll
       for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o, 64);
       n1 = lane_read(n1, 0);  // lanes may differ in the last bit: lane 0 decides
       grew = n1 >= 0.1 / rootn && n1 < 1e300;
     }
-    double n2 = act ? cabs2(x) : 0.0;
+    double n2 = in ? cabs2(x) : 0.0;
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) n2 += __shfl_xor(n2, o, 64);
     n2 = lane_read(n2, 0);
@@ -1203,7 +1207,9 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
       // the Aberth iteration on wave 0, its scratch in the Z and X areas
       // (Z is set to the identity afterwards)
       __shared__ cd seeds_s[NM];
+      __shared__ int bend_s[NM];  // last row of the unreduced block of each eigenvalue
       int peel = 0;
+      bool split_vals = false;  // eigenvalues of a split H, block by block
       double hnorm = 0.0;
       const bool try_peel = a.schur_method >= 1 && mm <= kAberthMaxM && mm >= 3 && nthr >= 512;
       if (try_peel) {
@@ -1221,8 +1227,9 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
         }
         __syncthreads();
         hnorm = tnorm_s;
-        if (negligible_mask(T, mm) == 0ull) {  // the same in every wave
-          __shared__ int aberth_flags[2];
+        __shared__ int aberth_flags[2];
+        const unsigned long long negl = negligible_mask(T, mm);  // the same in every wave
+        if (negl == 0ull) {
           // the previous restart of this factorisation left its Ritz values
           // in the status: warm starting points
           const bool warm = a.warm && st->ev_count == mm;
@@ -1230,6 +1237,45 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
                               warm ? st->ev_re : nullptr, warm ? st->ev_im : nullptr)
                      ? 1
                      : 0;
+          for (int i = lane; i < mm; i += nthr) bend_s[i] = mm - 1;
+        } else if (a.schur_method == 2 && a.phase == 0) {
+          // H splits at its negligible subdiagonals (set to zero, as the
+          // Schur iteration does): the eigenvalues block by block, Aberth
+          // for blocks of three or more, closed forms below
+          if (lane < mm && lane >= 1 && ((negl >> lane) & 1ull)) T.set(lane, lane - 1, {0.0, 0.0});
+          __syncthreads();
+          bool ok = true;
+          int lo = 0;
+          while (lo < mm) {  // the same blocks in every thread
+            int hi = lo;
+            while (hi + 1 < mm && !((negl >> (hi + 1)) & 1ull)) ++hi;
+            const int sz = hi - lo + 1;
+            if (sz >= 3) {
+              const Mat W{T.re + lo * ld + lo, T.im + lo * ld + lo, ld};
+              ok = aberth_roots(W, sz, hnorm, ev + lo, smem + 2 * mm * ld, aberth_flags,
+                                seeds_s + lo) && ok;
+            } else if (lane == 0) {
+              if (sz == 1) {
+                ev[lo] = T.get(lo, lo);
+              } else {
+                // (a + d) / 2 +- sqrt(((a - d) / 2)^2 + b c)
+                const cd aa = T.get(lo, lo), b = T.get(lo, hi), c = T.get(hi, lo),
+                         d = T.get(hi, hi);
+                const cd h = {0.5 * (aa.re - d.re), 0.5 * (aa.im - d.im)};
+                const cd q = cadd(cmul(h, h), cmul(b, c));
+                const double mod = hypot(q.re, q.im);
+                double sr = sqrt(0.5 * (mod + q.re)), si = sqrt(0.5 * (mod - q.re));
+                if (q.im < 0) si = -si;
+                const cd mid = {0.5 * (aa.re + d.re), 0.5 * (aa.im + d.im)};
+                ev[lo] = {mid.re + sr, mid.im + si};
+                ev[hi] = {mid.re - sr, mid.im - si};
+              }
+            }
+            for (int i = lo + lane; i <= hi; i += nthr) bend_s[i] = hi;
+            __syncthreads();
+            lo = hi + 1;
+          }
+          split_vals = ok;
         }
         if (peel) {
           // smallest modulus first (stable rank sort)
@@ -1245,7 +1291,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
         }
         __syncthreads();
       }
-      if (peel && a.schur_method == 2 && a.phase == 0) {
+      if ((peel || split_vals) && a.schur_method == 2 && a.phase == 0) {
         // the Aberth values in wanted order, then the wanted eigenvectors of
         // H by inverse iteration, U factors in the Z and X areas; the Schur
         // form is formed only where a vector is not accepted
@@ -1265,7 +1311,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
         if (nwe > nthr / 64) nwe = nthr / 64;
         const int cnt = a.ky < mm ? a.ky : mm;
         if (nwe >= 1)
-          inv = inverse_vectors(T, mm, hnorm, ev, order, cnt,
+          inv = inverse_vectors(T, mm, hnorm, ev, order, bend_s, cnt,
                                 reinterpret_cast<cd*>(smem + 2 * mm * ld), nwe, a.Yre, a.Yim,
                                 a.ky, st->beta, est, scr_s);
       }
