@@ -528,18 +528,11 @@ __device__ __forceinline__ bool split_ok(Mat T, int i, double hnorm) {
   return sub <= eps * d || sub < 1e-300;
 }
 
-// All mm eigenvalues of H (in T, unreduced: every subdiagonal nonzero) into
-// z (LDS, mm entries), by all 8 waves of the block; lane i iterates root i.
-// Wave w owns the columns j = w + 8 q of the recurrence vectors and keeps
-// x_j, x'_j in registers (slot q); for each row r every wave adds its
-// columns' terms, the partial sums meet in LDS (part: 2 x 8 x 4 x 64
-// doubles, double-buffered by row parity) and the owner of column r - 1
-// forms x_{r-1}, x'_{r-1}.  Wave 0 then takes the Aberth step for every
-// root.  Returns (the same in every thread) whether all roots converged,
-// are finite, and sum to the trace.
-__device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, int* flags,
-                             cd* inv_sub, const double* warm_re = nullptr,
-                             const double* warm_im = nullptr) {
+// The Aberth iteration of aberth_roots (below) for BS rows per block.
+constexpr int kAbBlocks = 16;  // row blocks of the recurrence (two per wave)
+template <int BS>  // rows per block: ceil(mm / 16)
+__device__ bool aberth_roots_t(Mat T, int mm, double hnorm, cd* z, double* part, int* flags,
+                               cd* inv_sub, const double* warm_re, const double* warm_im) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
   const bool act = lane < mm;
@@ -571,7 +564,10 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
         z[lane] = {c0.re + rad * cos(ang), c0.im + rad * sin(ang)};
       }
     }
-    if (lane == 0) flags[0] = 0;  // 1: every root converged
+    if (lane == 0) {
+      flags[0] = 0;  // 1: every root converged
+      flags[2] = 0;  // iterations taken
+    }
     // the reciprocal subdiagonals 1 / t_{r, r-1} (the recurrence divides by
     // them every row)
     for (int r = 1 + lane; r < mm; r += 64) inv_sub[r] = cdiv({1.0, 0.0}, T.get(r, r - 1));
@@ -579,77 +575,125 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
   __syncthreads();
   bool done = !act;  // wave 0's view
   double prev = 1e300;
-  // this wave's columns of H, one row per lane (H[r][j] for j = wave + 8 q in
-  // lane r, read back by readlane: the same value for every root)
-  double hre[8], him[8];
+  const int nb = (mm + BS - 1) / BS;
+  // this wave's rows of H, lane = column: block K = wave + 8 s, row mm - 1 - K BS - i
+  double hre[2][BS], him[2][BS];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int j = wave + 8 * q;
-    const bool in = j < mm && lane < mm && lane <= j + 1;
-    const cd h = in ? T.get(lane, j) : cd{0.0, 0.0};
-    hre[q] = h.re;
-    him[q] = h.im;
-  }
+  for (int s = 0; s < 2; ++s)
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+      const int K = wave + 8 * s, r = mm - 1 - K * BS - i;
+      const bool in = K < nb && r >= 0 && lane < mm && lane >= r - 1;
+      const cd h = in ? T.get(r, lane) : cd{0.0, 0.0};
+      hre[s][i] = h.re;
+      him[s][i] = h.im;
+    }
+  double* qbuf = part + 2 * BS * 4 * 64;  // q, q' of row 0 (4 x 64 doubles)
   for (int it = 0; it < kAberthMaxIter; ++it) {
     // no early exit from the loop: the iterations left once every root has
     // converged skip their work (an early exit here miscompiled for gfx950)
     const bool all_done = flags[0] != 0;
     const cd zi = act ? z[lane] : cd{0.0, 0.0};
-    double xr[8], xi[8], dr[8], di[8];
+    // running sums of this wave's rows: S = sum_j (H - zI)_{rj} x_j and
+    // D = sum_j (H - zI)_{rj} x'_j - x_r; first column mm - 1 (x = 1, x' = 0)
+    double sre[2][BS], sim[2][BS], dre[2][BS], dim[2][BS];
 #pragma unroll
-    for (int q = 0; q < 8; ++q) { xr[q] = 0.0; xi[q] = 0.0; dr[q] = 0.0; di[q] = 0.0; }
-    if (!all_done && (mm - 1) % 8 == wave) {
+    for (int s = 0; s < 2; ++s)
 #pragma unroll
-      for (int q = 0; q < 8; ++q)
-        if (wave + 8 * q == mm - 1) xr[q] = 1.0;
-    }
-    cd q0 = {0.0, 0.0}, qd0 = {0.0, 0.0};
-    for (int r = mm - 1; r >= 0 && !all_done; --r) {
-      // this wave's columns of row r of (H - zI) against x and x'
-      cd s = {0.0, 0.0}, sd = {0.0, 0.0};
-#pragma unroll
-      for (int q = 0; q < 8; ++q) {
-        const int j = wave + 8 * q;
-        if (j >= r && j < mm) {  // the same for the whole wave
-          cd h = {lane_read(hre[q], r), lane_read(him[q], r)};
-          if (j == r) {
-            h = csub(h, zi);
-            sd = csub(sd, {xr[q], xi[q]});
-          }
-          s = cadd(s, cmul(h, {xr[q], xi[q]}));
-          sd = cadd(sd, cmul(h, {dr[q], di[q]}));
-        }
+      for (int i = 0; i < BS; ++i) {
+        const bool diag = (wave + 8 * s) == 0 && i == 0;  // row mm - 1
+        cd h = {lane_read(hre[s][i], mm - 1), lane_read(him[s][i], mm - 1)};
+        if (diag) h = csub(h, zi);
+        sre[s][i] = h.re;
+        sim[s][i] = h.im;
+        dre[s][i] = diag ? -1.0 : 0.0;
+        dim[s][i] = 0.0;
       }
-      double* pb = part + (size_t)(r & 1) * (8 * 4 * 64) + (size_t)wave * (4 * 64);
-      pb[lane] = s.re;
-      pb[64 + lane] = s.im;
-      pb[128 + lane] = sd.re;
-      pb[192 + lane] = sd.im;
-      __syncthreads();
-      const int owner = r > 0 ? (r - 1) % 8 : 0;
-      if (wave == owner) {
-        const double* pr = part + (size_t)(r & 1) * (8 * 4 * 64);
-        cd S = {0.0, 0.0}, SD = {0.0, 0.0};
+    for (int K = 0; K < nb && !all_done; ++K) {
+      const int bot = mm - 1 - K * BS;
+      double* pb = part + (size_t)(K & 1) * (BS * 4 * 64);
+      if ((K & 7) == wave) {
+        // solve block K from its bottom row up; each new column goes into
+        // the block's rows above at once
 #pragma unroll
-        for (int w = 0; w < 8; ++w) {
-          S = cadd(S, {pr[w * 256 + lane], pr[w * 256 + 64 + lane]});
-          SD = cadd(SD, {pr[w * 256 + 128 + lane], pr[w * 256 + 192 + lane]});
-        }
-        if (r == 0) {
-          q0 = S;
-          qd0 = SD;
-        } else {
-          const cd ih = inv_sub[r];
-          const cd xn = cmul(S, ih), dn = cmul(SD, ih);
+        for (int s = 0; s < 2; ++s) {
+          if (wave + 8 * s != K) continue;  // the same for the whole wave
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
-            if (wave + 8 * q == r - 1) {
-              xr[q] = -xn.re;
-              xi[q] = -xn.im;
-              dr[q] = -dn.re;
-              di[q] = -dn.im;
+          for (int i = 0; i < BS; ++i) {
+            const int r = bot - i;
+            if (r >= 1) {
+              const cd ih = inv_sub[r];
+              const cd xn = cmul({-sre[s][i], -sim[s][i]}, ih);
+              const cd dn = cmul({-dre[s][i], -dim[s][i]}, ih);
+              pb[(4 * i) * 64 + lane] = xn.re;
+              pb[(4 * i + 1) * 64 + lane] = xn.im;
+              pb[(4 * i + 2) * 64 + lane] = dn.re;
+              pb[(4 * i + 3) * 64 + lane] = dn.im;
+#pragma unroll
+              for (int i2 = i + 1; i2 < BS; ++i2) {
+                cd h = {lane_read(hre[s][i2], r - 1), lane_read(him[s][i2], r - 1)};
+                if (i2 == i + 1) h = csub(h, zi);  // row r - 1, column r - 1
+                sre[s][i2] = fma(h.re, xn.re, fma(-h.im, xn.im, sre[s][i2]));
+                sim[s][i2] = fma(h.re, xn.im, fma(h.im, xn.re, sim[s][i2]));
+                dre[s][i2] = fma(h.re, dn.re, fma(-h.im, dn.im, dre[s][i2]));
+                dim[s][i2] = fma(h.re, dn.im, fma(h.im, dn.re, dim[s][i2]));
+                if (i2 == i + 1) {
+                  dre[s][i2] -= xn.re;
+                  dim[s][i2] -= xn.im;
+                }
+              }
+            } else {
+              // row 0: q and q'; the columns the block does not produce
+              // (rows 0 and above) read as zero
+              if (r == 0) {
+                qbuf[lane] = sre[s][i];
+                qbuf[64 + lane] = sim[s][i];
+                qbuf[128 + lane] = dre[s][i];
+                qbuf[192 + lane] = dim[s][i];
+              }
+              pb[(4 * i) * 64 + lane] = 0.0;
+              pb[(4 * i + 1) * 64 + lane] = 0.0;
+              pb[(4 * i + 2) * 64 + lane] = 0.0;
+              pb[(4 * i + 3) * 64 + lane] = 0.0;
             }
           }
+        }
+      }
+      __syncthreads();
+      // every wave: block K's columns j = bot - 1 - i into its rows still to
+      // be solved (blocks after K); rows and columns that do not exist hold
+      // zeros (H rows) or read zeros (published columns)
+#pragma unroll
+      for (int s = 0; s < 2; ++s) {
+        const int K2 = wave + 8 * s;
+        if (K2 <= K || K2 >= nb) continue;  // the same for the whole wave
+        cd xs[BS], ds[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+          xs[i] = {pb[(4 * i) * 64 + lane], pb[(4 * i + 1) * 64 + lane]};
+          ds[i] = {pb[(4 * i + 2) * 64 + lane], pb[(4 * i + 3) * 64 + lane]};
+        }
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+          const int j = bot - 1 - i >= 0 ? bot - 1 - i : 0;
+#pragma unroll
+          for (int i2 = 0; i2 < BS; ++i2) {
+            const cd h = {lane_read(hre[s][i2], j), lane_read(him[s][i2], j)};
+            sre[s][i2] = fma(h.re, xs[i].re, fma(-h.im, xs[i].im, sre[s][i2]));
+            sim[s][i2] = fma(h.re, xs[i].im, fma(h.im, xs[i].re, sim[s][i2]));
+            dre[s][i2] = fma(h.re, ds[i].re, fma(-h.im, ds[i].im, dre[s][i2]));
+            dim[s][i2] = fma(h.re, ds[i].im, fma(h.im, ds[i].re, dim[s][i2]));
+          }
+        }
+        if (K2 == K + 1) {
+          // the diagonal term of row bot - BS (this block's bottom row) and
+          // column bot - BS: -z x and -(z x' + x)
+          const cd xj = xs[BS - 1], dj = ds[BS - 1];
+          const cd zx = cmul(zi, xj), zd = cmul(zi, dj);
+          sre[s][0] -= zx.re;
+          sim[s][0] -= zx.im;
+          dre[s][0] -= zd.re + xj.re;
+          dim[s][0] -= zd.im + xj.im;
         }
       }
     }
@@ -658,18 +702,22 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
       cd w = {0.0, 0.0};
       bool hit = false;
       if (!done) {
+        const cd q0 = {qbuf[lane], qbuf[64 + lane]}, qd0 = {qbuf[128 + lane], qbuf[192 + lane]};
         if (q0.re == 0.0 && q0.im == 0.0) {
           hit = true;  // z is a root
         } else {
           const cd ratio = cdiv(qd0, q0);
           cd sum = {0.0, 0.0};
+#pragma unroll 4
           for (int j = 0; j < mm; ++j) {
-            if (j == lane) continue;
             // 1 / (z_i - z_j) = conj(d) / |d|^2 (the differences stay far
-            // from the overflow range)
+            // from the overflow range); the hardware reciprocal and one
+            // Newton step
             const cd d = csub(zi, z[j]);
-            const double inv = 1.0 / cabs2(d);
-            sum = cadd(sum, {d.re * inv, -d.im * inv});
+            const double a2 = cabs2(d);
+            double inv = __builtin_amdgcn_rcp(a2);
+            inv = fma(fma(-a2, inv, 1.0), inv, inv);
+            if (j != lane) sum = cadd(sum, {d.re * inv, -d.im * inv});
           }
           w = cdiv({1.0, 0.0}, csub(ratio, sum));
         }
@@ -689,6 +737,7 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
       }
       const bool left = __ballot(!done) != 0ull;
       if (lane == 0 && !left) flags[0] = 1;
+      if (lane == 0) flags[2] = it + 1;
     }
     __syncthreads();
   }
@@ -709,6 +758,35 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
   }
   __syncthreads();
   return flags[1] != 0;
+}
+
+
+// All mm eigenvalues of H (in T, unreduced: every subdiagonal nonzero) into
+// z (LDS, mm entries), by all 8 waves of the block; lane i iterates root i.
+// Hyman's recurrence for det(H - zI) (x_{mm-1} = 1, x_{r-1} = -(sum_{j>=r}
+// (H - zI)_{rj} x_j) / h_{r,r-1}, and the same for x' = dx/dz) is a back
+// substitution by rows: its rows form nb <= 16 blocks of BS = ceil(mm / 16)
+// rows from the bottom, block K owned by wave K % 8, which keeps the
+// block's rows of H in registers (lane = column, read back by readlane) and
+// the running sums of those rows (lane = root).  The owner solves its block
+// row by row once every column below it is in the sums and publishes the
+// block's new x, x' in LDS (part: 2 x BS x 4 x 64 doubles, double-buffered
+// by block parity); after one barrier every wave adds them to the sums of
+// its rows still to be solved, with fused multiply-adds and no branches
+// inside a block.  Row 0 leaves q(z) = det(H - zI) / (the product of the
+// subdiagonals, up to sign) and q'(z) in LDS, and wave 0 takes the Aberth
+// step for every root.  Returns (the same in every thread) whether all
+// roots converged, are finite, and sum to the trace; flags[2] = the
+// iterations taken.
+__device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, int* flags,
+                             cd* inv_sub, const double* warm_re = nullptr,
+                             const double* warm_im = nullptr) {
+  switch ((mm + kAbBlocks - 1) / kAbBlocks) {
+    case 1: return aberth_roots_t<1>(T, mm, hnorm, z, part, flags, inv_sub, warm_re, warm_im);
+    case 2: return aberth_roots_t<2>(T, mm, hnorm, z, part, flags, inv_sub, warm_re, warm_im);
+    case 3: return aberth_roots_t<3>(T, mm, hnorm, z, part, flags, inv_sub, warm_re, warm_im);
+    default: return aberth_roots_t<4>(T, mm, hnorm, z, part, flags, inv_sub, warm_re, warm_im);
+  }
 }
 
 // Right eigenvectors of H (in T, upper Hessenberg, mm <= 64) for the
@@ -1100,10 +1178,10 @@ constexpr int kRestartLdsMax = 144 * 1024;
 __host__ __device__ constexpr bool x_in_lds(int m) {
   return 6 * m * lds_ld(m) * (int)sizeof(double) <= kRestartLdsMax;
 }
-// The Aberth iteration's partial sums (2 x 8 x 4 x 64 doubles) follow T
-// (in the Z area, set to the identity afterwards); for small m the launch
-// allocates beyond the restart's own need to hold them.
-constexpr int kAberthPartDoubles = 2 * 8 * 4 * 64;
+// The Aberth iteration's published columns (2 x 4 x 4 x 64 doubles) and
+// q, q' (4 x 64) follow T (in the Z area, free until the eigenvectors); for
+// small m the launch allocates beyond the restart's own need to hold them.
+constexpr int kAberthPartDoubles = 2 * 4 * 4 * 64 + 4 * 64;
 __host__ __device__ constexpr int restart_lds_bytes(int m) {
   return ((x_in_lds(m) ? 6 : 4) * m * lds_ld(m) > 2 * m * lds_ld(m) + kAberthPartDoubles
               ? (x_in_lds(m) ? 6 : 4) * m * lds_ld(m)
@@ -1227,7 +1305,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
         }
         __syncthreads();
         hnorm = tnorm_s;
-        __shared__ int aberth_flags[2];
+        __shared__ int aberth_flags[3];
         const unsigned long long negl = negligible_mask(T, mm);  // the same in every wave
         if (negl == 0ull) {
           // the previous restart of this factorisation left its Ritz values
