@@ -528,6 +528,78 @@ __device__ __forceinline__ bool split_ok(Mat T, int i, double hnorm) {
   return sub <= eps * d || sub < 1e-300;
 }
 
+// Block of BS rows of Hyman's recurrence (bottom row bot, lane = root), its
+// running sums S, D complete for every column below the block: solves it
+// from the bottom row up (x_{r-1} = -S_r / h_{r,r-1}, x'_{r-1} = -D_r /
+// h_{r,r-1}), adding each new column to the block's rows above at once, and
+// publishes the BS new columns x, x' to pb (zeros past row 0); row 0 leaves
+// q, q' in qbuf.
+template <int BS>
+__device__ __forceinline__ void ab_solve(double (&sre)[BS], double (&sim)[BS], double (&dre)[BS],
+                                         double (&dim)[BS], const double (&hre)[BS],
+                                         const double (&him)[BS], int bot, cd zi,
+                                         const cd* inv_sub, double* pb, double* qbuf, int lane) {
+#pragma unroll
+  for (int i = 0; i < BS; ++i) {
+    const int r = bot - i;
+    if (r >= 1) {
+      const cd ih = inv_sub[r];
+      const cd xn = cmul({-sre[i], -sim[i]}, ih);
+      const cd dn = cmul({-dre[i], -dim[i]}, ih);
+      pb[(4 * i) * 64 + lane] = xn.re;
+      pb[(4 * i + 1) * 64 + lane] = xn.im;
+      pb[(4 * i + 2) * 64 + lane] = dn.re;
+      pb[(4 * i + 3) * 64 + lane] = dn.im;
+#pragma unroll
+      for (int i2 = i + 1; i2 < BS; ++i2) {
+        cd h = {lane_read(hre[i2], r - 1), lane_read(him[i2], r - 1)};
+        if (i2 == i + 1) h = csub(h, zi);  // row r - 1, column r - 1
+        sre[i2] = fma(h.re, xn.re, fma(-h.im, xn.im, sre[i2]));
+        sim[i2] = fma(h.re, xn.im, fma(h.im, xn.re, sim[i2]));
+        dre[i2] = fma(h.re, dn.re, fma(-h.im, dn.im, dre[i2]));
+        dim[i2] = fma(h.re, dn.im, fma(h.im, dn.re, dim[i2]));
+        if (i2 == i + 1) {
+          dre[i2] -= xn.re;
+          dim[i2] -= xn.im;
+        }
+      }
+    } else {
+      if (r == 0) {
+        qbuf[lane] = sre[i];
+        qbuf[64 + lane] = sim[i];
+        qbuf[128 + lane] = dre[i];
+        qbuf[192 + lane] = dim[i];
+      }
+      pb[(4 * i) * 64 + lane] = 0.0;
+      pb[(4 * i + 1) * 64 + lane] = 0.0;
+      pb[(4 * i + 2) * 64 + lane] = 0.0;
+      pb[(4 * i + 3) * 64 + lane] = 0.0;
+    }
+  }
+}
+
+// The BS columns j = bot - 1 - i (x, x' in xs, ds) into the running sums
+// of BS rows (H rows hre, him, lane = column); no diagonal terms (rows
+// above the columns, or the caller adds them).
+template <int BS>
+__device__ __forceinline__ void ab_update(double (&sre)[BS], double (&sim)[BS], double (&dre)[BS],
+                                          double (&dim)[BS], const double (&hre)[BS],
+                                          const double (&him)[BS], int bot, const cd (&xs)[BS],
+                                          const cd (&ds)[BS]) {
+#pragma unroll
+  for (int i = 0; i < BS; ++i) {
+    const int j = bot - 1 - i >= 0 ? bot - 1 - i : 0;
+#pragma unroll
+    for (int i2 = 0; i2 < BS; ++i2) {
+      const cd h = {lane_read(hre[i2], j), lane_read(him[i2], j)};
+      sre[i2] = fma(h.re, xs[i].re, fma(-h.im, xs[i].im, sre[i2]));
+      sim[i2] = fma(h.re, xs[i].im, fma(h.im, xs[i].re, sim[i2]));
+      dre[i2] = fma(h.re, ds[i].re, fma(-h.im, ds[i].im, dre[i2]));
+      dim[i2] = fma(h.re, ds[i].im, fma(h.im, ds[i].re, dim[i2]));
+    }
+  }
+}
+
 // The Aberth iteration of aberth_roots (below) for BS rows per block.
 constexpr int kAbBlocks = 16;  // row blocks of the recurrence (two per wave)
 template <int BS>  // rows per block: ceil(mm / 16)
@@ -589,6 +661,7 @@ __device__ bool aberth_roots_t(Mat T, int mm, double hnorm, cd* z, double* part,
       him[s][i] = h.im;
     }
   double* qbuf = part + 2 * BS * 4 * 64;  // q, q' of row 0 (4 x 64 doubles)
+  double* sump = part + 2 * 4 * 4 * 64 + 4 * 64;  // the waves' Aberth sums (8 x 2 x 64)
   for (int it = 0; it < kAberthMaxIter; ++it) {
     // no early exit from the loop: the iterations left once every root has
     // converged skip their work (an early exit here miscompiled for gfx950)
@@ -609,64 +682,19 @@ __device__ bool aberth_roots_t(Mat T, int mm, double hnorm, cd* z, double* part,
         dre[s][i] = diag ? -1.0 : 0.0;
         dim[s][i] = 0.0;
       }
-    for (int K = 0; K < nb && !all_done; ++K) {
-      const int bot = mm - 1 - K * BS;
-      double* pb = part + (size_t)(K & 1) * (BS * 4 * 64);
-      if ((K & 7) == wave) {
-        // solve block K from its bottom row up; each new column goes into
-        // the block's rows above at once
-#pragma unroll
-        for (int s = 0; s < 2; ++s) {
-          if (wave + 8 * s != K) continue;  // the same for the whole wave
-#pragma unroll
-          for (int i = 0; i < BS; ++i) {
-            const int r = bot - i;
-            if (r >= 1) {
-              const cd ih = inv_sub[r];
-              const cd xn = cmul({-sre[s][i], -sim[s][i]}, ih);
-              const cd dn = cmul({-dre[s][i], -dim[s][i]}, ih);
-              pb[(4 * i) * 64 + lane] = xn.re;
-              pb[(4 * i + 1) * 64 + lane] = xn.im;
-              pb[(4 * i + 2) * 64 + lane] = dn.re;
-              pb[(4 * i + 3) * 64 + lane] = dn.im;
-#pragma unroll
-              for (int i2 = i + 1; i2 < BS; ++i2) {
-                cd h = {lane_read(hre[s][i2], r - 1), lane_read(him[s][i2], r - 1)};
-                if (i2 == i + 1) h = csub(h, zi);  // row r - 1, column r - 1
-                sre[s][i2] = fma(h.re, xn.re, fma(-h.im, xn.im, sre[s][i2]));
-                sim[s][i2] = fma(h.re, xn.im, fma(h.im, xn.re, sim[s][i2]));
-                dre[s][i2] = fma(h.re, dn.re, fma(-h.im, dn.im, dre[s][i2]));
-                dim[s][i2] = fma(h.re, dn.im, fma(h.im, dn.re, dim[s][i2]));
-                if (i2 == i + 1) {
-                  dre[s][i2] -= xn.re;
-                  dim[s][i2] -= xn.im;
-                }
-              }
-            } else {
-              // row 0: q and q'; the columns the block does not produce
-              // (rows 0 and above) read as zero
-              if (r == 0) {
-                qbuf[lane] = sre[s][i];
-                qbuf[64 + lane] = sim[s][i];
-                qbuf[128 + lane] = dre[s][i];
-                qbuf[192 + lane] = dim[s][i];
-              }
-              pb[(4 * i) * 64 + lane] = 0.0;
-              pb[(4 * i + 1) * 64 + lane] = 0.0;
-              pb[(4 * i + 2) * 64 + lane] = 0.0;
-              pb[(4 * i + 3) * 64 + lane] = 0.0;
-            }
-          }
-        }
-      }
-      __syncthreads();
-      // every wave: block K's columns j = bot - 1 - i into its rows still to
-      // be solved (blocks after K); rows and columns that do not exist hold
-      // zeros (H rows) or read zeros (published columns)
-#pragma unroll
-      for (int s = 0; s < 2; ++s) {
-        const int K2 = wave + 8 * s;
-        if (K2 <= K || K2 >= nb) continue;  // the same for the whole wave
+    if (!all_done) {
+      if (wave == 0) ab_solve<BS>(sre[0], sim[0], dre[0], dim[0], hre[0], him[0], mm - 1, zi, inv_sub,
+                                  part, qbuf, lane);
+      for (int K = 0; K + 1 < nb; ++K) {
+        __syncthreads();
+        // block K's columns j = bot - 1 - i into the rows still to be solved
+        // (blocks after K); the owner of block K + 1 takes its own rows, the
+        // diagonal term and the solve first, at raised priority (the chain
+        // every other wave waits for), then its rows of a later block
+        const int bot = mm - 1 - K * BS;
+        const double* pb = part + (size_t)(K & 1) * (BS * 4 * 64);
+        double* pn = part + (size_t)((K + 1) & 1) * (BS * 4 * 64);
+        if (wave + 8 <= K || (wave <= K && wave + 8 >= nb)) continue;  // no rows left here
         cd xs[BS], ds[BS];
 #pragma unroll
         for (int i = 0; i < BS; ++i) {
@@ -674,28 +702,48 @@ __device__ bool aberth_roots_t(Mat T, int mm, double hnorm, cd* z, double* part,
           ds[i] = {pb[(4 * i + 2) * 64 + lane], pb[(4 * i + 3) * 64 + lane]};
         }
 #pragma unroll
-        for (int i = 0; i < BS; ++i) {
-          const int j = bot - 1 - i >= 0 ? bot - 1 - i : 0;
-#pragma unroll
-          for (int i2 = 0; i2 < BS; ++i2) {
-            const cd h = {lane_read(hre[s][i2], j), lane_read(him[s][i2], j)};
-            sre[s][i2] = fma(h.re, xs[i].re, fma(-h.im, xs[i].im, sre[s][i2]));
-            sim[s][i2] = fma(h.re, xs[i].im, fma(h.im, xs[i].re, sim[s][i2]));
-            dre[s][i2] = fma(h.re, ds[i].re, fma(-h.im, ds[i].im, dre[s][i2]));
-            dim[s][i2] = fma(h.re, ds[i].im, fma(h.im, ds[i].re, dim[s][i2]));
-          }
-        }
-        if (K2 == K + 1) {
-          // the diagonal term of row bot - BS (this block's bottom row) and
-          // column bot - BS: -z x and -(z x' + x)
+        for (int s = 0; s < 2; ++s) {
+          if (wave + 8 * s != K + 1) continue;  // the same for the whole wave
+          __builtin_amdgcn_s_setprio(3);
+          ab_update<BS>(sre[s], sim[s], dre[s], dim[s], hre[s], him[s], bot, xs, ds);
+          // the diagonal term of row bot - BS and column bot - BS: -z x and
+          // -(z x' + x)
           const cd xj = xs[BS - 1], dj = ds[BS - 1];
           const cd zx = cmul(zi, xj), zd = cmul(zi, dj);
           sre[s][0] -= zx.re;
           sim[s][0] -= zx.im;
           dre[s][0] -= zd.re + xj.re;
           dim[s][0] -= zd.im + xj.im;
+          ab_solve<BS>(sre[s], sim[s], dre[s], dim[s], hre[s], him[s], bot - BS, zi, inv_sub, pn,
+                       qbuf, lane);
+          __builtin_amdgcn_s_setprio(0);
+        }
+#pragma unroll
+        for (int s = 0; s < 2; ++s) {
+          const int K2 = wave + 8 * s;
+          if (K2 <= K + 1 || K2 >= nb) continue;  // the same for the whole wave
+          ab_update<BS>(sre[s], sim[s], dre[s], dim[s], hre[s], him[s], bot, xs, ds);
         }
       }
+      __syncthreads();  // q, q' of row 0 from the last block's solve
+      // the Aberth sums sum_{j != i} 1 / (z_i - z_j), j = wave + 8 q on
+      // this wave: conj(d) / |d|^2 with the hardware reciprocal and one
+      // Newton step (the differences stay far from the overflow range)
+      cd ps = {0.0, 0.0};
+#pragma unroll
+      for (int q = 0; q < 8; ++q) {
+        const int j = wave + 8 * q;
+        if (j < mm) {
+          const cd d = csub(zi, z[j]);
+          const double a2 = cabs2(d);
+          double inv = __builtin_amdgcn_rcp(a2);
+          inv = fma(fma(-a2, inv, 1.0), inv, inv);
+          if (j != lane) ps = cadd(ps, {d.re * inv, -d.im * inv});
+        }
+      }
+      sump[wave * 128 + lane] = ps.re;
+      sump[wave * 128 + 64 + lane] = ps.im;
+      __syncthreads();
     }
     // wave 0: the Aberth step w = 1 / (q'/q - sum_{j != i} 1 / (z_i - z_j))
     if (wave == 0 && !all_done) {
@@ -708,17 +756,8 @@ __device__ bool aberth_roots_t(Mat T, int mm, double hnorm, cd* z, double* part,
         } else {
           const cd ratio = cdiv(qd0, q0);
           cd sum = {0.0, 0.0};
-#pragma unroll 4
-          for (int j = 0; j < mm; ++j) {
-            // 1 / (z_i - z_j) = conj(d) / |d|^2 (the differences stay far
-            // from the overflow range); the hardware reciprocal and one
-            // Newton step
-            const cd d = csub(zi, z[j]);
-            const double a2 = cabs2(d);
-            double inv = __builtin_amdgcn_rcp(a2);
-            inv = fma(fma(-a2, inv, 1.0), inv, inv);
-            if (j != lane) sum = cadd(sum, {d.re * inv, -d.im * inv});
-          }
+#pragma unroll
+          for (int w2 = 0; w2 < 8; ++w2) sum = cadd(sum, {sump[w2 * 128 + lane], sump[w2 * 128 + 64 + lane]});
           w = cdiv({1.0, 0.0}, csub(ratio, sum));
         }
       }
@@ -1178,10 +1217,10 @@ constexpr int kRestartLdsMax = 144 * 1024;
 __host__ __device__ constexpr bool x_in_lds(int m) {
   return 6 * m * lds_ld(m) * (int)sizeof(double) <= kRestartLdsMax;
 }
-// The Aberth iteration's published columns (2 x 4 x 4 x 64 doubles) and
-// q, q' (4 x 64) follow T (in the Z area, free until the eigenvectors); for
+// The Aberth iteration's published columns (2 x 4 x 4 x 64 doubles), q, q'
+// (4 x 64) and the waves' sums (8 x 2 x 64) follow T (in the Z area, free until the eigenvectors); for
 // small m the launch allocates beyond the restart's own need to hold them.
-constexpr int kAberthPartDoubles = 2 * 4 * 4 * 64 + 4 * 64;
+constexpr int kAberthPartDoubles = 2 * 4 * 4 * 64 + 4 * 64 + 8 * 2 * 64;
 __host__ __device__ constexpr int restart_lds_bytes(int m) {
   return ((x_in_lds(m) ? 6 : 4) * m * lds_ld(m) > 2 * m * lds_ld(m) + kAberthPartDoubles
               ? (x_in_lds(m) ? 6 : 4) * m * lds_ld(m)
