@@ -242,6 +242,9 @@ oaph_status get_info(const uint8_t* b, size_t len, int* present, oaph_matrix_inf
   info->columns = c;
   info->rows = r;
   *data_off = 1 + kInfoBytes;
+  // a header of 2^32 or more elements is rejected before the record size is
+  // formed (the same limit as oaph_matrix_new), so `need` cannot wrap
+  OAPH_CHECK((uint64_t)c * r < (1ull << 32), OAPH_ERR_ARG);
   const size_t need = record_bytes(true, info->isRe, info->isIm, (size_t)c * r);
   OAPH_CHECK(len >= need, OAPH_ERR_ARG);
   return OAPH_OK;

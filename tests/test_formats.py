@@ -87,6 +87,20 @@ def test_record_layout():
         formats.load_host(data[:-1])  # truncated record
 
 
+def test_oversized_header_rejected():
+    # columns = rows = 2^31: 2^62 elements, whose byte count wraps a 64-bit size
+    # to 11; the header must be refused before any size is formed from it
+    head = bytes([1, 1, 1]) + (1 << 31).to_bytes(4, "little") * 2
+    for tail in (b"", bytes(64)):
+        with pytest.raises(OapError):
+            formats.load_host(head + tail)
+    # 2^32 elements exactly is the first size refused; one less passes the
+    # header check and fails only for want of data
+    head = bytes([1, 1, 0]) + (1 << 16).to_bytes(4, "little") * 2
+    with pytest.raises(OapError):
+        formats.load_host(head)
+
+
 def test_records_concatenate(tmp_path):
     a = formats.save_host(2, 2, np.eye(2), np.ones((2, 2)))
     b = formats.save_host(0, 0, present=False)
