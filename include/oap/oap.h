@@ -555,6 +555,9 @@ class CuHArnoldi {
   void setMixedTolerance(double t) {
     oap::check(oaph_arnoldi_set_mixed_tolerance(m_a, t), "setMixedTolerance");
   }
+  void setMixedCheck(bool on, double fraction = 0.5) {
+    oap::check(oaph_arnoldi_set_mixed_check(m_a, on ? 1 : 0, fraction), "setMixedCheck");
+  }
   void setRestartStages(const std::vector<double>& tols) {
     oap::check(oaph_arnoldi_set_restart_stages(m_a, tols.empty() ? nullptr : tols.data(),
                                                (uint32_t)tols.size()),

@@ -466,6 +466,17 @@ oaph_status oaph_arnoldi_set_schur_method(oaph_arnoldi* a, int method);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
  * the largest Ritz value (default 1e-6) */
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* a, double tol_low);
+/* OAPH_PREC_MIXED_FP32, what follows the FP32 phase (no reference
+ * counterpart).  1 (default): FP32-operand cycles go on from the Ritz-vector
+ * sum down to `fraction` (0 < fraction <= 1, default 0.5) of the requested
+ * tolerance; then each wanted Ritz vector gets its FP64 Rayleigh quotient
+ * and explicit FP64 residual against the FP64 operator, and FP64 cycles run
+ * only when some residual exceeds the tolerance (stats.fp64_polish).  The
+ * FP32-operand matvec accumulates in FP64, so those cycles are Arnoldi on
+ * the FP32-rounded operator to FP64 rounding; whether its eigenvectors meet
+ * the tolerance for the FP64 operator depends on the operator and is decided
+ * by the check alone.  0: FP64 cycles always follow the FP32 phase. */
+oaph_status oaph_arnoldi_set_mixed_check(oaph_arnoldi* a, int on, double fraction);
 /* start vector for the next execute (n complex values, NULL im = real) */
 oaph_status oaph_arnoldi_set_start_vector(oaph_arnoldi* a, const double* re, const double* im,
                                           uint32_t n);
@@ -488,6 +499,9 @@ typedef struct oaph_arnoldi_stats {
   uint32_t restarts_inverse_iteration; /* restarts that took the Aberth eigenvalues
                                           and inverse-iteration eigenvectors of H
                                           (method 2) without a Schur form */
+  uint32_t fp64_polish; /* mixed flow with the FP64 check (the default): 1 when
+                           the Ritz pairs of the FP32 cycles failed the explicit
+                           FP64 residual check and FP64 cycles polished them */
 } oaph_arnoldi_stats;
 oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
 

@@ -624,6 +624,14 @@ class CuHArnoldiCallback:
     def setMixedTolerance(self, tol_low):
         check(self._lib.oaph_arnoldi_set_mixed_tolerance(self._h, tol_low), "set_mixed_tolerance")
 
+    def setMixedCheck(self, on=True, fraction=0.5):
+        """mixed flow after the FP32 phase: True (default) runs FP32 cycles
+        down to fraction x the tolerance and accepts the pairs on an explicit
+        FP64 residual check, with FP64 cycles only where it fails; False
+        always polishes with FP64 cycles"""
+        check(self._lib.oaph_arnoldi_set_mixed_check(self._h, int(bool(on)), fraction),
+              "set_mixed_check")
+
     def setStartVector(self, v):
         if v is None:
             check(self._lib.oaph_arnoldi_set_start_vector(self._h, None, None, 0), "setStartVector")

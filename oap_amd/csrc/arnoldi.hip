@@ -48,6 +48,10 @@ struct oaph_arnoldi {
   uint32_t max_restarts = 1000;
   int precision = OAPH_PREC_FP64;
   double tol_low = 1e-6;  // FP32 phase tolerance, relative to max |Ritz value|
+  // after the FP32 phase: FP32 cycles down to check_fraction x tol, then the
+  // explicit FP64 residual check, FP64 cycles only where it fails
+  bool mixed_check = true;
+  double check_fraction = 0.5;
   // tolerances (relative to max |Ritz value|, descending) at which the solve
   // restarts from the sum of the wanted Ritz vectors before going on
   std::vector<double> stage_tols;
@@ -90,7 +94,7 @@ struct oaph_arnoldi {
   bool a32_usable = false;  // the FP32-operand matvec can run on the copy
   Status host_status{};
   Status* pinned_status = nullptr;  // page-locked landing buffer of read_status
-  std::vector<double> ev_re, ev_im, resid;
+  std::vector<double> ev_re, ev_im, resid, xnorm2;
   oaph_arnoldi_stats stats{};
   bool executed = false;
 };
@@ -403,9 +407,11 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
       *cycles = cycle + 1;
       return OAPH_OK;
     }
-    if (watch && !low && s->check == OAPH_CHECK_INTERNAL && wanted_set_changed(s, *watch)) {
-      // the factorisation goes on with FP32-operand steps: its FP64 columns
-      // differ from FP32 ones by far less than the FP32 tolerance
+    if (watch && !watch->fell_back && s->check == OAPH_CHECK_INTERNAL &&
+        wanted_set_changed(s, *watch)) {
+      // the factorisation goes on with FP32-operand steps (FP64 columns
+      // already in it differ from FP32 ones by far less than the FP32
+      // tolerance) until the new set reaches the FP32 phase's tolerance
       low = true;
       tol = watch->low_tol;
       norm = 0.0;
@@ -479,7 +485,9 @@ oaph_status normalize_v0(oaph_arnoldi* s) {
   return normalize(st, w, fre, fim, Vre(s, s->V, 0), Vim(s, s->V, 0));
 }
 
-oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
+// FP64 Rayleigh quotients and explicit residuals of the k wanted Ritz
+// vectors (the operator applied in FP64); X keeps the vectors
+oaph_status ritz_pairs(oaph_arnoldi* s, uint32_t mact) {
   hipStream_t st = s->api->stream;
   const uint32_t k = s->k, n = s->n;
   OAPH_TRY(ritz_vectors(s, mact, k));
@@ -503,6 +511,7 @@ oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
   s->ev_re.assign(k, 0.0);
   s->ev_im.assign(k, 0.0);
   s->resid.assign(k, 0.0);
+  s->xnorm2.assign(k, 0.0);
   double maxr = 0;
   for (uint32_t t = 0; t < k; ++t) {
     // Rayleigh quotient x^H A x / x^H x (FP64) refines the Ritz value
@@ -513,7 +522,17 @@ oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
     maxr = fmax(maxr, s->resid[t]);
   }
   s->stats.max_resid = maxr;
-  // outputs (extractOutput, ArnoldiProcedures.cpp:229-258)
+  for (uint32_t t = 0; t < k; ++t) s->xnorm2[t] = out_h[t].y;
+  return OAPH_OK;
+}
+
+// outputs of the pairs ritz_pairs formed (extractOutput,
+// ArnoldiProcedures.cpp:229-258)
+oaph_status emit_outputs(oaph_arnoldi* s) {
+  hipStream_t st = s->api->stream;
+  const uint32_t k = s->k, n = s->n;
+  double* Xre = s->X.as<double>();
+  double* Xim = Xre + (size_t)s->ky * n;
   for (uint32_t t = 0; t < k; ++t) {
     if (s->out_re) s->out_re[t] = s->ev_re[t];
     if (s->out_im) s->out_im[t] = s->ev_im[t];
@@ -523,7 +542,7 @@ oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
       oaph_matrix* o = s->out_vecs[t];
       if (!o) continue;
       OAPH_CHECK((size_t)o->columns * o->rows == n, OAPH_ERR_SHAPE);
-      double inv = 1.0 / sqrt(out_h[t].y);
+      double inv = 1.0 / sqrt(s->xnorm2[t]);
       oaph_matrix src{1, n, 1, n, Xre + (size_t)t * n, Xim + (size_t)t * n};
       if (s->out_type == OAPH_HOST) {
         std::vector<double> tmp(n);
@@ -541,6 +560,11 @@ oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
     OAPH_HIP(hipStreamSynchronize(st));
   }
   return OAPH_OK;
+}
+
+oaph_status finish(oaph_arnoldi* s, uint32_t mact) {
+  OAPH_TRY(ritz_pairs(s, mact));
+  return emit_outputs(s);
 }
 
 }  // namespace
@@ -647,6 +671,12 @@ oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* s, double w) {
   return OAPH_OK;
 }
 
+oaph_status oaph_arnoldi_set_mixed_check(oaph_arnoldi* s, int on, double fraction) {
+  OAPH_CHECK(s && fraction > 0.0 && fraction <= 1.0, OAPH_ERR_ARG);
+  s->mixed_check = on != 0;
+  s->check_fraction = fraction;
+  return OAPH_OK;
+}
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* s, double tol_low) {
   OAPH_CHECK(s && tol_low > 0 && tol_low < 1, OAPH_ERR_ARG);
   s->tol_low = tol_low;
@@ -802,7 +832,31 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     // kMaxFallbacks times; then the FP64 cycles finish on their own).
     constexpr int kMaxFallbacks = 3;
     OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
-    for (int round = 0;; ++round) {
+    bool accepted = false;
+    if (s->mixed_check && s->check == OAPH_CHECK_INTERNAL) {
+      // FP32-operand cycles from the Ritz-vector sum down to a fraction of
+      // the target (Arnoldi on the FP32-rounded operator: its matvec
+      // accumulates in FP64), watched as the FP64 polish is; then the FP64
+      // Rayleigh quotient and explicit FP64 residual of each wanted pair
+      for (int round = 0;; ++round) {
+        Watch wt;
+        wt.re.assign(s->host_status.ev_re, s->host_status.ev_re + s->k);
+        wt.im.assign(s->host_status.ev_im, s->host_status.ev_im + s->k);
+        wt.low_tol = s->tol_low;
+        OAPH_TRY(restart_from_ritz_sum(s->ky));
+        OAPH_TRY(run_cycles(s, true, s->check_fraction * s->tol, norm, &c2, &converged,
+                            round < kMaxFallbacks ? &wt : nullptr));
+        if (!wt.fell_back) break;
+        s->stats.fallbacks++;
+      }
+      if (converged) {
+        const int bd2 = s->host_status.breakdown;
+        OAPH_TRY(ritz_pairs(s, bd2 > 0 ? (uint32_t)bd2 : s->m));
+        accepted = s->stats.max_resid <= s->tol * norm;
+      }
+      if (!accepted) s->stats.fp64_polish = 1;
+    }
+    for (int round = 0; !accepted; ++round) {
       Watch wt;
       wt.re.assign(s->host_status.ev_re, s->host_status.ev_re + s->k);
       wt.im.assign(s->host_status.ev_im, s->host_status.ev_im + s->k);
@@ -818,7 +872,11 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
   }
   int bd = s->host_status.breakdown;
   uint32_t mact = bd > 0 ? (uint32_t)bd : s->m;
-  OAPH_TRY(finish(s, mact));
+  if (mixed && !all_low && !s->stats.fp64_polish && s->mixed_check &&
+      s->check == OAPH_CHECK_INTERNAL && converged)
+    OAPH_TRY(emit_outputs(s));  // the checked pairs
+  else
+    OAPH_TRY(finish(s, mact));
   s->stats.nconv = (uint32_t)s->host_status.nconv;
   if (bd > 0) s->stats.nconv = s->k;
   s->stats.seconds =

@@ -45,13 +45,15 @@ def solver(hdim_tol=1e-10, which=ArnUtils.SortLargestValues, seed=7):
     return s
 
 
-def run_dense(A, k, m, which=ArnUtils.SortLargestValues, precision=Precision.FP64, vectors=True):
+def run_dense(A, k, m, which=ArnUtils.SortLargestValues, precision=Precision.FP64, vectors=True,
+              mixed_check=True):
     n = A.shape[0]
     cplx = np.iscomplexobj(A)
     dA = DeviceMatrix.from_numpy(A.real, A.imag if cplx else None)
     s = solver(which=which)
     s.setMatrix(dA)
     s.setPrecision(precision)
+    s.setMixedCheck(mixed_check)
     re, im = np.zeros(k), np.zeros(k)
     s.setOutputsEigenvalues(re, im)
     vecs = [DeviceMatrix(1, n) for _ in range(k)] if vectors else None
@@ -121,12 +123,50 @@ def test_mixed_with_fp32_valued_operator_runs_all_cycles_in_fp32(oracle, rng):
     want = top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues)
     match(ev, want, 1e-10)
     check_residuals(s, A, ev, vecs, 8)
-    # one entry that is not an FP32 value: FP32 phase, then the FP64 restart
+    # one entry that is not an FP32 value: the FP32 cycles' pairs pass the
+    # FP64 residual check (the copy differs from A by 1e-12 in one entry),
+    # so no FP64 cycle runs; without the check the FP64 restart follows
     A[3, 5] += 1e-12
     s, ev, vecs = run_dense(A, 8, 40, precision=Precision.MIXED_FP32)
     st = s.stats()
+    assert st["matvecs_low"] == st["matvecs"] - 8 and st["fp64_polish"] == 0
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+    s, ev, vecs = run_dense(A, 8, 40, precision=Precision.MIXED_FP32, mixed_check=False)
+    st = s.stats()
     assert 0 < st["matvecs_low"] < st["matvecs"] - 8
     match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+
+
+def test_mixed_check_accepts_fp32_pairs_of_c1(oracle):
+    # C1 construction: the FP32 rounding of A moves mostly the diagonal, whose
+    # effect on each eigenvector lies along the vector itself (the FP64
+    # Rayleigh quotient absorbs it), so the FP32 cycles' eigenvectors meet
+    # 1e-10 ||A||_F against the FP64 operator and no FP64 cycle runs
+    n = 1024
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    s, ev, vecs = run_dense(A, 8, 48, precision=Precision.MIXED_FP32)
+    st = s.stats()
+    assert st["fp64_polish"] == 0 and st["matvecs_low"] == st["matvecs"] - 8
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
+
+
+def test_mixed_check_polishes_when_fp32_vectors_fail():
+    # dense off-diagonal part of size 1/sqrt(n): its FP32 rounding (~3e-9 an
+    # entry) gives the FP32 eigenvectors residuals near 6e-8 against A, ten
+    # times 1e-10 ||A||_F (~6e-9): the check fails and FP64 cycles polish
+    rng = np.random.default_rng(7)
+    n = 512
+    d = rng.uniform(0, 1, n) * np.exp(2j * np.pi * rng.uniform(0, 1, n))
+    d[:8] = np.arange(20, 12, -1) * np.exp(2j * np.pi * rng.uniform(0, 1, 8))
+    A = np.diag(d) + (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))) / np.sqrt(n)
+    s, ev, vecs = run_dense(A, 8, 32, precision=Precision.MIXED_FP32)
+    st = s.stats()
+    assert st["fp64_polish"] == 1 and 0 < st["matvecs_low"] < st["matvecs"] - 8
+    match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 8)
 
 
 @pytest.mark.parametrize("n", [1022, 1023])
