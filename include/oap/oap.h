@@ -555,6 +555,9 @@ class CuHArnoldi {
   void setMixedTolerance(double t) {
     oap::check(oaph_arnoldi_set_mixed_tolerance(m_a, t), "setMixedTolerance");
   }
+  void setF16Phase(bool on, double tol = 2e-3) {
+    oap::check(oaph_arnoldi_set_f16_phase(m_a, on ? 1 : 0, tol), "setF16Phase");
+  }
   void setMixedCheck(bool on, double fraction = 0.5) {
     oap::check(oaph_arnoldi_set_mixed_check(m_a, on ? 1 : 0, fraction), "setMixedCheck");
   }

@@ -477,6 +477,19 @@ oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* a, double tol_low);
  * the tolerance for the FP64 operator depends on the operator and is decided
  * by the check alone.  0: FP64 cycles always follow the FP32 phase. */
 oaph_status oaph_arnoldi_set_mixed_check(oaph_arnoldi* a, int on, double fraction);
+/* OAPH_PREC_MIXED_FP32: a first phase on an FP16 copy of the operator (no
+ * reference counterpart).  on = 1: cycles with the FP16-operand matvec (the
+ * copy holds the FP16 rounding of 2^s A, the largest entry in [2^14, 2^15);
+ * x rounded to FP32,
+ * FP32 products and per-lane sums, 2 bytes per operator value) until the
+ * Ritz estimates reach tol (0 < tol < 1, relative to the largest Ritz value,
+ * default 2e-3); the FP32 phase then starts from the sum of their wanted and
+ * guard Ritz vectors.  The copy is made (and kept) on the first solve that
+ * uses it: 2 bytes per operator value per plane of device memory; an
+ * operator whose row length is not a multiple of 8 or with a non-finite
+ * entry skips the phase, as does one whose entries are all FP32 values (its
+ * cycles run on the FP32 copy alone).  On by default. */
+oaph_status oaph_arnoldi_set_f16_phase(oaph_arnoldi* a, int on, double tol);
 /* start vector for the next execute (n complex values, NULL im = real) */
 oaph_status oaph_arnoldi_set_start_vector(oaph_arnoldi* a, const double* re, const double* im,
                                           uint32_t n);
@@ -502,16 +515,28 @@ typedef struct oaph_arnoldi_stats {
   uint32_t fp64_polish; /* mixed flow with the FP64 check (the default): 1 when
                            the Ritz pairs of the FP32 cycles failed the explicit
                            FP64 residual check and FP64 cycles polished them */
+  uint32_t matvecs_f16; /* of matvecs_low, those on the FP16 copy */
+  double check_resid;   /* mixed flow: max explicit FP64 residual of the FP32
+                           cycles' pairs at the FP64 check (0: no check ran) */
 } oaph_arnoldi_stats;
 oaph_status oaph_arnoldi_get_stats(oaph_arnoldi* a, oaph_arnoldi_stats* stats);
 
 /* Average duration (HIP events on the solver's stream) of the solver's own
  * operator application for the dense operator set with
- * oaph_arnoldi_set_matrix: FP64 (low = 0) or the FP32 copy (low = 1), as the
+ * oaph_arnoldi_set_matrix: FP64 (low = 0), the FP32 copy (low = 1) or the
+ * FP16 copy (low = 2), as the
  * last Arnoldi step of a cycle launches it (the matvec kernel also forms the
  * per-block partial sums of V^H y against the m basis vectors, the first
  * Gram-Schmidt pass's reduction, when its grid has at most 256 blocks). */
 oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* a, int low, uint32_t iters, double* avg_ms);
+/* y = A x with the solver's dense operator (oaph_arnoldi_set_matrix) at the
+ * given precision: 0 the FP64 planes, 1 the FP32 copy (FP64 products and
+ * sums: y is the FP64 product of the rounded operator with x), 2 the FP16
+ * copy (x rounded to FP32, FP32 products and per-lane sums, FP64 cross-lane
+ * sums); copies are made on first use; x and y complex device vectors of n
+ * elements.  No reference counterpart (testing and measurement). */
+oaph_status oaph_arnoldi_apply_operator(oaph_arnoldi* a, int low, const oaph_matrix* x,
+                                        oaph_matrix* y);
 /* Shader-clock stamps of the phases of the last restart kernel: load+Schur,
  * eigenvectors+estimates, ordering, wanted Ritz vectors, exact shifts
  * (6 values; consecutive differences are the phase durations in cycles). */
@@ -531,8 +556,9 @@ oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1
  * written dense into the re plane (im zeroed if present) with every
  * coefficient rounded to FP32.  A must be grid^2 x grid^2. */
 oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, double alpha);
-/* Launch shape of the streaming matvec (fp32 != 0: the FP32-operand form used
- * by OAPH_PREC_MIXED_FP32): variant index into a table of (rows per wave read
+/* Launch shape of the streaming matvec (fp32 = 1: the FP32-operand form used
+ * by OAPH_PREC_MIXED_FP32, 2: its FP16-operand form, 0: FP64): variant index
+ * into a table of (rows per wave read
  * together, loads in flight per row and plane, groups of such rows per wave
  * taken one after another); process-wide.  No reference counterpart (tuning
  * knob). */

@@ -50,7 +50,8 @@ class oaph_arnoldi_stats(C.Structure):
                 ("nconv", C.c_uint32), ("seconds", C.c_double), ("max_resid", C.c_double),
                 ("fallbacks", C.c_uint32), ("mixed_declined", C.c_uint32),
                 ("restarts_perfect_shift", C.c_uint32),
-                ("restarts_inverse_iteration", C.c_uint32), ("fp64_polish", C.c_uint32)]
+                ("restarts_inverse_iteration", C.c_uint32), ("fp64_polish", C.c_uint32),
+                ("matvecs_f16", C.c_uint32), ("check_resid", C.c_double)]
 
 
 class oaph_matrix_ex(C.Structure):
@@ -159,6 +160,8 @@ _SIGS = {
     "oaph_arnoldi_set_projection": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_mixed_tolerance": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_mixed_check": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "oaph_arnoldi_set_f16_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    "oaph_arnoldi_apply_operator": (C.c_int, [C.c_void_p, C.c_int, MP, MP]),
     "oaph_arnoldi_set_restart_guard_weight": (C.c_int, [C.c_void_p, C.c_double]),
     "oaph_arnoldi_set_schur_method": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_start_vector": (C.c_int, [C.c_void_p, DP, DP, U32]),

@@ -624,6 +624,12 @@ class CuHArnoldiCallback:
     def setMixedTolerance(self, tol_low):
         check(self._lib.oaph_arnoldi_set_mixed_tolerance(self._h, tol_low), "set_mixed_tolerance")
 
+    def setF16Phase(self, on=True, tol=2e-3):
+        """mixed flow: cycles on an FP16 copy of the operator (scaled by a
+        power of two) until the Ritz estimates reach tol (relative to the
+        largest Ritz value), then the FP32 phase from their Ritz-vector sum"""
+        check(self._lib.oaph_arnoldi_set_f16_phase(self._h, int(bool(on)), tol), "set_f16_phase")
+
     def setMixedCheck(self, on=True, fraction=0.5):
         """mixed flow after the FP32 phase: True (default) runs FP32 cycles
         down to fraction x the tolerance and accepts the pairs on an explicit
@@ -694,6 +700,13 @@ class CuHArnoldiCallback:
             return st
         check(st, "execute")
         return st
+
+    def applyOperator(self, precision: int, x: "DeviceMatrix", y: "DeviceMatrix"):
+        """y = A x with the dense operator at a precision: 0 FP64, 1 the FP32
+        copy (FP64 products and sums), 2 the FP16 copy (FP32 products and
+        per-lane sums)"""
+        check(self._lib.oaph_arnoldi_apply_operator(self._h, precision, x.ptr, y.ptr),
+              "apply_operator")
 
     def testOutcome(self, index: int) -> float:
         out = C.c_double()

@@ -20,6 +20,8 @@
 // and polishes to the requested tolerance; every returned pair is checked
 // with an explicit FP64 residual.
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <chrono>
 #include <numeric>
 #include <vector>
@@ -92,6 +94,19 @@ struct oaph_arnoldi {
   const oaph_matrix* a32_of = nullptr;
   bool a32_exact = false;   // every operator entry is an FP32 value
   bool a32_usable = false;  // the FP32-operand matvec can run on the copy
+  // FP16 copy of 2^f16_shift A (the first phase of the mixed flow): cycles on
+  // it down to tol_f16 (relative to max |Ritz value|), then the FP32 phase
+  // from the sum of its wanted and guard Ritz vectors
+  DevBuf A16;
+  const oaph_matrix* a16_of = nullptr;
+  bool a16_usable = false;
+  size_t a16_elems = 0;
+  int f16_shift = 0;
+  // Measured on the dense complex C1 operator (n = 8192, k = 8, m = 48, 32
+  // start vectors): no FP16 phase 91.7 eigenpairs/s; tol_f16 4e-3 103.1,
+  // 3e-3 103.3, 2e-3 105.6, 1.5e-3 102.9 (FP16 matvec 57 us vs FP32 83 us)
+  bool f16_phase = true;
+  double tol_f16 = 2e-3;
   Status host_status{};
   Status* pinned_status = nullptr;  // page-locked landing buffer of read_status
   std::vector<double> ev_re, ev_im, resid, xnorm2;
@@ -100,6 +115,16 @@ struct oaph_arnoldi {
 };
 
 namespace {
+
+// the solver's small device counters and flags (zeroed once; the kernels
+// that count reset their own)
+oaph_status ensure_counter(oaph_arnoldi* s) {
+  if (!s->counter.ptr) {
+    OAPH_TRY(s->counter.reserve(256));
+    OAPH_HIP(hipMemset(s->counter.ptr, 0, 256));
+  }
+  return OAPH_OK;
+}
 
 oaph_status alloc(oaph_arnoldi* s) {
   s->ky = s->k + s->guard < s->m ? s->k + s->guard : s->m;
@@ -118,10 +143,7 @@ oaph_status alloc(oaph_arnoldi* s) {
   size_t pb = orth_partials_bytes(s->n);
   size_t pp = pairs_partials_bytes(s->n, s->k);
   OAPH_TRY(s->partials.reserve(pb > pp ? pb : pp));
-  if (!s->counter.ptr) {
-    OAPH_TRY(s->counter.reserve(256));
-    OAPH_HIP(hipMemset(s->counter.ptr, 0, 256));
-  }
+  OAPH_TRY(ensure_counter(s));
   OAPH_TRY(s->status.reserve(sizeof(Status)));
   // T and Z between the comparator's two restart launches, and the general
   // restart kernel's Z (and T where it does not fit in LDS)
@@ -157,11 +179,18 @@ Work work_of(oaph_arnoldi* s) {
 
 // y = A x with x, y n-vectors (complex planes); counts the matvec
 oaph_status apply_operator(oaph_arnoldi* s, double* xre, double* xim, double* yre, double* yim,
-                           int type, bool low, MvProj* pj = nullptr) {
+                           int type, int low, MvProj* pj = nullptr) {
   hipStream_t st = s->api->stream;
   if (s->A) {
     const oaph_matrix* A = s->A;
-    if (low) {
+    if (low == kOpF16) {
+      const _Float16* a16 = s->A16.as<_Float16>();
+      OAPH_TRY(launch_hgemv16(st, s->n, s->n, A->columns, a16,
+                              A->imValues ? a16 + s->a16_elems : nullptr, xre, xim, yre, yim,
+                              std::ldexp(1.0, -s->f16_shift), pj));
+      s->stats.matvecs_low++;
+      s->stats.matvecs_f16++;
+    } else if (low) {
       const float* a32 = s->A32.as<float>();
       OAPH_TRY(launch_cgemv32(st, s->n, s->n, A->columns, a32,
                               A->imValues ? a32 + s->a32_elems : nullptr, xre, xim, yre, yim, pj));
@@ -199,6 +228,7 @@ oaph_status read_status(oaph_arnoldi* s) {
 // row length that is a multiple of 4, and an entry beyond the FP32 range
 // would turn into an infinity.  An unusable copy is not made; the solve then
 // runs FP64 cycles.
+oaph_status make_f16_copy(oaph_arnoldi* s);
 oaph_status make_fp32_copy(oaph_arnoldi* s) {
   hipStream_t st = s->api->stream;
   s->a32_of = s->A;
@@ -208,6 +238,7 @@ oaph_status make_fp32_copy(oaph_arnoldi* s) {
   const size_t elems = (size_t)s->A->rows * s->A->columns;
   OAPH_TRY(s->A32.reserve((s->A->imValues ? 2 : 1) * elems * sizeof(float)));
   s->a32_elems = elems;
+  OAPH_TRY(ensure_counter(s));
   float* a32 = s->A32.as<float>();
   unsigned* flags = s->counter.as<unsigned>() + 48;
   OAPH_HIP(hipMemsetAsync(flags, 0, 2 * sizeof(unsigned), st));
@@ -221,9 +252,41 @@ oaph_status make_fp32_copy(oaph_arnoldi* s) {
   return OAPH_OK;
 }
 
+// The FP16 copy of the dense operator (both planes), scaled by 2^f16_shift
+// so that the largest |entry| lands in [2^14, 2^15) (no FP16 overflow; entries
+// below 2^-39 of the largest flush to zero).  Usable when the row length is
+// a multiple of 8 (16-byte row loads of 8 values) and every entry is finite.
+oaph_status make_f16_copy(oaph_arnoldi* s) {
+  hipStream_t st = s->api->stream;
+  s->a16_of = s->A;
+  s->a16_usable = false;
+  if (s->A->columns % 8 != 0) return OAPH_OK;
+  const size_t elems = (size_t)s->A->rows * s->A->columns;
+  OAPH_TRY(ensure_counter(s));
+  unsigned long long* amax = reinterpret_cast<unsigned long long*>(s->counter.as<unsigned>() + 56);
+  OAPH_HIP(hipMemsetAsync(amax, 0, sizeof(unsigned long long), st));
+  OAPH_TRY(abs_max(st, elems, s->A->reValues, amax));
+  if (s->A->imValues) OAPH_TRY(abs_max(st, elems, s->A->imValues, amax));
+  unsigned long long h = 0;
+  OAPH_HIP(hipMemcpyAsync(&h, amax, sizeof(h), hipMemcpyDeviceToHost, st));
+  OAPH_HIP(hipStreamSynchronize(st));
+  double mx = 0;
+  std::memcpy(&mx, &h, sizeof(mx));
+  if (!(mx > 0) || !std::isfinite(mx)) return OAPH_OK;  // zero, infinite or NaN entries
+  s->f16_shift = 14 - std::ilogb(mx);
+  OAPH_TRY(s->A16.reserve((s->A->imValues ? 2 : 1) * elems * sizeof(_Float16)));
+  s->a16_elems = elems;
+  _Float16* a16 = s->A16.as<_Float16>();
+  const double scale = std::ldexp(1.0, s->f16_shift);
+  OAPH_TRY(to_f16(st, elems, s->A->reValues, scale, a16));
+  if (s->A->imValues) OAPH_TRY(to_f16(st, elems, s->A->imValues, scale, a16 + elems));
+  s->a16_usable = true;
+  return OAPH_OK;
+}
+
 // One Arnoldi step: column j of H and v_{j+1} (CGS2; ArnoldiProcedures.cpp:416-459)
 
-oaph_status arnoldi_step(oaph_arnoldi* s, uint32_t j, bool low) {
+oaph_status arnoldi_step(oaph_arnoldi* s, uint32_t j, int low) {
   hipStream_t st = s->api->stream;
   Work w = work_of(s);
   double* wre = s->W.as<double>();
@@ -335,7 +398,7 @@ oaph_status restart_with_comparator(oaph_arnoldi* s, RestartArgs ra) {
 // the watched ones switches the remaining cycles to the FP32 operand at the
 // watch's tolerance (a wanted eigenvalue turned up late: its discovery runs
 // on the cheaper matvec) and returns when they reach it.
-oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint32_t* cycles,
+oaph_status run_cycles(oaph_arnoldi* s, int low, double tol, double norm, uint32_t* cycles,
                        bool* converged, Watch* watch = nullptr) {
   hipStream_t st = s->api->stream;
   uint32_t start = 0;
@@ -412,7 +475,7 @@ oaph_status run_cycles(oaph_arnoldi* s, bool low, double tol, double norm, uint3
       // the factorisation goes on with FP32-operand steps (FP64 columns
       // already in it differ from FP32 ones by far less than the FP32
       // tolerance) until the new set reaches the FP32 phase's tolerance
-      low = true;
+      if (low == kOpFP64) low = kOpFP32;
       tol = watch->low_tol;
       norm = 0.0;
       watch->fell_back = true;
@@ -588,7 +651,7 @@ oaph_status oaph_arnoldi_destroy(oaph_arnoldi* s) {
   if (!s) return OAPH_OK;
   if (s->api) (void)hipStreamSynchronize(s->api->stream);
   for (DevBuf* b : {&s->V, &s->V2, &s->W, &s->F, &s->H, &s->Q, &s->Y, &s->X, &s->AX, &s->coef,
-                    &s->partials, &s->counter, &s->status, &s->A32, &s->TZ, &s->Vb})
+                    &s->partials, &s->counter, &s->status, &s->A32, &s->TZ, &s->Vb, &s->A16})
     b->release();
   if (s->pinned_status) (void)hipHostFree(s->pinned_status);
   oaph_api_destroy(s->api);
@@ -671,6 +734,12 @@ oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* s, double w) {
   return OAPH_OK;
 }
 
+oaph_status oaph_arnoldi_set_f16_phase(oaph_arnoldi* s, int on, double tol) {
+  OAPH_CHECK(s && tol > 0.0 && tol < 1.0, OAPH_ERR_ARG);
+  s->f16_phase = on != 0;
+  s->tol_f16 = tol;
+  return OAPH_OK;
+}
 oaph_status oaph_arnoldi_set_mixed_check(oaph_arnoldi* s, int on, double fraction) {
   OAPH_CHECK(s && fraction > 0.0 && fraction <= 1.0, OAPH_ERR_ARG);
   s->mixed_check = on != 0;
@@ -831,7 +900,18 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     // FP64 polish starts again from its Ritz-vector sum (at most
     // kMaxFallbacks times; then the FP64 cycles finish on their own).
     constexpr int kMaxFallbacks = 3;
-    OAPH_TRY(run_cycles(s, true, s->tol_low, 0.0, &cycles, &converged));
+    if (s->f16_phase && s->check == OAPH_CHECK_INTERNAL) {
+      // FP16-operand cycles (half the bytes of the FP32 copy) until the Ritz
+      // estimates reach tol_f16; the FP32 phase starts from the sum of their
+      // wanted and guard Ritz vectors
+      if (s->a16_of != s->A) OAPH_TRY(make_f16_copy(s));
+      if (s->a16_usable) {
+        uint32_t cb = 0;
+        OAPH_TRY(run_cycles(s, kOpF16, s->tol_f16, 0.0, &cb, &converged));
+        OAPH_TRY(restart_from_ritz_sum(s->ky));
+      }
+    }
+    OAPH_TRY(run_cycles(s, kOpFP32, s->tol_low, 0.0, &cycles, &converged));
     bool accepted = false;
     if (s->mixed_check && s->check == OAPH_CHECK_INTERNAL) {
       // FP32-operand cycles from the Ritz-vector sum down to a fraction of
@@ -852,6 +932,7 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
       if (converged) {
         const int bd2 = s->host_status.breakdown;
         OAPH_TRY(ritz_pairs(s, bd2 > 0 ? (uint32_t)bd2 : s->m));
+        s->stats.check_resid = s->stats.max_resid;
         accepted = s->stats.max_resid <= s->tol * norm;
       }
       if (!accepted) s->stats.fp64_polish = 1;
@@ -905,8 +986,11 @@ oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* s, int low, uint32_t iters, d
     s->k = s->k ? s->k : 1;
     OAPH_TRY(alloc(s));
   }
-  if (low && s->a32_of != s->A) OAPH_TRY(make_fp32_copy(s));
-  OAPH_CHECK(!low || s->a32_usable, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(low >= kOpFP64 && low <= kOpF16, OAPH_ERR_ARG);
+  if (low == kOpFP32 && s->a32_of != s->A) OAPH_TRY(make_fp32_copy(s));
+  if (low == kOpF16 && s->a16_of != s->A) OAPH_TRY(make_f16_copy(s));
+  OAPH_CHECK(low != kOpFP32 || s->a32_usable, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(low != kOpF16 || s->a16_usable, OAPH_ERR_UNSUPPORTED);
   double* xre = Vre(s, s->V, 0);
   double* xim = Vim(s, s->V, 0);
   double* yre = s->W.as<double>();
@@ -916,13 +1000,13 @@ oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* s, int low, uint32_t iters, d
   // as the last Arnoldi step of a cycle launches it: with the partial sums of
   // the first Gram-Schmidt pass against m basis vectors
   MvProj pj = cgs_matvec_projection(work_of(s), s->m);
-  OAPH_TRY(apply_operator(s, xre, xim, yre, yim, OAPH_TYPE_WV, low != 0, &pj));
+  OAPH_TRY(apply_operator(s, xre, xim, yre, yim, OAPH_TYPE_WV, low, &pj));
   hipEvent_t e0, e1;
   OAPH_HIP(hipEventCreate(&e0));
   OAPH_HIP(hipEventCreate(&e1));
   OAPH_HIP(hipEventRecord(e0, st));
   for (uint32_t i = 0; i < iters; ++i)
-    OAPH_TRY(apply_operator(s, xre, xim, yre, yim, OAPH_TYPE_WV, low != 0, &pj));
+    OAPH_TRY(apply_operator(s, xre, xim, yre, yim, OAPH_TYPE_WV, low, &pj));
   OAPH_HIP(hipEventRecord(e1, st));
   OAPH_HIP(hipEventSynchronize(e1));
   float ms = 0;
@@ -931,6 +1015,31 @@ oaph_status oaph_arnoldi_time_matvec(oaph_arnoldi* s, int low, uint32_t iters, d
   (void)hipEventDestroy(e1);
   s->stats = keep;
   *avg_ms = ms / iters;
+  return OAPH_OK;
+}
+
+oaph_status oaph_arnoldi_apply_operator(oaph_arnoldi* s, int low, const oaph_matrix* x,
+                                        oaph_matrix* y) {
+  OAPH_CHECK(s && x && y, OAPH_ERR_ARG);
+  OAPH_CHECK(low >= kOpFP64 && low <= kOpF16, OAPH_ERR_ARG);
+  OAPH_CHECK(s->A != nullptr, OAPH_ERR_STATE);
+  const uint32_t n = s->A->rows;
+  OAPH_CHECK((size_t)x->columns * x->rows == n && (size_t)y->columns * y->rows == n,
+             OAPH_ERR_SHAPE);
+  OAPH_CHECK(x->reValues && x->imValues && y->reValues && y->imValues, OAPH_ERR_PLANE);
+  if (low == kOpFP32 && s->a32_of != s->A) OAPH_TRY(make_fp32_copy(s));
+  if (low == kOpF16 && s->a16_of != s->A) OAPH_TRY(make_f16_copy(s));
+  OAPH_CHECK(low != kOpFP32 || s->a32_usable, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(low != kOpF16 || s->a16_usable, OAPH_ERR_UNSUPPORTED);
+  const uint32_t keep_n = s->n;
+  s->n = n;
+  auto keep = s->stats;
+  const oaph_status rc = apply_operator(s, x->reValues, x->imValues, y->reValues, y->imValues,
+                                        OAPH_TYPE_WV, low);
+  s->stats = keep;
+  s->n = keep_n;
+  OAPH_TRY(rc);
+  OAPH_HIP(hipStreamSynchronize(s->api->stream));
   return OAPH_OK;
 }
 

@@ -228,22 +228,31 @@ __global__ __launch_bounds__(256) void k_zgemv_scalar(uint32_t rows, uint32_t co
   }
 }
 
-// ---------------------------------------------------------------- complex FP32-operand GEMV
-// A stored as FP32 planes (the mixed-precision copy of the operator), x and y
-// FP64, products and sums in FP64: the only rounding beyond FP64 is the
-// storage of A.  4 elements (16 B) per lane per load.
+// ---------------------------------------------------------------- reduced-precision-operand GEMV
+// A stored in a reduced-precision copy (the mixed-precision solver's FP32
+// copy of the operator), x and y FP64, products and sums in FP64: the only
+// rounding beyond FP64 is the storage of A.  One load per lane brings 4
+// consecutive row elements (16 B of FP32); OP::at widens element i of a load
+// to FP64 exactly.
 typedef float v4f __attribute__((ext_vector_type(4)));
+struct OpF32 {
+  using elem = float;
+  using vec = v4f;
+  static __device__ __forceinline__ double at(const vec& v, int i) {
+    return i == 0 ? v.x : i == 1 ? v.y : i == 2 ? v.z : v.w;
+  }
+};
 // CPLX = false: real A (aim unused), complex x -- half the operator bytes
 // G groups of R rows per wave, one group after another.
-template <int R, int U, bool CPLX = true, int G = 1>
-__global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, uint32_t lda,
-                                                 const float* __restrict__ are,
-                                                 const float* __restrict__ aim,
-                                                 const double* __restrict__ xre,
-                                                 const double* __restrict__ xim,
-                                                 double* __restrict__ yre,
-                                                 double* __restrict__ yim, MvProj pj) {
-  __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
+template <class OP, int R, int U, bool CPLX, int G>
+__device__ __forceinline__ void lowp_gemv(uint32_t rows, uint32_t cols, uint32_t lda,
+                                          const typename OP::elem* __restrict__ are,
+                                          const typename OP::elem* __restrict__ aim,
+                                          const double* __restrict__ xre,
+                                          const double* __restrict__ xim,
+                                          double* __restrict__ yre, double* __restrict__ yim,
+                                          const MvProj& pj, double2* ys) {
+  using vec = typename OP::vec;
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t nquad = cols >> 2;
@@ -257,17 +266,17 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
   for (int q = 0; q < R; ++q) { sr[q] = 0.0; si[q] = 0.0; }
   const v2d* xr2 = reinterpret_cast<const v2d*>(xre);
   const v2d* xi2 = reinterpret_cast<const v2d*>(xim);
-  const v4f* ar4[R];
-  const v4f* ai4[R];
+  const vec* ar4[R];
+  const vec* ai4[R];
 #pragma unroll
   for (int q = 0; q < R; ++q) {
     uint32_t r = r0 + q < rows ? r0 + q : rows - 1;
-    ar4[q] = reinterpret_cast<const v4f*>(are + (size_t)r * lda);
-    ai4[q] = CPLX ? reinterpret_cast<const v4f*>(aim + (size_t)r * lda) : nullptr;
+    ar4[q] = reinterpret_cast<const vec*>(are + (size_t)r * lda);
+    ai4[q] = CPLX ? reinterpret_cast<const vec*>(aim + (size_t)r * lda) : nullptr;
   }
   for (uint32_t p = lane; p < nquad; p += 64 * U) {
     v2d xa[U][2], xb[U][2];
-    v4f a[R][U], b[R][U];
+    vec a[R][U], b[R][U];
 #pragma unroll
     for (int u = 0; u < U; ++u) {
       uint32_t pp = p + 64 * u;
@@ -291,18 +300,21 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
 #pragma unroll
       for (int u = 0; u < U; ++u) {
         if (p + 64 * u < nquad) {
-          double a0 = a[q][u].x, a1 = a[q][u].y, a2 = a[q][u].z, a3 = a[q][u].w;
+          double a0 = OP::at(a[q][u], 0), a1 = OP::at(a[q][u], 1), a2 = OP::at(a[q][u], 2),
+                 a3 = OP::at(a[q][u], 3);
           sr[q] = fma(a0, xa[u][0].x, sr[q]); sr[q] = fma(a1, xa[u][0].y, sr[q]);
           sr[q] = fma(a2, xa[u][1].x, sr[q]); sr[q] = fma(a3, xa[u][1].y, sr[q]);
           if (CPLX) {
-            double b0 = b[q][u].x, b1 = b[q][u].y, b2 = b[q][u].z, b3 = b[q][u].w;
+            double b0 = OP::at(b[q][u], 0), b1 = OP::at(b[q][u], 1), b2 = OP::at(b[q][u], 2),
+                   b3 = OP::at(b[q][u], 3);
             sr[q] = fma(-b0, xb[u][0].x, sr[q]); sr[q] = fma(-b1, xb[u][0].y, sr[q]);
             sr[q] = fma(-b2, xb[u][1].x, sr[q]); sr[q] = fma(-b3, xb[u][1].y, sr[q]);
           }
           si[q] = fma(a0, xb[u][0].x, si[q]); si[q] = fma(a1, xb[u][0].y, si[q]);
           si[q] = fma(a2, xb[u][1].x, si[q]); si[q] = fma(a3, xb[u][1].y, si[q]);
           if (CPLX) {
-            double b0 = b[q][u].x, b1 = b[q][u].y, b2 = b[q][u].z, b3 = b[q][u].w;
+            double b0 = OP::at(b[q][u], 0), b1 = OP::at(b[q][u], 1), b2 = OP::at(b[q][u], 2),
+                   b3 = OP::at(b[q][u], 3);
             si[q] = fma(b0, xa[u][0].x, si[q]); si[q] = fma(b1, xa[u][0].y, si[q]);
             si[q] = fma(b2, xa[u][1].x, si[q]); si[q] = fma(b3, xa[u][1].y, si[q]);
           }
@@ -323,6 +335,124 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
     yim[r0 + lane] = vi;
     ys[((threadIdx.x >> 6) * G + g) * R + lane] = make_double2(vr, vi);
   }
+  }
+  if (pj.part) {  // uniform over the grid
+    __syncthreads();
+    block_project<4 * G * R>(pj, rows, ys, pv);
+  }
+}
+
+// FP32 operand (the mixed-precision solver's FP32 copy)
+template <int R, int U, bool CPLX = true, int G = 1>
+__global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, uint32_t lda,
+                                                 const float* __restrict__ are,
+                                                 const float* __restrict__ aim,
+                                                 const double* __restrict__ xre,
+                                                 const double* __restrict__ xim,
+                                                 double* __restrict__ yre,
+                                                 double* __restrict__ yim, MvProj pj) {
+  __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
+  lowp_gemv<OpF32, R, U, CPLX, G>(rows, cols, lda, are, aim, xre, xim, yre, yim, pj, ys);
+}
+
+// ---------------------------------------------------------------- FP16-operand GEMV
+// A stored as FP16 values of 2^s A (the mixed-precision solver's 16-bit copy,
+// 2 bytes per value; the power of two keeps the largest entry below 2^15),
+// x rounded to FP32 as it is loaded, products and per-lane sums in FP32 --
+// v_fma_mix_f32 takes the FP16 halves of a load as they are, one instruction
+// per product -- then the cross-lane sums in FP64 and y = 2^-s times them.
+// 8 elements (16 B) per lane per load, so rows need cols % 8 == 0.
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+template <int R, int U, bool CPLX = true, int G = 1>
+__global__ __launch_bounds__(256) void k_hgemv16(uint32_t rows, uint32_t cols, uint32_t lda,
+                                                 const _Float16* __restrict__ are,
+                                                 const _Float16* __restrict__ aim,
+                                                 const double* __restrict__ xre,
+                                                 const double* __restrict__ xim,
+                                                 double* __restrict__ yre,
+                                                 double* __restrict__ yim, double unscale,
+                                                 MvProj pj) {
+  __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
+  const int lane = threadIdx.x & 63;
+  const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t noct = cols >> 3;
+  ProjRegs<4 * G * R> pv;
+  if (pj.part) project_prefetch(pj, rows, pv);
+  const v2d* xr2 = reinterpret_cast<const v2d*>(xre);
+  const v2d* xi2 = reinterpret_cast<const v2d*>(xim);
+  for (int g = 0; g < G; ++g) {
+    const uint32_t r0 = (wave * G + g) * R;
+    if (r0 >= rows) break;
+    float sr[R], si[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) { sr[q] = 0.0f; si[q] = 0.0f; }
+    const h8* ar8[R];
+    const h8* ai8[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      uint32_t r = r0 + q < rows ? r0 + q : rows - 1;
+      ar8[q] = reinterpret_cast<const h8*>(are + (size_t)r * lda);
+      ai8[q] = CPLX ? reinterpret_cast<const h8*>(aim + (size_t)r * lda) : nullptr;
+    }
+    for (uint32_t p = lane; p < noct; p += 64 * U) {
+      float xa[U][8], xb[U][8];
+      h8 a[R][U], b[R][U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        uint32_t pp = p + 64 * u;
+        if (pp < noct) {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const v2d tr = xr2[4 * pp + e], ti = xi2[4 * pp + e];
+            xa[u][2 * e] = (float)tr.x; xa[u][2 * e + 1] = (float)tr.y;
+            xb[u][2 * e] = (float)ti.x; xb[u][2 * e + 1] = (float)ti.y;
+          }
+        }
+      }
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          uint32_t pp = p + 64 * u;
+          if (pp < noct) {
+            a[q][u] = __builtin_nontemporal_load(ar8[q] + pp);
+            if (CPLX) b[q][u] = __builtin_nontemporal_load(ai8[q] + pp);
+          }
+        }
+#pragma unroll
+      for (int q = 0; q < R; ++q)
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+          if (p + 64 * u < noct) {
+#pragma unroll
+            for (int e = 0; e < 8; ++e) {
+              const float av = (float)a[q][u][e];
+              sr[q] = __builtin_fmaf(av, xa[u][e], sr[q]);
+              si[q] = __builtin_fmaf(av, xb[u][e], si[q]);
+              if (CPLX) {
+                const float bv = (float)b[q][u][e];
+                sr[q] = __builtin_fmaf(-bv, xb[u][e], sr[q]);
+                si[q] = __builtin_fmaf(bv, xa[u][e], si[q]);
+              }
+            }
+          }
+        }
+    }
+    double dr[R], di[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      dr[q] = wave_sum((double)sr[q]) * unscale;
+      di[q] = wave_sum((double)si[q]) * unscale;
+    }
+    if (lane < R && r0 + lane < rows) {
+      double vr = dr[0], vi = di[0];
+#pragma unroll
+      for (int q = 1; q < R; ++q)
+        if (lane == q) { vr = dr[q]; vi = di[q]; }
+      yre[r0 + lane] = vr;
+      yim[r0 + lane] = vi;
+      ys[((threadIdx.x >> 6) * G + g) * R + lane] = make_double2(vr, vi);
+    }
   }
   if (pj.part) {  // uniform over the grid
     __syncthreads();
@@ -436,9 +566,15 @@ constexpr Shape kZShapes[] = {{2, 3, 4}, {4, 2, 1}, {2, 4, 4}, {4, 3, 2}, {2, 6,
                               {2, 4, 2}, {4, 6, 2}, {8, 2, 1}};
 constexpr Shape kCShapes[] = {{4, 3, 2}, {4, 3, 1}, {4, 2, 2}, {4, 4, 2}, {2, 2, 4},
                               {2, 3, 4}, {8, 2, 1}, {4, 2, 1}};
+// FP16 operand: 16 B per load as for FP32, but twice the elements; all keep
+// R G = 8 (the fused projection epilogue's 256 workgroups at 8192 rows).
+// Entry 0 measured fastest at n = 8192 (57.2 us; (8, 1, 1) 70.3 us).
+constexpr Shape kHShapes[] = {{4, 2, 2}, {8, 1, 1}, {8, 2, 1}, {2, 4, 4}, {4, 1, 2},
+                              {4, 3, 2}, {2, 2, 4}, {8, 3, 1}};
 constexpr int kNumZ = sizeof(kZShapes) / sizeof(kZShapes[0]);
 constexpr int kNumC = sizeof(kCShapes) / sizeof(kCShapes[0]);
-int g_zshape = 0, g_cshape = 0;
+constexpr int kNumH = sizeof(kHShapes) / sizeof(kHShapes[0]);
+int g_zshape = 0, g_cshape = 0, g_hshape = 0;
 // dotProducts on the matrix cores: 1 large ones (gemm_mfma_eligible), 2 all,
 // 0 none (the vector-FMA tiles)
 int g_gemm_mfma = 1;
@@ -447,6 +583,8 @@ using ZFn = void (*)(uint32_t, uint32_t, uint32_t, const double*, const double*,
                      const double*, double*, double*, MvProj);
 using CFn = void (*)(uint32_t, uint32_t, uint32_t, const float*, const float*, const double*,
                      const double*, double*, double*, MvProj);
+using HFn = void (*)(uint32_t, uint32_t, uint32_t, const _Float16*, const _Float16*,
+                     const double*, const double*, double*, double*, double, MvProj);
 
 // kZ / kC: each shape as listed; kZ1 / kC1: what runs when the grouped grid
 // would leave SIMDs without a wave -- the shape itself when G = 1, else the
@@ -467,6 +605,17 @@ template <bool CPLX>
 constexpr CFn kC1[] = {k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
                        k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
                        k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
+template <bool CPLX>
+constexpr HFn kH[] = {k_hgemv16<4, 2, CPLX, 2>, k_hgemv16<8, 1, CPLX, 1>, k_hgemv16<8, 2, CPLX, 1>,
+                      k_hgemv16<2, 4, CPLX, 4>, k_hgemv16<4, 1, CPLX, 2>, k_hgemv16<4, 3, CPLX, 2>,
+                      k_hgemv16<2, 2, CPLX, 4>, k_hgemv16<8, 3, CPLX, 1>};
+// G = 1 stand-ins (the shape itself when G = 1, else four rows per wave)
+template <bool CPLX>
+constexpr HFn kH1[] = {k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<8, 1, CPLX, 1>, k_hgemv16<8, 2, CPLX, 1>,
+                       k_hgemv16<4, 4, CPLX, 1>, k_hgemv16<4, 1, CPLX, 1>, k_hgemv16<4, 3, CPLX, 1>,
+                       k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<8, 3, CPLX, 1>};
+static_assert(sizeof(kH<true>) / sizeof(HFn) == kNumH, "one kernel per FP16 shape");
+static_assert(sizeof(kH1<true>) / sizeof(HFn) == kNumH, "one kernel per FP16 shape");
 static_assert(sizeof(kZ1<0>) / sizeof(ZFn) == kNumZ, "one kernel per FP64 shape");
 static_assert(sizeof(kC1<true>) / sizeof(CFn) == kNumC, "one kernel per FP32 shape");
 // At most kSmallRows rows without row groups: two rows per wave, six loads in
@@ -478,6 +627,8 @@ template <int MODE>
 constexpr ZFn kZs = k_zgemv<2, 6, MODE, 1>;
 template <bool CPLX>
 constexpr CFn kCs = k_cgemv32<2, 6, CPLX, 1>;
+template <bool CPLX>
+constexpr HFn kHs = k_hgemv16<2, 4, CPLX, 1>;
 
 // Waves the device runs at once with one per SIMD (4 SIMDs per CU).
 unsigned simd_count() {
@@ -570,6 +721,32 @@ oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   return OAPH_OK;
 }
 
+oaph_status launch_hgemv16(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
+                           const _Float16* are, const _Float16* aim, const double* xre,
+                           const double* xim, double* yre, double* yim, double unscale,
+                           MvProj* pj) {
+  OAPH_CHECK(cols % 8 == 0 && lda % 8 == 0, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(((((uintptr_t)are) | ((uintptr_t)aim) | ((uintptr_t)xre) | ((uintptr_t)xim)) & 15) == 0,
+             OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(xim != nullptr && yim != nullptr, OAPH_ERR_PLANE);
+  const Shape& sh = kHShapes[g_hshape];
+  const bool grp = sh.G == 1 || use_groups(sh, rows);
+  const bool small = !grp && rows <= kSmallRows;
+  // rows per wave of the kernel that runs: kH1 takes the shape itself when
+  // G = 1 and four rows per wave otherwise; kHs two
+  const int RG = grp ? sh.R * sh.G : (small ? 2 : (sh.G == 1 ? sh.R : 4));
+  const unsigned waves = (rows + RG - 1) / RG;
+  const dim3 grid((waves + 3) / 4);
+  const HFn fn = grp ? (aim ? kH<true>[g_hshape] : kH<false>[g_hshape])
+                 : small ? (aim ? kHs<true> : kHs<false>)
+                     : (aim ? kH1<true>[g_hshape] : kH1<false>[g_hshape]);
+  const MvProj kp = fused_projection(pj, true, grid.x);
+  hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim,
+                     unscale, kp);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
 oaph_status launch_gemm(hipStream_t s, oaph_matrix* out, const oaph_matrix* a,
                         const oaph_matrix* b, uint32_t W, uint32_t H, uint32_t K,
                         bool dims_mode) {
@@ -634,7 +811,10 @@ oaph_status oaph_set_gemm_path(int mfma) {
 
 oaph_status oaph_set_matvec_shape(int fp32, int variant) {
   using namespace oaph;
-  if (fp32) {
+  if (fp32 == 2) {
+    OAPH_CHECK(variant >= 0 && variant < kNumH, OAPH_ERR_ARG);
+    g_hshape = variant;
+  } else if (fp32) {
     OAPH_CHECK(variant >= 0 && variant < kNumC, OAPH_ERR_ARG);
     g_cshape = variant;
   } else {
@@ -647,9 +827,9 @@ oaph_status oaph_set_matvec_shape(int fp32, int variant) {
 oaph_status oaph_matvec_shape_info(int fp32, int variant, int* rows_per_wave, int* loads_in_flight,
                                    int* row_groups) {
   using namespace oaph;
-  const int n = fp32 ? kNumC : kNumZ;
+  const int n = fp32 == 2 ? kNumH : fp32 ? kNumC : kNumZ;
   OAPH_CHECK(variant >= 0 && variant < n, OAPH_ERR_ARG);
-  const Shape sh = fp32 ? kCShapes[variant] : kZShapes[variant];
+  const Shape sh = fp32 == 2 ? kHShapes[variant] : fp32 ? kCShapes[variant] : kZShapes[variant];
   if (rows_per_wave) *rows_per_wave = sh.R;
   if (loads_in_flight) *loads_in_flight = sh.U;
   if (row_groups) *row_groups = sh.G;

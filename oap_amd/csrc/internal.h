@@ -136,6 +136,12 @@ oaph_status launch_gemm_nt(hipStream_t s, uint32_t W, uint32_t H, uint32_t K, co
 oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                            const float* are, const float* aim, const double* xre,
                            const double* xim, double* yre, double* yim, MvProj* pj = nullptr);
+// FP16 operand holding 2^s A (aim NULL: real operator), complex FP64 x and
+// y: products and per-lane sums in FP32, y = unscale (= 2^-s) times the sums
+oaph_status launch_hgemv16(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
+                           const _Float16* are, const _Float16* aim, const double* xre,
+                           const double* xim, double* yre, double* yim, double unscale,
+                           MvProj* pj = nullptr);
 // small.hip
 oaph_status launch_qrgr(hipStream_t s, oaph_matrix* Q, oaph_matrix* R, const oaph_matrix* A);
 oaph_status launch_is_upper_triangular(hipStream_t s, const oaph_matrix* m, int* dflag);

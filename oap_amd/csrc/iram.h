@@ -109,7 +109,15 @@ oaph_status pairs(hipStream_t s, int mode, uint32_t n, uint32_t k, const double*
 size_t pairs_partials_bytes(uint32_t n, uint32_t k);
 // dst = (float)src; flags (may be NULL): flags[0] set to 1 when an element
 // rounds, flags[1] when a finite element overflows the FP32 range
+// precision of the operator a matvec reads (the solver's FP64 operator or
+// its reduced-precision copies)
+enum : int { kOpFP64 = 0, kOpFP32 = 1, kOpF16 = 2 };
 oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsigned* flags);
+// *amax <- max(*amax, max |src|) over n values (bit patterns of non-negative
+// doubles order as unsigned integers; a NaN sorts above every number)
+oaph_status abs_max(hipStream_t s, size_t n, const double* src, unsigned long long* amax);
+// dst = FP16 of src * scale (round to nearest even, once)
+oaph_status to_f16(hipStream_t s, size_t n, const double* src, double scale, _Float16* dst);
 // V[:, 0] = sum of the first k columns of X plus w times the next count - k
 oaph_status colsum(hipStream_t s, uint32_t n, uint32_t k, uint32_t count, double w,
                    const double* Xre, const double* Xim, double* vre, double* vim);

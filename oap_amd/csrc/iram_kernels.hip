@@ -693,6 +693,32 @@ __global__ void k_to_f32(size_t n, const double* __restrict__ src, float* __rest
   if (over && flags) flags[1] = 1u;
 }
 
+// largest |value| of an operator plane: per-thread maxima, a wave maximum,
+// one atomic max per wave on the bit pattern
+__global__ void k_abs_max(size_t n, const double* __restrict__ src, unsigned long long* amax) {
+  unsigned long long m = 0;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(fabs(src[i]));
+    m = b > m ? b : m;
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned long long o = __shfl_xor(m, off, 64);
+    m = o > m ? o : m;
+  }
+  if ((threadIdx.x & 63) == 0 && m) atomicMax(amax, m);
+}
+
+// FP64 -> FP16 copy of an operator plane scaled by a power of two: the FP16
+// rounding (nearest even, one rounding) of src * scale (the product is exact)
+__global__ void k_to_f16(size_t n, const double* __restrict__ src, double scale,
+                         _Float16* __restrict__ dst) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = (_Float16)(src[i] * scale);
+}
+
 // x = sum_{t < k} X[:, t] + w sum_{k <= t < count} X[:, t] (start vector of
 // a Ritz-sum restart: the wanted Ritz vectors and the weighted guards),
 // written to V[:, 0]
@@ -965,6 +991,22 @@ oaph_status to_f32(hipStream_t s, size_t n, const double* src, float* dst, unsig
   size_t blocks = (n + 255) / 256;
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_to_f32, dim3((unsigned)blocks), dim3(256), 0, s, n, src, dst, flags);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status abs_max(hipStream_t s, size_t n, const double* src, unsigned long long* amax) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_abs_max, dim3((unsigned)blocks), dim3(256), 0, s, n, src, amax);
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status to_f16(hipStream_t s, size_t n, const double* src, double scale, _Float16* dst) {
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_to_f16, dim3((unsigned)blocks), dim3(256), 0, s, n, src, scale, dst);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

@@ -80,6 +80,14 @@ def test_c1_top8_against_fixture(c1, precision, projection):
               f"max |lambda - fixture| {max(err):.3e}, stats {s.stats()}", flush=True)
         assert sorted(idx) == list(range(K)), (seed, sorted(idx))
         assert max(err) <= 1e-10 * lam1, (seed, max(err))
+        if precision == Precision.MIXED_FP32:
+            # the FP16 phase ran, and the FP32 cycles' pairs passed the FP64
+            # check with room to spare (0.24-0.66 of the tolerance measured):
+            # no FP64 cycle
+            st = s.stats()
+            assert st["matvecs_f16"] > 0 and st["fp64_polish"] == 0, (seed, st)
+            assert 0 < st["check_resid"] <= 0.8e-10 * FIX["fro"], (seed, st["check_resid"])
+            assert st["matvecs_low"] == st["matvecs"] - K
         for t in range(K):
             assert s.testOutcome(t) <= 1e-10 * FIX["fro"], (seed, t, s.testOutcome(t))
     # the returned vectors of the last solve are eigenvectors of the same
