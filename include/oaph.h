@@ -557,7 +557,8 @@ oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1
  * coefficient rounded to FP32.  A must be grid^2 x grid^2. */
 oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, double alpha);
 /* Launch shape of the streaming matvec (fp32 = 1: the FP32-operand form used
- * by OAPH_PREC_MIXED_FP32, 2: its FP16-operand form, 0: FP64): variant index
+ * by OAPH_PREC_MIXED_FP32, 2: its FP16-operand form, 0: FP64; fp32 = 3 sets
+ * whether the FP16-operand form stages x in LDS, variant 1 (default) or 0): variant index
  * into a table of (rows per wave read
  * together, loads in flight per row and plane, groups of such rows per wave
  * taken one after another); process-wide.  No reference counterpart (tuning

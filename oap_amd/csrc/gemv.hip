@@ -363,7 +363,80 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
 // per product -- then the cross-lane sums in FP64 and y = 2^-s times them.
 // 8 elements (16 B) per lane per load, so rows need cols % 8 == 0.
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
-template <int R, int U, bool CPLX = true, int G = 1>
+// V octets (8 columns each) of R rows at octet index p, p + 64, ...: the A
+// loads (hgemv_load) and the products (hgemv_fma, x read at p: XL FP32 x
+// staged in LDS, else FP64 x from global memory converted on arrival).  No
+// octet index may be out of range (no guards, so no branch splits the loads).
+typedef float f4v __attribute__((ext_vector_type(4)));
+template <int R, int V, bool CPLX>
+struct HOct {
+  h8 a[R][V], b[R][V];
+};
+template <int R, int V, bool CPLX>
+__device__ __forceinline__ void hgemv_load(HOct<R, V, CPLX>& o, uint32_t p,
+                                           const h8* const (&ar8)[R], const h8* const (&ai8)[R]) {
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int u = 0; u < V; ++u) {
+      o.a[q][u] = __builtin_nontemporal_load(ar8[q] + p + 64 * u);
+      if (CPLX) o.b[q][u] = __builtin_nontemporal_load(ai8[q] + p + 64 * u);
+    }
+}
+template <int R, int V, bool CPLX, bool XL>
+__device__ __forceinline__ void hgemv_fma(const HOct<R, V, CPLX>& o, uint32_t p, const v2d* xr2,
+                                          const v2d* xi2, const float* xs, uint32_t xsim,
+                                          float (&sr)[R], float (&si)[R]) {
+  float xa[V][8], xb[V][8];
+  if (XL) {
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const f4v r4 = *reinterpret_cast<const f4v*>(xs + 8 * (p + 64 * u) + 4 * h);
+        const f4v i4 = *reinterpret_cast<const f4v*>(xs + xsim + 8 * (p + 64 * u) + 4 * h);
+        xa[u][4 * h] = r4.x; xa[u][4 * h + 1] = r4.y; xa[u][4 * h + 2] = r4.z; xa[u][4 * h + 3] = r4.w;
+        xb[u][4 * h] = i4.x; xb[u][4 * h + 1] = i4.y; xb[u][4 * h + 2] = i4.z; xb[u][4 * h + 3] = i4.w;
+      }
+  } else {
+    v2d tr[V][4], ti[V][4];
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        tr[u][e] = xr2[4 * (p + 64 * u) + e];
+        ti[u][e] = xi2[4 * (p + 64 * u) + e];
+      }
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        xa[u][2 * e] = (float)tr[u][e].x; xa[u][2 * e + 1] = (float)tr[u][e].y;
+        xb[u][2 * e] = (float)ti[u][e].x; xb[u][2 * e + 1] = (float)ti[u][e].y;
+      }
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int u = 0; u < V; ++u)
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        const float av = (float)o.a[q][u][e];
+        sr[q] = __builtin_fmaf(av, xa[u][e], sr[q]);
+        si[q] = __builtin_fmaf(av, xb[u][e], si[q]);
+        if (CPLX) {
+          const float bv = (float)o.b[q][u][e];
+          sr[q] = __builtin_fmaf(-bv, xb[u][e], sr[q]);
+          si[q] = __builtin_fmaf(bv, xa[u][e], si[q]);
+        }
+      }
+}
+
+// x staged in LDS as FP32 when the row length is at most kXLdsCols (64 KiB
+// per workgroup): one coalesced pass over x per workgroup instead of every
+// wave reading FP64 x from L2 and converting it each step
+constexpr uint32_t kXLdsCols = 8192;
+template <int R, int U, bool CPLX = true, int G = 1, bool XL = false>
 __global__ __launch_bounds__(256) void k_hgemv16(uint32_t rows, uint32_t cols, uint32_t lda,
                                                  const _Float16* __restrict__ are,
                                                  const _Float16* __restrict__ aim,
@@ -373,6 +446,7 @@ __global__ __launch_bounds__(256) void k_hgemv16(uint32_t rows, uint32_t cols, u
                                                  double* __restrict__ yim, double unscale,
                                                  MvProj pj) {
   __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
+  __shared__ float xs[XL ? 2 * kXLdsCols : 1];
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
   const uint32_t noct = cols >> 3;
@@ -380,6 +454,13 @@ __global__ __launch_bounds__(256) void k_hgemv16(uint32_t rows, uint32_t cols, u
   if (pj.part) project_prefetch(pj, rows, pv);
   const v2d* xr2 = reinterpret_cast<const v2d*>(xre);
   const v2d* xi2 = reinterpret_cast<const v2d*>(xim);
+  if (XL) {
+    for (uint32_t j = threadIdx.x; j < cols; j += 256) {
+      xs[j] = (float)xre[j];
+      xs[kXLdsCols + j] = (float)xim[j];
+    }
+    __syncthreads();
+  }
   for (int g = 0; g < G; ++g) {
     const uint32_t r0 = (wave * G + g) * R;
     if (r0 >= rows) break;
@@ -392,51 +473,29 @@ __global__ __launch_bounds__(256) void k_hgemv16(uint32_t rows, uint32_t cols, u
     for (int q = 0; q < R; ++q) {
       uint32_t r = r0 + q < rows ? r0 + q : rows - 1;
       ar8[q] = reinterpret_cast<const h8*>(are + (size_t)r * lda);
-      ai8[q] = CPLX ? reinterpret_cast<const h8*>(aim + (size_t)r * lda) : nullptr;
+      ai8[q] = CPLX ? reinterpret_cast<const h8*>(aim + (size_t)r * lda) : ar8[q];
     }
-    for (uint32_t p = lane; p < noct; p += 64 * U) {
-      float xa[U][8], xb[U][8];
-      h8 a[R][U], b[R][U];
-#pragma unroll
-      for (int u = 0; u < U; ++u) {
-        uint32_t pp = p + 64 * u;
-        if (pp < noct) {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) {
-            const v2d tr = xr2[4 * pp + e], ti = xi2[4 * pp + e];
-            xa[u][2 * e] = (float)tr.x; xa[u][2 * e + 1] = (float)tr.y;
-            xb[u][2 * e] = (float)ti.x; xb[u][2 * e + 1] = (float)ti.y;
-          }
-        }
+    // software pipeline: the next U octets' loads are in flight while the
+    // current ones are multiplied (one wave per SIMD at 8192 rows, so no
+    // other wave hides the load latency)
+    uint32_t p = lane;
+    constexpr uint32_t kStep = 64 * U;
+    if (p + kStep - 64 < noct) {
+      HOct<R, U, CPLX> cur;
+      hgemv_load(cur, p, ar8, ai8);
+      for (; p + 2 * kStep - 64 < noct; p += kStep) {
+        HOct<R, U, CPLX> nxt;
+        hgemv_load(nxt, p + kStep, ar8, ai8);
+        hgemv_fma<R, U, CPLX, XL>(cur, p, xr2, xi2, xs, kXLdsCols, sr, si);
+        cur = nxt;
       }
-#pragma unroll
-      for (int q = 0; q < R; ++q)
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          uint32_t pp = p + 64 * u;
-          if (pp < noct) {
-            a[q][u] = __builtin_nontemporal_load(ar8[q] + pp);
-            if (CPLX) b[q][u] = __builtin_nontemporal_load(ai8[q] + pp);
-          }
-        }
-#pragma unroll
-      for (int q = 0; q < R; ++q)
-#pragma unroll
-        for (int u = 0; u < U; ++u) {
-          if (p + 64 * u < noct) {
-#pragma unroll
-            for (int e = 0; e < 8; ++e) {
-              const float av = (float)a[q][u][e];
-              sr[q] = __builtin_fmaf(av, xa[u][e], sr[q]);
-              si[q] = __builtin_fmaf(av, xb[u][e], si[q]);
-              if (CPLX) {
-                const float bv = (float)b[q][u][e];
-                sr[q] = __builtin_fmaf(-bv, xb[u][e], sr[q]);
-                si[q] = __builtin_fmaf(bv, xa[u][e], si[q]);
-              }
-            }
-          }
-        }
+      hgemv_fma<R, U, CPLX, XL>(cur, p, xr2, xi2, xs, kXLdsCols, sr, si);
+      p += kStep;
+    }
+    for (; p < noct; p += 64) {
+      HOct<R, 1, CPLX> one;
+      hgemv_load(one, p, ar8, ai8);
+      hgemv_fma<R, 1, CPLX, XL>(one, p, xr2, xi2, xs, kXLdsCols, sr, si);
     }
     double dr[R], di[R];
 #pragma unroll
@@ -568,13 +627,14 @@ constexpr Shape kCShapes[] = {{4, 3, 2}, {4, 3, 1}, {4, 2, 2}, {4, 4, 2}, {2, 2,
                               {2, 3, 4}, {8, 2, 1}, {4, 2, 1}};
 // FP16 operand: 16 B per load as for FP32, but twice the elements; all keep
 // R G = 8 (the fused projection epilogue's 256 workgroups at 8192 rows).
-// Entry 0 measured fastest at n = 8192 (57.2 us; (8, 1, 1) 70.3 us).
-constexpr Shape kHShapes[] = {{4, 2, 2}, {8, 1, 1}, {8, 2, 1}, {2, 4, 4}, {4, 1, 2},
+// Entry 0 measured fastest at n = 8192 with x in LDS: 53.0 us (0.65 of HBM;
+// (4, 2, 2) 63 us, (8, 2, 1) 70 us, (4, 3, 2) 58 us).
+constexpr Shape kHShapes[] = {{2, 4, 4}, {8, 1, 1}, {8, 2, 1}, {4, 2, 2}, {4, 1, 2},
                               {4, 3, 2}, {2, 2, 4}, {8, 3, 1}};
 constexpr int kNumZ = sizeof(kZShapes) / sizeof(kZShapes[0]);
 constexpr int kNumC = sizeof(kCShapes) / sizeof(kCShapes[0]);
 constexpr int kNumH = sizeof(kHShapes) / sizeof(kHShapes[0]);
-int g_zshape = 0, g_cshape = 0, g_hshape = 0;
+int g_zshape = 0, g_cshape = 0, g_hshape = 0, g_hxlds = 1;
 // dotProducts on the matrix cores: 1 large ones (gemm_mfma_eligible), 2 all,
 // 0 none (the vector-FMA tiles)
 int g_gemm_mfma = 1;
@@ -605,16 +665,17 @@ template <bool CPLX>
 constexpr CFn kC1[] = {k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
                        k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
                        k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
-template <bool CPLX>
-constexpr HFn kH[] = {k_hgemv16<4, 2, CPLX, 2>, k_hgemv16<8, 1, CPLX, 1>, k_hgemv16<8, 2, CPLX, 1>,
-                      k_hgemv16<2, 4, CPLX, 4>, k_hgemv16<4, 1, CPLX, 2>, k_hgemv16<4, 3, CPLX, 2>,
-                      k_hgemv16<2, 2, CPLX, 4>, k_hgemv16<8, 3, CPLX, 1>};
+template <bool CPLX, bool XL>
+constexpr HFn kH[] = {k_hgemv16<2, 4, CPLX, 4, XL>, k_hgemv16<8, 1, CPLX, 1, XL>,
+                      k_hgemv16<8, 2, CPLX, 1, XL>, k_hgemv16<4, 2, CPLX, 2, XL>,
+                      k_hgemv16<4, 1, CPLX, 2, XL>, k_hgemv16<4, 3, CPLX, 2, XL>,
+                      k_hgemv16<2, 2, CPLX, 4, XL>, k_hgemv16<8, 3, CPLX, 1, XL>};
 // G = 1 stand-ins (the shape itself when G = 1, else four rows per wave)
 template <bool CPLX>
-constexpr HFn kH1[] = {k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<8, 1, CPLX, 1>, k_hgemv16<8, 2, CPLX, 1>,
-                       k_hgemv16<4, 4, CPLX, 1>, k_hgemv16<4, 1, CPLX, 1>, k_hgemv16<4, 3, CPLX, 1>,
+constexpr HFn kH1[] = {k_hgemv16<4, 4, CPLX, 1>, k_hgemv16<8, 1, CPLX, 1>, k_hgemv16<8, 2, CPLX, 1>,
+                       k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<4, 1, CPLX, 1>, k_hgemv16<4, 3, CPLX, 1>,
                        k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<8, 3, CPLX, 1>};
-static_assert(sizeof(kH<true>) / sizeof(HFn) == kNumH, "one kernel per FP16 shape");
+static_assert(sizeof(kH<true, false>) / sizeof(HFn) == kNumH, "one kernel per FP16 shape");
 static_assert(sizeof(kH1<true>) / sizeof(HFn) == kNumH, "one kernel per FP16 shape");
 static_assert(sizeof(kZ1<0>) / sizeof(ZFn) == kNumZ, "one kernel per FP64 shape");
 static_assert(sizeof(kC1<true>) / sizeof(CFn) == kNumC, "one kernel per FP32 shape");
@@ -737,7 +798,11 @@ oaph_status launch_hgemv16(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   const int RG = grp ? sh.R * sh.G : (small ? 2 : (sh.G == 1 ? sh.R : 4));
   const unsigned waves = (rows + RG - 1) / RG;
   const dim3 grid((waves + 3) / 4);
-  const HFn fn = grp ? (aim ? kH<true>[g_hshape] : kH<false>[g_hshape])
+  // row groups (the large operators) with rows of at most kXLdsCols take x
+  // from LDS (g_hxlds = 0 turns that off)
+  const bool xl = g_hxlds && cols <= kXLdsCols;
+  const HFn fn = grp ? (aim ? (xl ? kH<true, true>[g_hshape] : kH<true, false>[g_hshape])
+                            : (xl ? kH<false, true>[g_hshape] : kH<false, false>[g_hshape]))
                  : small ? (aim ? kHs<true> : kHs<false>)
                      : (aim ? kH1<true>[g_hshape] : kH1<false>[g_hshape]);
   const MvProj kp = fused_projection(pj, true, grid.x);
@@ -811,7 +876,10 @@ oaph_status oaph_set_gemm_path(int mfma) {
 
 oaph_status oaph_set_matvec_shape(int fp32, int variant) {
   using namespace oaph;
-  if (fp32 == 2) {
+  if (fp32 == 3) {  // FP16 operand: x staged in LDS (1) or read from global memory (0)
+    OAPH_CHECK(variant == 0 || variant == 1, OAPH_ERR_ARG);
+    g_hxlds = variant;
+  } else if (fp32 == 2) {
     OAPH_CHECK(variant >= 0 && variant < kNumH, OAPH_ERR_ARG);
     g_hshape = variant;
   } else if (fp32) {
