@@ -15,10 +15,12 @@ set miss (reported).  The operator is resident in HBM before the timed
 region; step i starts from seed 1000 + i (warm-up steps use other seeds), so
 the steps sample the solver's start-vector dependence.
 
-Reported `value`: eigenpairs/s of the mixed-precision solver (FP32 matvec
-bulk, FP64 orthogonalisation, QR, shifts, polish and residual check) over the
-K timed steps; the plain-FP64 solver on the same start vectors is reported
-beside it (`plain_fp64`).  `batched_c3` is the 512-problem covariance batch
+Reported `value`: eigenpairs/s of the mixed-precision solver (FP16-operand
+cycles to 2e-3, then FP32-operand cycles; FP64 orthogonalisation, QR,
+shifts, Rayleigh quotients and the residual check that accepts the pairs,
+FP64 cycles only if it fails) over the K timed steps; the plain-FP64 solver
+on the same start vectors is reported beside it (`plain_fp64`), and the
+mixed flow's restart schedule run in FP64 throughout (`fp64_mixed_schedule`).  `batched_c3` is the 512-problem covariance batch
 (n_i in [64, 1024], s = 256, k = 4, m = 16) solved in one persistent launch.
 `learned_c4` compares the random start with the trained network's start
 vector on held-out convection-diffusion-reaction operators (n = 4096, 6
@@ -32,7 +34,7 @@ matvec), timed with HIP events on the solver's stream; `cpu_baseline` runs the
 oracle's FP64 IRAM (threaded dense matvec) for one full solve of the same
 operator on the host cores.
 
-Usage: python bench.py [--gpus 1] [--steps 6] [--warmup 1]
+Usage: python bench.py [--gpus 1] [--steps 12] [--warmup 1]
        python bench.py --traffic-probe   (a few matvec launches and a copy
        kernel of known size, for rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)
 """
@@ -124,7 +126,8 @@ def solve_series(s, re, im, steps, warmup, fro, want):
         stats.append(st)
         per_run.append(ok / dt)
         log(f"  solve {i}: {ok}/{K} pairs verified{' (SET MISS)' if miss else ''}, "
-            f"{st['matvecs']} matvecs ({st['matvecs_low']} FP32), {st['restarts']} restarts, "
+            f"{st['matvecs']} matvecs ({st['matvecs_f16']} FP16, "
+            f"{st['matvecs_low'] - st['matvecs_f16']} FP32), {st['restarts']} restarts, "
             f"{dt*1e3:.1f} ms, max resid {st['max_resid']:.3e}, max |lambda - fixture| {err:.2e}")
     wall = time.perf_counter() - t0
     return good, wall, stats, float(np.median(per_run)), misses, errmax
@@ -419,7 +422,7 @@ def traffic_probe() -> int:
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6)
+    ap.add_argument("--steps", type=int, default=12)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
@@ -456,11 +459,23 @@ def main() -> int:
 
     # mixed precision
     smx, re, im = make_solver(dA, Precision.MIXED_FP32)
-    log("mixed FP32/FP64 solver:")
+    log("mixed FP16/FP32/FP64 solver:")
     goodmx, wallmx, stmx, medmx, missmx, errmx = solve_series(smx, re, im, args.steps,
                                                               args.warmup, fro, want)
     check(lib.oaph_arnoldi_time_matvec(smx._h, 1, 20, C.byref(ms)), "time_matvec")
     t32 = ms.value
+    check(lib.oaph_arnoldi_time_matvec(smx._h, 2, 20, C.byref(ms)), "time_matvec")
+    t16 = ms.value
+
+    # the mixed flow's restart schedule (Ritz-vector-sum restarts at 2e-3 and
+    # 1e-6 of the largest Ritz value) with every cycle in FP64: what the
+    # schedule gains apart from precision
+    ssc, resc, imsc = make_solver(dA, Precision.FP64)
+    ssc.setRestartStages([2e-3, 1e-6])
+    log("FP64 solver, the mixed flow's restart schedule:")
+    goodsc, wallsc, stsc, medsc, misssc, errsc = solve_series(ssc, resc, imsc, args.steps,
+                                                              args.warmup, fro, want)
+    del ssc
 
     # config C2's BF16-split variant: the first Gram-Schmidt pass on the
     # matrix cores from a BF16 hi + lo copy of the basis (FP32 accumulation),
@@ -477,6 +492,8 @@ def main() -> int:
     # epilogue reads (the first Gram-Schmidt pass of the Arnoldi step)
     bytes64 = 16 * N * N + 32 * N + 16 * N * M
     bytes32 = 8 * N * N + 32 * N + 16 * N * M
+    bytes16 = 4 * N * N + 32 * N + 16 * N * M
+    gbs16 = bytes16 / (t16 * 1e-3) / 1e9
     gbs64 = bytes64 / (t64 * 1e-3) / 1e9
     gbs32 = bytes32 / (t32 * 1e-3) / 1e9
     eps64 = good64 / wall64
@@ -512,7 +529,7 @@ def main() -> int:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 (FP32 operator copy for the matvec bulk)",
+        "dtype": "f64 (FP16 then FP32 operator copies for the matvec bulk)",
         "data": "synthetic (seeded, generated in HBM)",
         "config": {"workload": "dense complex 8192x8192 IRAM, k=8 largest |lambda|, m=48, "
                                "tol 1e-10 ||A||_F (C1 matrix, mixed precision)",
@@ -522,17 +539,31 @@ def main() -> int:
                    "largest of the dense oracle (tests/golden/c1_top8.json) within 1e-10 |lambda_1|",
         "set_misses": missmx,
         "max_eigenvalue_err_vs_oracle": errmx,
-        "matvecs_median": {"fp32": int(np.median([s["matvecs_low"] for s in stmx])),
+        "matvecs_median": {"f16": int(np.median([s["matvecs_f16"] for s in stmx])),
+                           "fp32": int(np.median([s["matvecs_low"] - s["matvecs_f16"] for s in stmx])),
                            "fp64": int(np.median([s["matvecs"] - s["matvecs_low"] for s in stmx]))},
+        "restarts_median": int(np.median([s["restarts"] for s in stmx])),
+        "fp64_polish_solves": int(sum(s["fp64_polish"] for s in stmx)),
+        "max_check_resid_over_tol": round(max(s["check_resid"] for s in stmx) / (TOL * fro), 4),
         "plain_fp64": {"value": round(eps64, 4), "ms_per_step": round(wall64 / args.steps * 1e3, 3),
                        "median_per_run": round(med64, 4), "matvecs_median": matvecs64,
                        "set_misses": miss64, "max_eigenvalue_err_vs_oracle": err64,
+                       "restarts_median": int(np.median([s["restarts"] for s in st64])),
                        "ratio_mixed_over_fp64": round(epsmx / eps64, 3) if eps64 > 0 else None},
+        "fp64_mixed_schedule": {
+            "value": round(goodsc / wallsc, 4), "ms_per_step": round(wallsc / args.steps * 1e3, 3),
+            "median_per_run": round(medsc, 4), "set_misses": misssc,
+            "max_eigenvalue_err_vs_oracle": errsc,
+            "matvecs_median": int(np.median([x["matvecs"] for x in stsc])),
+            "ratio_mixed_over_this": round(epsmx / (goodsc / wallsc), 3) if goodsc > 0 else None,
+            "note": "every cycle FP64, with the mixed flow's Ritz-vector-sum restarts at 2e-3 and "
+                    "1e-6: the mixed flow's gain from precision alone is ratio_mixed_over_this"},
         "mixed_bf16x3_projection": {
             "value": round(goodbf / wallbf, 4), "ms_per_step": round(wallbf / args.steps * 1e3, 3),
             "median_per_run": round(medbf, 4), "set_misses": missbf,
             "max_eigenvalue_err_vs_oracle": errbf,
-            "matvecs_median": {"fp32": int(np.median([x["matvecs_low"] for x in stbf])),
+            "matvecs_median": {"f16": int(np.median([x["matvecs_f16"] for x in stbf])),
+                               "fp32": int(np.median([x["matvecs_low"] - x["matvecs_f16"] for x in stbf])),
                                "fp64": int(np.median([x["matvecs"] - x["matvecs_low"] for x in stbf]))},
             "note": "first Gram-Schmidt pass V^H w on v_mfma_f32_16x16x16_bf16 from a BF16 hi+lo "
                     "basis (hi.hi + hi.lo + lo.hi, FP32 accumulation), second pass FP64"},
@@ -544,6 +575,10 @@ def main() -> int:
                      "traffic": profiled_traffic("k_cgemv32"),
                      "kernel": "k_cgemv32 (FP32 operand, FP64 accumulate, V^H y partial sums)",
                      "kernel_ms": round(t32, 5), "bytes_per_launch": bytes32},
+        "roofline_f16_matvec": {"achieved": round(gbs16, 1), "frac": round(gbs16 / HBM_PEAK_GBS, 4),
+                                "kernel": "k_hgemv16 (FP16 operand via v_fma_mix_f32, FP32 sums, "
+                                          "V^H y partial sums)",
+                                "kernel_ms": round(t16, 5), "bytes_per_launch": bytes16},
         "roofline_fp64_matvec": {"achieved": round(gbs64, 1), "frac": round(gbs64 / HBM_PEAK_GBS, 4),
                                  "kernel": "k_zgemv (complex FP64)", "kernel_ms": round(t64, 5),
                                  "bytes_per_launch": bytes64},

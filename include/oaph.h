@@ -558,7 +558,9 @@ oaph_status oaph_synth_matrix(oaph_matrix* A, int kind, uint64_t seed, double p1
 oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, double alpha);
 /* Launch shape of the streaming matvec (fp32 = 1: the FP32-operand form used
  * by OAPH_PREC_MIXED_FP32, 2: its FP16-operand form, 0: FP64; fp32 = 3 sets
- * whether the FP16-operand form stages x in LDS, variant 1 (default) or 0): variant index
+ * whether the FP16-operand form stages x in LDS, variant 1 (default) or 0,
+ * fp32 = 4 whether the waves of every streaming matvec start at rotated
+ * column chunks, 1 (default) or 0): variant index
  * into a table of (rows per wave read
  * together, loads in flight per row and plane, groups of such rows per wave
  * taken one after another); process-wide.  No reference counterpart (tuning

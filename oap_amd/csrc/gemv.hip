@@ -89,7 +89,8 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
                                                const double* __restrict__ xre,
                                                const double* __restrict__ xim,
                                                double* __restrict__ yre,
-                                               double* __restrict__ yim, MvProj pj) {
+                                               double* __restrict__ yim, uint32_t rotate,
+                                               MvProj pj) {
   __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -112,9 +113,14 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
     ar2[q] = reinterpret_cast<const v2d*>(are + (size_t)r * lda);
     ai2[q] = CPLX ? reinterpret_cast<const v2d*>(aim + (size_t)r * lda) : nullptr;
   }
-  uint32_t p = lane;
-  // main loop: U steps of 64 pairs per iteration, all loads issued first
-  for (; p + 64 * (U - 1) < npair; p += 64 * U) {
+  // main loop: nch full chunks of U steps of 64 pairs, all loads of a chunk
+  // issued first; with `rotate` each wave starts at chunk (wave mod nch) and
+  // wraps, so that waves in step read different column ranges (the same
+  // offsets in every row at once concentrate the traffic on few channels)
+  const uint32_t nch = npair / (64 * U);
+  uint32_t c = rotate && nch ? wave % nch : 0;
+  for (uint32_t it = 0; it < nch; ++it, c = c + 1 == nch ? 0 : c + 1) {
+    const uint32_t p = c * 64 * U + lane;
     v2d xa[U], xb[U];
     v2d a[R][U], b[R][U];
 #pragma unroll
@@ -147,7 +153,7 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
         }
       }
   }
-  for (; p < npair; p += 64) {
+  for (uint32_t p = nch * 64 * U + lane; p < npair; p += 64) {
     v2d xa = xr2[p], xb = XIM ? xi2[p] : v2d{0, 0};
 #pragma unroll
     for (int q = 0; q < R; ++q) {
@@ -251,7 +257,7 @@ __device__ __forceinline__ void lowp_gemv(uint32_t rows, uint32_t cols, uint32_t
                                           const double* __restrict__ xre,
                                           const double* __restrict__ xim,
                                           double* __restrict__ yre, double* __restrict__ yim,
-                                          const MvProj& pj, double2* ys) {
+                                          uint32_t rotate, const MvProj& pj, double2* ys) {
   using vec = typename OP::vec;
   const int lane = threadIdx.x & 63;
   const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6);
@@ -274,6 +280,11 @@ __device__ __forceinline__ void lowp_gemv(uint32_t rows, uint32_t cols, uint32_t
     ar4[q] = reinterpret_cast<const vec*>(are + (size_t)r * lda);
     ai4[q] = CPLX ? reinterpret_cast<const vec*>(aim + (size_t)r * lda) : nullptr;
   }
+  // with `rotate` the wave walks its rows from quad `off` (a multiple of 64
+  // that differs between neighbouring waves) and wraps at the end, so that
+  // waves in step read different column ranges
+  const uint32_t nq64 = nquad / 64;
+  const uint32_t off = rotate && nq64 ? (wave * U % nq64) * 64 : 0;
   for (uint32_t p = lane; p < nquad; p += 64 * U) {
     v2d xa[U][2], xb[U][2];
     vec a[R][U], b[R][U];
@@ -281,8 +292,9 @@ __device__ __forceinline__ void lowp_gemv(uint32_t rows, uint32_t cols, uint32_t
     for (int u = 0; u < U; ++u) {
       uint32_t pp = p + 64 * u;
       if (pp < nquad) {
-        xa[u][0] = xr2[2 * pp]; xa[u][1] = xr2[2 * pp + 1];
-        xb[u][0] = xi2[2 * pp]; xb[u][1] = xi2[2 * pp + 1];
+        const uint32_t wq = pp + off < nquad ? pp + off : pp + off - nquad;
+        xa[u][0] = xr2[2 * wq]; xa[u][1] = xr2[2 * wq + 1];
+        xb[u][0] = xi2[2 * wq]; xb[u][1] = xi2[2 * wq + 1];
       }
     }
 #pragma unroll
@@ -291,8 +303,9 @@ __device__ __forceinline__ void lowp_gemv(uint32_t rows, uint32_t cols, uint32_t
       for (int u = 0; u < U; ++u) {
         uint32_t pp = p + 64 * u;
         if (pp < nquad) {
-          a[q][u] = __builtin_nontemporal_load(ar4[q] + pp);
-          if (CPLX) b[q][u] = __builtin_nontemporal_load(ai4[q] + pp);
+          const uint32_t wq = pp + off < nquad ? pp + off : pp + off - nquad;
+          a[q][u] = __builtin_nontemporal_load(ar4[q] + wq);
+          if (CPLX) b[q][u] = __builtin_nontemporal_load(ai4[q] + wq);
         }
       }
 #pragma unroll
@@ -350,9 +363,10 @@ __global__ __launch_bounds__(256) void k_cgemv32(uint32_t rows, uint32_t cols, u
                                                  const double* __restrict__ xre,
                                                  const double* __restrict__ xim,
                                                  double* __restrict__ yre,
-                                                 double* __restrict__ yim, MvProj pj) {
+                                                 double* __restrict__ yim, uint32_t rotate,
+                                                 MvProj pj) {
   __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
-  lowp_gemv<OpF32, R, U, CPLX, G>(rows, cols, lda, are, aim, xre, xim, yre, yim, pj, ys);
+  lowp_gemv<OpF32, R, U, CPLX, G>(rows, cols, lda, are, aim, xre, xim, yre, yim, rotate, pj, ys);
 }
 
 // ---------------------------------------------------------------- FP16-operand GEMV
@@ -444,7 +458,7 @@ __global__ __launch_bounds__(256) void k_hgemv16(uint32_t rows, uint32_t cols, u
                                                  const double* __restrict__ xim,
                                                  double* __restrict__ yre,
                                                  double* __restrict__ yim, double unscale,
-                                                 MvProj pj) {
+                                                 uint32_t rotate, MvProj pj) {
   __shared__ double2 ys[4 * G * R];  // the block's rows of y (projection epilogue)
   __shared__ float xs[XL ? 2 * kXLdsCols : 1];
   const int lane = threadIdx.x & 63;
@@ -477,22 +491,26 @@ __global__ __launch_bounds__(256) void k_hgemv16(uint32_t rows, uint32_t cols, u
     }
     // software pipeline: the next U octets' loads are in flight while the
     // current ones are multiplied (one wave per SIMD at 8192 rows, so no
-    // other wave hides the load latency)
-    uint32_t p = lane;
+    // other wave hides the load latency).  The row is walked in nch chunks
+    // of kStep octets; with `rotate` each wave starts at chunk (wave mod nch)
+    // and wraps, so that waves in step read different column ranges
     constexpr uint32_t kStep = 64 * U;
-    if (p + kStep - 64 < noct) {
+    const uint32_t nch = noct / kStep;
+    if (nch) {
+      uint32_t c = rotate ? wave % nch : 0;
       HOct<R, U, CPLX> cur;
-      hgemv_load(cur, p, ar8, ai8);
-      for (; p + 2 * kStep - 64 < noct; p += kStep) {
+      hgemv_load(cur, c * kStep + lane, ar8, ai8);
+      for (uint32_t i = 1; i < nch; ++i) {
+        const uint32_t cn = c + 1 == nch ? 0 : c + 1;
         HOct<R, U, CPLX> nxt;
-        hgemv_load(nxt, p + kStep, ar8, ai8);
-        hgemv_fma<R, U, CPLX, XL>(cur, p, xr2, xi2, xs, kXLdsCols, sr, si);
+        hgemv_load(nxt, cn * kStep + lane, ar8, ai8);
+        hgemv_fma<R, U, CPLX, XL>(cur, c * kStep + lane, xr2, xi2, xs, kXLdsCols, sr, si);
         cur = nxt;
+        c = cn;
       }
-      hgemv_fma<R, U, CPLX, XL>(cur, p, xr2, xi2, xs, kXLdsCols, sr, si);
-      p += kStep;
+      hgemv_fma<R, U, CPLX, XL>(cur, c * kStep + lane, xr2, xi2, xs, kXLdsCols, sr, si);
     }
-    for (; p < noct; p += 64) {
+    for (uint32_t p = nch * kStep + lane; p < noct; p += 64) {
       HOct<R, 1, CPLX> one;
       hgemv_load(one, p, ar8, ai8);
       hgemv_fma<R, 1, CPLX, XL>(one, p, xr2, xi2, xs, kXLdsCols, sr, si);
@@ -615,54 +633,58 @@ namespace {
 // more loads in flight hide HBM latency; groups give fewer, longer-lived
 // waves (one per SIMD at n = 8192 with R G = 8).  Selected per process with
 // oaph_set_matvec_shape.  Entry 0 is the default, the fastest measured on
-// MI355X at n = 8192 (averages of 20 launches): FP64 (2, 3, 4) 160 us =
-// 6.71 TB/s vs (4, 2, 1) 162.5 us; FP32 operand (4, 3, 2) 80.5 us =
-// 6.68 TB/s vs (4, 3, 1) 84 us.
+// MI355X at n = 8192 with rotated column chunks (averages of 50 launches,
+// operator, x, y and the 48 basis vectors of the projection epilogue):
+// FP64 (2, 4, 2) 155.3 us = 6.95 TB/s vs (2, 3, 4) 156.9 us (161.7 us
+// without rotation); FP32 operand (4, 2, 2) 79.4 us = 6.85 TB/s vs
+// (4, 3, 2) 82.3 us (83.1 us without rotation).
 struct Shape {
   int R, U, G;
 };
-constexpr Shape kZShapes[] = {{2, 3, 4}, {4, 2, 1}, {2, 4, 4}, {4, 3, 2}, {2, 6, 4},
-                              {2, 4, 2}, {4, 6, 2}, {8, 2, 1}};
-constexpr Shape kCShapes[] = {{4, 3, 2}, {4, 3, 1}, {4, 2, 2}, {4, 4, 2}, {2, 2, 4},
+constexpr Shape kZShapes[] = {{2, 4, 2}, {4, 2, 1}, {2, 4, 4}, {4, 3, 2}, {2, 6, 4},
+                              {2, 3, 4}, {4, 6, 2}, {8, 2, 1}};
+constexpr Shape kCShapes[] = {{4, 2, 2}, {4, 3, 1}, {4, 3, 2}, {4, 4, 2}, {2, 2, 4},
                               {2, 3, 4}, {8, 2, 1}, {4, 2, 1}};
 // FP16 operand: 16 B per load as for FP32, but twice the elements; all keep
 // R G = 8 (the fused projection epilogue's 256 workgroups at 8192 rows).
-// Entry 0 measured fastest at n = 8192 with x in LDS: 53.0 us (0.65 of HBM;
-// (4, 2, 2) 63 us, (8, 2, 1) 70 us, (4, 3, 2) 58 us).
+// Entry 0 measured fastest at n = 8192 with x in LDS and rotated column
+// chunks: 45.4 us (0.76 of HBM; 53.5 us without rotation; (4, 3, 2) 46.0
+// us, (8, 2, 1) 51.6 us).
 constexpr Shape kHShapes[] = {{2, 4, 4}, {8, 1, 1}, {8, 2, 1}, {4, 2, 2}, {4, 1, 2},
                               {4, 3, 2}, {2, 2, 4}, {8, 3, 1}};
 constexpr int kNumZ = sizeof(kZShapes) / sizeof(kZShapes[0]);
 constexpr int kNumC = sizeof(kCShapes) / sizeof(kCShapes[0]);
 constexpr int kNumH = sizeof(kHShapes) / sizeof(kHShapes[0]);
-int g_zshape = 0, g_cshape = 0, g_hshape = 0, g_hxlds = 1;
+// g_rot: waves of the streaming matvecs start at rotated column chunks
+int g_zshape = 0, g_cshape = 0, g_hshape = 0, g_hxlds = 1, g_rot = 1;
 // dotProducts on the matrix cores: 1 large ones (gemm_mfma_eligible), 2 all,
 // 0 none (the vector-FMA tiles)
 int g_gemm_mfma = 1;
 
 using ZFn = void (*)(uint32_t, uint32_t, uint32_t, const double*, const double*, const double*,
-                     const double*, double*, double*, MvProj);
+                     const double*, double*, double*, uint32_t, MvProj);
 using CFn = void (*)(uint32_t, uint32_t, uint32_t, const float*, const float*, const double*,
-                     const double*, double*, double*, MvProj);
+                     const double*, double*, double*, uint32_t, MvProj);
 using HFn = void (*)(uint32_t, uint32_t, uint32_t, const _Float16*, const _Float16*,
-                     const double*, const double*, double*, double*, double, MvProj);
+                     const double*, const double*, double*, double*, double, uint32_t, MvProj);
 
 // kZ / kC: each shape as listed; kZ1 / kC1: what runs when the grouped grid
 // would leave SIMDs without a wave -- the shape itself when G = 1, else the
 // fastest G = 1 shape (FP64 (4, 2), FP32 operand (4, 3)).
 template <int MODE>
-constexpr ZFn kZ[] = {k_zgemv<2, 3, MODE, 4>, k_zgemv<4, 2, MODE, 1>, k_zgemv<2, 4, MODE, 4>,
-                      k_zgemv<4, 3, MODE, 2>, k_zgemv<2, 6, MODE, 4>, k_zgemv<2, 4, MODE, 2>,
+constexpr ZFn kZ[] = {k_zgemv<2, 4, MODE, 2>, k_zgemv<4, 2, MODE, 1>, k_zgemv<2, 4, MODE, 4>,
+                      k_zgemv<4, 3, MODE, 2>, k_zgemv<2, 6, MODE, 4>, k_zgemv<2, 3, MODE, 4>,
                       k_zgemv<4, 6, MODE, 2>, k_zgemv<8, 2, MODE, 1>};
 template <int MODE>
 constexpr ZFn kZ1[] = {k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>,
                        k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>,
                        k_zgemv<4, 2, MODE, 1>, k_zgemv<8, 2, MODE, 1>};
 template <bool CPLX>
-constexpr CFn kC[] = {k_cgemv32<4, 3, CPLX, 2>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 2, CPLX, 2>,
+constexpr CFn kC[] = {k_cgemv32<4, 2, CPLX, 2>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 2>,
                       k_cgemv32<4, 4, CPLX, 2>, k_cgemv32<2, 2, CPLX, 4>, k_cgemv32<2, 3, CPLX, 4>,
                       k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
 template <bool CPLX>
-constexpr CFn kC1[] = {k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
+constexpr CFn kC1[] = {k_cgemv32<4, 2, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
                        k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
                        k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
 template <bool CPLX, bool XL>
@@ -744,7 +766,7 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
                        : (mode == 0 ? kZ1<0>[g_zshape] : mode == 1 ? kZ1<1>[g_zshape] : kZ1<2>[g_zshape]);
     const MvProj kp = fused_projection(pj, mode != 1, grid.x);
     hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim,
-                       kp);
+                       (uint32_t)g_rot, kp);
   } else {
     if (pj) pj->blocks = 0;
     dim3 grid((rows + 3) / 4);
@@ -777,7 +799,7 @@ oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
                      : (aim ? kC1<true>[g_cshape] : kC1<false>[g_cshape]);
   const MvProj kp = fused_projection(pj, true, grid.x);
   hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda,
-                     are, aim, xre, xim, yre, yim, kp);
+                     are, aim, xre, xim, yre, yim, (uint32_t)g_rot, kp);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
@@ -807,7 +829,7 @@ oaph_status launch_hgemv16(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
                      : (aim ? kH1<true>[g_hshape] : kH1<false>[g_hshape]);
   const MvProj kp = fused_projection(pj, true, grid.x);
   hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim,
-                     unscale, kp);
+                     unscale, (uint32_t)g_rot, kp);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }
@@ -876,7 +898,10 @@ oaph_status oaph_set_gemm_path(int mfma) {
 
 oaph_status oaph_set_matvec_shape(int fp32, int variant) {
   using namespace oaph;
-  if (fp32 == 3) {  // FP16 operand: x staged in LDS (1) or read from global memory (0)
+  if (fp32 == 4) {  // every streaming matvec: waves start at rotated column chunks (1) or at 0
+    OAPH_CHECK(variant == 0 || variant == 1, OAPH_ERR_ARG);
+    g_rot = variant;
+  } else if (fp32 == 3) {  // FP16 operand: x staged in LDS (1) or read from global memory (0)
     OAPH_CHECK(variant == 0 || variant == 1, OAPH_ERR_ARG);
     g_hxlds = variant;
   } else if (fp32 == 2) {
