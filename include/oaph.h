@@ -566,6 +566,12 @@ oaph_status oaph_synth_cdr(oaph_matrix* A, uint32_t grid, double pe, double c, d
  * taken one after another); process-wide.  No reference counterpart (tuning
  * knob). */
 oaph_status oaph_set_matvec_shape(int fp32, int variant);
+/* Launch shape (variant index into a table of (rows per wave, loads in
+ * flight), one group) of the matvec that runs where row groups would leave
+ * SIMDs without a wave (the operators of a few thousand rows): fp32 = 0
+ * FP64, 1 FP32 operand, 2 FP16 operand; process-wide.  No reference
+ * counterpart (tuning knob). */
+oaph_status oaph_set_matvec_fallback_shape(int fp32, int variant);
 /* extension (no reference counterpart): which kernel forms dotProduct.
  * 1 (default): products with W, H >= 256 and K >= 64 on the FP64 matrix
  * cores (v_mfma_f64_16x16x4_f64, 128 x 128 tiles), smaller ones on the

@@ -170,6 +170,7 @@ _SIGS = {
     "oaph_arnoldi_restart_profile": (C.c_int, [C.c_void_p, C.POINTER(C.c_longlong)]),
     "oaph_synth_matrix": (C.c_int, [MP, C.c_int, C.c_uint64, C.c_double, C.c_double]),
     "oaph_set_matvec_shape": (C.c_int, [C.c_int, C.c_int]),
+    "oaph_set_matvec_fallback_shape": (C.c_int, [C.c_int, C.c_int]),
     "oaph_set_gemm_path": (C.c_int, [C.c_int]),
     "oaph_matvec_shape_info": (C.c_int, [C.c_int, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
                                          C.POINTER(C.c_int)]),

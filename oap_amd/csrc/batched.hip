@@ -750,6 +750,9 @@ const Variant kVariants[] = {
     {256, 1, k_batched_solve<256, 1>},
     {128, 1, k_batched_solve<128, 1>},
     {512, 1, k_batched_solve<512, 1>},
+    {256, 2, k_batched_solve<256, 2>},
+    {512, 2, k_batched_solve<512, 2>},
+    {1024, 1, k_batched_solve<1024, 1>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 

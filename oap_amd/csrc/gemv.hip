@@ -657,6 +657,8 @@ constexpr int kNumC = sizeof(kCShapes) / sizeof(kCShapes[0]);
 constexpr int kNumH = sizeof(kHShapes) / sizeof(kHShapes[0]);
 // g_rot: waves of the streaming matvecs start at rotated column chunks
 int g_zshape = 0, g_cshape = 0, g_hshape = 0, g_hxlds = 1, g_rot = 1;
+// fallback (G = 1) shapes per operand precision
+int g_z1 = 0, g_c1 = 0, g_h1 = 0;
 // dotProducts on the matrix cores: 1 large ones (gemm_mfma_eligible), 2 all,
 // 0 none (the vector-FMA tiles)
 int g_gemm_mfma = 1;
@@ -668,39 +670,48 @@ using CFn = void (*)(uint32_t, uint32_t, uint32_t, const float*, const float*, c
 using HFn = void (*)(uint32_t, uint32_t, uint32_t, const _Float16*, const _Float16*,
                      const double*, const double*, double*, double*, double, uint32_t, MvProj);
 
-// kZ / kC: each shape as listed; kZ1 / kC1: what runs when the grouped grid
-// would leave SIMDs without a wave -- the shape itself when G = 1, else the
-// fastest G = 1 shape (FP64 (4, 2), FP32 operand (4, 3)).
+// kZ / kC / kH: each shape as listed; kZ1 / kC1 / kH1 (below): what runs
+// when the grouped grid would leave SIMDs without a wave.
 template <int MODE>
 constexpr ZFn kZ[] = {k_zgemv<2, 4, MODE, 2>, k_zgemv<4, 2, MODE, 1>, k_zgemv<2, 4, MODE, 4>,
                       k_zgemv<4, 3, MODE, 2>, k_zgemv<2, 6, MODE, 4>, k_zgemv<2, 3, MODE, 4>,
                       k_zgemv<4, 6, MODE, 2>, k_zgemv<8, 2, MODE, 1>};
+// The G = 1 shapes that run where row groups would leave SIMDs without a
+// wave (rows < 4 R G simd_count, e.g. 4096 rows), selected independently of
+// the grouped shape with oaph_set_matvec_fallback_shape; entry 0 the default
+// (at 4096 rows, 50 launches: FP32-operand real 14.2 us (0.59 of HBM for the
+// operator) vs 14.5-20.8 us for the others, complex 23.7 us vs 23.5-34.1 us;
+// FP16-operand complex 17.6 us vs 17.5-26.1 us).
+struct Shape1 {
+  int R, U;
+};
+constexpr Shape1 kS1Shapes[] = {{4, 2}, {4, 3}, {4, 4}, {4, 6}, {8, 2}, {8, 3}, {4, 8}, {8, 4}};
+constexpr int kNumS1 = sizeof(kS1Shapes) / sizeof(kS1Shapes[0]);
 template <int MODE>
-constexpr ZFn kZ1[] = {k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>,
-                       k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 2, MODE, 1>,
-                       k_zgemv<4, 2, MODE, 1>, k_zgemv<8, 2, MODE, 1>};
+constexpr ZFn kZ1[] = {k_zgemv<4, 2, MODE, 1>, k_zgemv<4, 3, MODE, 1>, k_zgemv<4, 4, MODE, 1>,
+                       k_zgemv<4, 6, MODE, 1>, k_zgemv<8, 2, MODE, 1>, k_zgemv<8, 3, MODE, 1>,
+                       k_zgemv<4, 8, MODE, 1>, k_zgemv<8, 4, MODE, 1>};
 template <bool CPLX>
 constexpr CFn kC[] = {k_cgemv32<4, 2, CPLX, 2>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 2>,
                       k_cgemv32<4, 4, CPLX, 2>, k_cgemv32<2, 2, CPLX, 4>, k_cgemv32<2, 3, CPLX, 4>,
                       k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
 template <bool CPLX>
-constexpr CFn kC1[] = {k_cgemv32<4, 2, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
-                       k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>,
-                       k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<4, 2, CPLX, 1>};
+constexpr CFn kC1[] = {k_cgemv32<4, 2, CPLX, 1>, k_cgemv32<4, 3, CPLX, 1>, k_cgemv32<4, 4, CPLX, 1>,
+                       k_cgemv32<4, 6, CPLX, 1>, k_cgemv32<8, 2, CPLX, 1>, k_cgemv32<8, 3, CPLX, 1>,
+                       k_cgemv32<4, 8, CPLX, 1>, k_cgemv32<8, 4, CPLX, 1>};
 template <bool CPLX, bool XL>
 constexpr HFn kH[] = {k_hgemv16<2, 4, CPLX, 4, XL>, k_hgemv16<8, 1, CPLX, 1, XL>,
                       k_hgemv16<8, 2, CPLX, 1, XL>, k_hgemv16<4, 2, CPLX, 2, XL>,
                       k_hgemv16<4, 1, CPLX, 2, XL>, k_hgemv16<4, 3, CPLX, 2, XL>,
                       k_hgemv16<2, 2, CPLX, 4, XL>, k_hgemv16<8, 3, CPLX, 1, XL>};
-// G = 1 stand-ins (the shape itself when G = 1, else four rows per wave)
 template <bool CPLX>
-constexpr HFn kH1[] = {k_hgemv16<4, 4, CPLX, 1>, k_hgemv16<8, 1, CPLX, 1>, k_hgemv16<8, 2, CPLX, 1>,
-                       k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<4, 1, CPLX, 1>, k_hgemv16<4, 3, CPLX, 1>,
-                       k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<8, 3, CPLX, 1>};
+constexpr HFn kH1[] = {k_hgemv16<4, 2, CPLX, 1>, k_hgemv16<4, 3, CPLX, 1>, k_hgemv16<4, 4, CPLX, 1>,
+                       k_hgemv16<4, 6, CPLX, 1>, k_hgemv16<8, 2, CPLX, 1>, k_hgemv16<8, 3, CPLX, 1>,
+                       k_hgemv16<4, 8, CPLX, 1>, k_hgemv16<8, 4, CPLX, 1>};
 static_assert(sizeof(kH<true, false>) / sizeof(HFn) == kNumH, "one kernel per FP16 shape");
-static_assert(sizeof(kH1<true>) / sizeof(HFn) == kNumH, "one kernel per FP16 shape");
-static_assert(sizeof(kZ1<0>) / sizeof(ZFn) == kNumZ, "one kernel per FP64 shape");
-static_assert(sizeof(kC1<true>) / sizeof(CFn) == kNumC, "one kernel per FP32 shape");
+static_assert(sizeof(kH1<true>) / sizeof(HFn) == kNumS1, "one kernel per fallback shape");
+static_assert(sizeof(kZ1<0>) / sizeof(ZFn) == kNumS1, "one kernel per fallback shape");
+static_assert(sizeof(kC1<true>) / sizeof(CFn) == kNumS1, "one kernel per fallback shape");
 // At most kSmallRows rows without row groups: two rows per wave, six loads in
 // flight (n = 2048 complex: FP64 18.8 -> 12.8 us, FP32 operand 11.1 -> 8.4
 // us); at 4096 rows this grid would exceed the projection epilogue's 256
@@ -727,7 +738,7 @@ unsigned simd_count() {
 }
 
 // Row groups pay off only while the grouped grid still gives every SIMD a
-// wave; smaller operators take the kZ1 / kC1 fallback (four rows per wave).
+// wave; smaller operators take the kZ1 / kC1 / kH1 fallback shapes.
 bool use_groups(const Shape& sh, uint32_t rows) {
   return rows / (uint32_t)(sh.R * sh.G) >= simd_count();
 }
@@ -758,12 +769,12 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
     const Shape& sh = kZShapes[g_zshape];
     const bool grp = sh.G == 1 || use_groups(sh, rows);
     const bool small = !grp && rows <= kSmallRows;
-    const int RG = grp ? sh.R * sh.G : (small ? 2 : 4);
+    const int RG = grp ? sh.R * sh.G : (small ? 2 : kS1Shapes[g_z1].R);
     const unsigned waves = (rows + RG - 1) / RG;
     const dim3 grid((waves + 3) / 4);
     const ZFn fn = grp ? (mode == 0 ? kZ<0>[g_zshape] : mode == 1 ? kZ<1>[g_zshape] : kZ<2>[g_zshape])
                  : small ? (mode == 0 ? kZs<0> : mode == 1 ? kZs<1> : kZs<2>)
-                       : (mode == 0 ? kZ1<0>[g_zshape] : mode == 1 ? kZ1<1>[g_zshape] : kZ1<2>[g_zshape]);
+                       : (mode == 0 ? kZ1<0>[g_z1] : mode == 1 ? kZ1<1>[g_z1] : kZ1<2>[g_z1]);
     const MvProj kp = fused_projection(pj, mode != 1, grid.x);
     hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim,
                        (uint32_t)g_rot, kp);
@@ -791,12 +802,12 @@ oaph_status launch_cgemv32(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   const Shape& sh = kCShapes[g_cshape];
   const bool grp = sh.G == 1 || use_groups(sh, rows);
   const bool small = !grp && rows <= kSmallRows;
-  const int RG = grp ? sh.R * sh.G : (small ? 2 : 4);
+  const int RG = grp ? sh.R * sh.G : (small ? 2 : kS1Shapes[g_c1].R);
   const unsigned waves = (rows + RG - 1) / RG;
   const dim3 grid((waves + 3) / 4);
   const CFn fn = grp ? (aim ? kC<true>[g_cshape] : kC<false>[g_cshape])
                  : small ? (aim ? kCs<true> : kCs<false>)
-                     : (aim ? kC1<true>[g_cshape] : kC1<false>[g_cshape]);
+                     : (aim ? kC1<true>[g_c1] : kC1<false>[g_c1]);
   const MvProj kp = fused_projection(pj, true, grid.x);
   hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda,
                      are, aim, xre, xim, yre, yim, (uint32_t)g_rot, kp);
@@ -815,9 +826,7 @@ oaph_status launch_hgemv16(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   const Shape& sh = kHShapes[g_hshape];
   const bool grp = sh.G == 1 || use_groups(sh, rows);
   const bool small = !grp && rows <= kSmallRows;
-  // rows per wave of the kernel that runs: kH1 takes the shape itself when
-  // G = 1 and four rows per wave otherwise; kHs two
-  const int RG = grp ? sh.R * sh.G : (small ? 2 : (sh.G == 1 ? sh.R : 4));
+  const int RG = grp ? sh.R * sh.G : (small ? 2 : kS1Shapes[g_h1].R);
   const unsigned waves = (rows + RG - 1) / RG;
   const dim3 grid((waves + 3) / 4);
   // row groups (the large operators) with rows of at most kXLdsCols take x
@@ -826,7 +835,7 @@ oaph_status launch_hgemv16(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t
   const HFn fn = grp ? (aim ? (xl ? kH<true, true>[g_hshape] : kH<true, false>[g_hshape])
                             : (xl ? kH<false, true>[g_hshape] : kH<false, false>[g_hshape]))
                  : small ? (aim ? kHs<true> : kHs<false>)
-                     : (aim ? kH1<true>[g_hshape] : kH1<false>[g_hshape]);
+                     : (aim ? kH1<true>[g_h1] : kH1<false>[g_h1]);
   const MvProj kp = fused_projection(pj, true, grid.x);
   hipLaunchKernelGGL(fn, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim,
                      unscale, (uint32_t)g_rot, kp);
@@ -914,6 +923,13 @@ oaph_status oaph_set_matvec_shape(int fp32, int variant) {
     OAPH_CHECK(variant >= 0 && variant < kNumZ, OAPH_ERR_ARG);
     g_zshape = variant;
   }
+  return OAPH_OK;
+}
+
+oaph_status oaph_set_matvec_fallback_shape(int fp32, int variant) {
+  using namespace oaph;
+  OAPH_CHECK(fp32 >= 0 && fp32 <= 2 && variant >= 0 && variant < kNumS1, OAPH_ERR_ARG);
+  (fp32 == 2 ? g_h1 : fp32 == 1 ? g_c1 : g_z1) = variant;
   return OAPH_OK;
 }
 
