@@ -380,7 +380,8 @@ def profiled_traffic(kernel: str):
 
 def traffic_probe() -> int:
     """Launches for the rocprofv3 counter runs: 5 FP32-operand matvecs, 5 FP64
-    matvecs, 5 multiply_re_constant copies of a 2^27-double plane (1 GiB
+    matvecs, 5 FP16-operand matvecs, 5 multiply_re_constant copies of a
+    2^27-double plane (1 GiB
     read + 1 GiB write) as the calibration kernel of known size, then 3
     complex 4096^3 products on the FP64 matrix cores and 3 magnitude and 3
     sum reductions over 2^26 complex elements."""
@@ -394,6 +395,7 @@ def traffic_probe() -> int:
     ms = C.c_double()
     check(lib.oaph_arnoldi_time_matvec(s._h, 1, 5, C.byref(ms)), "time_matvec")
     check(lib.oaph_arnoldi_time_matvec(s._h, 0, 5, C.byref(ms)), "time_matvec")
+    check(lib.oaph_arnoldi_time_matvec(s._h, 2, 5, C.byref(ms)), "time_matvec")
     api = CuProceduresApi()
     a = DeviceMatrix(1 << 14, 1 << 13, True, False)
     o = DeviceMatrix(1 << 14, 1 << 13, True, False)
@@ -576,6 +578,7 @@ def main() -> int:
                      "kernel": "k_cgemv32 (FP32 operand, FP64 accumulate, V^H y partial sums)",
                      "kernel_ms": round(t32, 5), "bytes_per_launch": bytes32},
         "roofline_f16_matvec": {"achieved": round(gbs16, 1), "frac": round(gbs16 / HBM_PEAK_GBS, 4),
+                                "traffic": profiled_traffic("k_hgemv16"),
                                 "kernel": "k_hgemv16 (FP16 operand via v_fma_mix_f32, FP32 sums, "
                                           "V^H y partial sums)",
                                 "kernel_ms": round(t16, 5), "bytes_per_launch": bytes16},
