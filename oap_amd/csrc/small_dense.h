@@ -846,7 +846,7 @@ __device__ bool aberth_roots(Mat T, int mm, double hnorm, cd* z, double* part, i
 // beta |y_{mm-1}| to est[order[t]].  Returns (the same in every thread)
 // whether every vector was accepted.
 __device__ bool inverse_vectors(Mat T, int mm, double hnorm, const cd* ev, const int* order,
-                                const int* bend, int cnt, cd* ubuf, int nwe, double* Yre,
+                                const int* bend, int t0, int cnt, cd* ubuf, int nwe, double* Yre,
                                 double* Yim, int ky, double beta, double* est, int* flag) {
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
@@ -857,7 +857,7 @@ __device__ bool inverse_vectors(Mat T, int mm, double hnorm, const cd* ev, const
   const bool act = lane < mm;
   cd* U = ubuf + (size_t)wave * (mm * (mm + 1) / 2);
   bool bad = false;
-  for (int t = wave; wave < nwe && t < cnt; t += nwe) {
+  for (int t = t0 + wave; wave < nwe && t < cnt; t += nwe) {
     const cd mu = ev[order[t]];
     const int ms = __builtin_amdgcn_readfirstlane(bend[order[t]] + 1);  // leading block
     const bool in = lane < ms;
@@ -1427,10 +1427,34 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
         int nwe = avail / (mm * (mm + 1));
         if (nwe > nthr / 64) nwe = nthr / 64;
         const int cnt = a.ky < mm ? a.ky : mm;
+        const int cntw = a.k < cnt ? a.k : cnt;
+        // the wanted vectors first; the guards (a Ritz-vector-sum restart's
+        // extra terms) only when the wanted estimates pass, i.e. when this
+        // restart can end the cycles -- otherwise their columns are zeroed
         if (nwe >= 1)
-          inv = inverse_vectors(T, mm, hnorm, ev, order, bend_s, cnt,
+          inv = inverse_vectors(T, mm, hnorm, ev, order, bend_s, 0, cntw,
                                 reinterpret_cast<cd*>(smem + 2 * mm * ld), nwe, a.Yre, a.Yim,
                                 a.ky, st->beta, est, scr_s);
+        if (inv && cnt > cntw) {
+          double mx = 0.0;
+          for (int i = 0; i < mm; ++i) mx = fmax(mx, sqrt(cabs2(ev[i])));
+          const double tabs = a.tol * (a.anorm > 0 ? a.anorm : fmax(mx, 1e-300));
+          int nc = 0;
+          for (int t = 0; t < cntw; ++t)
+            if (est[order[t]] <= tabs) ++nc;
+          if (nc >= a.k) {
+            inv = inverse_vectors(T, mm, hnorm, ev, order, bend_s, cntw, cnt,
+                                  reinterpret_cast<cd*>(smem + 2 * mm * ld), nwe, a.Yre, a.Yim,
+                                  a.ky, st->beta, est, scr_s);
+          } else {
+            for (int idx = lane; idx < mm * (cnt - cntw); idx += nthr) {
+              const int r = idx / (cnt - cntw), t = cntw + idx % (cnt - cntw);
+              a.Yre[(size_t)r * a.ky + t] = 0.0;
+              a.Yim[(size_t)r * a.ky + t] = 0.0;
+            }
+            __syncthreads();
+          }
+        }
       }
       if (!inv) {
         for (int idx = lane; idx < mm * mm; idx += nthr) {
