@@ -572,6 +572,9 @@ class CuHArnoldi {
   void setRestartGuardWeight(double w) {
     oap::check(oaph_arnoldi_set_restart_guard_weight(m_a, w), "setRestartGuardWeight");
   }
+  void setRestartKeep(uintt keep) {
+    oap::check(oaph_arnoldi_set_restart_keep(m_a, (uint32_t)keep), "setRestartKeep");
+  }
   /* extension: 1 perfect-shift Schur form where possible (default), 0 Wilkinson */
   void setSchurMethod(int method) {
     oap::check(oaph_arnoldi_set_schur_method(m_a, method), "setSchurMethod");

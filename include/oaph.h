@@ -449,6 +449,15 @@ oaph_status oaph_arnoldi_set_restart_guard(oaph_arnoldi* a, uint32_t guard);
 /* Weight of the guard Ritz vectors in a Ritz-sum restart (0..1, default
  * 1e-3; the wanted ones have weight 1).  No reference counterpart. */
 oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* a, double weight);
+/* Smallest number of Ritz vectors an exact-shift restart keeps (the rest,
+ * m minus that many, are the shifts).  0: the reference's k, plus
+ * min(nconv, (m - k) / 2) as ARPACK adjusts it; a larger value keeps more of
+ * the basis per restart (fewer matvecs on clustered spectra, more restarts);
+ * capped at m - 1.  OAPH_KEEP_AUTO (default): m / 2 in the mixed flow (the
+ * Krylov-Schur practice), 0 in FP64 -- the FP64 solver restarts as the
+ * reference does.  No reference counterpart. */
+#define OAPH_KEEP_AUTO 0xFFFFFFFFu
+oaph_status oaph_arnoldi_set_restart_keep(oaph_arnoldi* a, uint32_t keep);
 /* How the restart finds the eigenpairs of H (extension, no reference
  * counterpart; the reference triangularises H by unshifted QR steps,
  * CuTriangularH.h).  For hdim <= 64 the eigenvalues of H come first

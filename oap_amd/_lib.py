@@ -163,6 +163,7 @@ _SIGS = {
     "oaph_arnoldi_set_f16_phase": (C.c_int, [C.c_void_p, C.c_int, C.c_double]),
     "oaph_arnoldi_apply_operator": (C.c_int, [C.c_void_p, C.c_int, MP, MP]),
     "oaph_arnoldi_set_restart_guard_weight": (C.c_int, [C.c_void_p, C.c_double]),
+    "oaph_arnoldi_set_restart_keep": (C.c_int, [C.c_void_p, C.c_uint32]),
     "oaph_arnoldi_set_schur_method": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_arnoldi_set_start_vector": (C.c_int, [C.c_void_p, DP, DP, U32]),
     "oaph_arnoldi_get_stats": (C.c_int, [C.c_void_p, C.POINTER(oaph_arnoldi_stats)]),

@@ -613,6 +613,14 @@ class CuHArnoldiCallback:
         check(self._lib.oaph_arnoldi_set_restart_guard_weight(self._h, weight),
               "setRestartGuardWeight")
 
+    KEEP_AUTO = 0xFFFFFFFF
+
+    def setRestartKeep(self, keep):
+        """smallest number of Ritz vectors an exact-shift restart keeps: 0 is
+        the reference's k plus min(nconv, (m - k) / 2) (ARPACK's adjustment);
+        KEEP_AUTO (the default) is m / 2 in the mixed flow and 0 in FP64"""
+        check(self._lib.oaph_arnoldi_set_restart_keep(self._h, keep), "setRestartKeep")
+
     def setSchurMethod(self, method):
         """How the restart finds the eigenpairs of H.  2 (default): the
         Aberth eigenvalues and the wanted eigenvectors by inverse iteration,

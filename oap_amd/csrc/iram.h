@@ -63,6 +63,7 @@ struct RestartArgs {
   int phase = 0;
   int schur_method = 2;  // 2: Aberth eigenvalues + inverse iteration; 1: perfect-shift Schur form
   int warm = 0;          // 1: an earlier restart of this factorisation left its Ritz values
+  int keep = 0;          // smallest kept size of an exact-shift restart (0: k + min(nconv, p / 2))
   double* TZ = nullptr;  // 4 m x m planes (T re, im, Z re, im), leading dimension m
                          // (always given when mact > kMaxM: the general kernel's T and Z)
 };

@@ -1590,6 +1590,7 @@ __device__ void restart_body_t(const RestartArgs& a, double* smem) {
     int kk = a.k + (nconv < p2 ? nconv : p2);
     if (kk == 1 && mm >= 6) kk = mm / 2;
     else if (kk == 1 && mm > 2) kk = 2;
+    if (kk < a.keep) kk = a.keep;
     if (kk >= mm) kk = mm - 1;
     st->nconv = nconv;
     st->kk = kk;

@@ -289,6 +289,7 @@ def test_mixed_fallback_to_fp32(oracle):
     s.setMatrix(dA)
     s.setPrecision(Precision.MIXED_FP32)
     s.setF16Phase(False)  # from this start vector the FP16 phase finds the set itself
+    s.setRestartKeep(0)  # and so do FP32 cycles that keep m / 2 Ritz vectors
     re, im = np.zeros(8), np.zeros(8)
     s.setOutputsEigenvalues(re, im)
     vecs = [DeviceMatrix(1, n) for _ in range(8)]
