@@ -436,7 +436,7 @@ __device__ void sym_restart(SymLds& L, int m, int k, int which, double tol, doub
 }
 
 template <int NT, int ROWS>
-__global__ __launch_bounds__(NT) void k_batched_solve(SolveArgs a) {
+__global__ __launch_bounds__(NT, 2) void k_batched_solve(SolveArgs a) {
   constexpr int CPT = Ctx<NT>::CPT;
   __shared__ double vl[kMaxN], wl[kMaxN];
   __shared__ double red[(NT / 64) * (kMaxMB + 2)];
@@ -753,6 +753,7 @@ const Variant kVariants[] = {
     {256, 2, k_batched_solve<256, 2>},
     {512, 2, k_batched_solve<512, 2>},
     {1024, 1, k_batched_solve<1024, 1>},
+    {128, 2, k_batched_solve<128, 2>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
