@@ -251,7 +251,8 @@ def batched_c3(steps: int) -> dict:
             "plain_fp64_one_problem_at_a_time": plain,
             "ratio_batched_over_plain": round(batched_value / plain["value"], 1),
             "ms_per_solve": round(wall / steps * 1e3, 3), "kernel_ms": round(kmed, 3),
-            "variant": {"threads": shapes[variant][0], "rows_per_wave_step": shapes[variant][1]},
+            "variant": {"threads": shapes[variant][0], "rows_per_wave_step": shapes[variant][1],
+                        "row_ring_depth": shapes[variant][2]},
             "matvecs_mean": round(float(np.mean(res["matvecs"])), 1),
             "restarts_mean": round(float(np.mean(res["restarts"])), 2),
             "residual_norm": "||C x - lambda x||_2 <= 1e-10 lambda_max (= ||C||_2)",

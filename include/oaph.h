@@ -793,10 +793,13 @@ oaph_status oaph_batch_download_x(oaph_batch* b, uint32_t i, float* X);
 /* Synthetic 16-bit grey patches (k/65535, k uniform) plus three planted
  * rank-1 components of amplitude 5, generated in HBM. */
 oaph_status oaph_batch_synth_c3(oaph_batch* b, uint64_t seed);
-/* Launch variant (threads per workgroup x rows per wave step); see
- * oaph_batch_variant_info.  Default 1. */
+/* Launch variant (threads per workgroup x rows per wave step x steps of
+ * rows in flight); see oaph_batch_variant_shape.  Default 1. */
 oaph_status oaph_batch_set_variant(oaph_batch* b, int variant);
 oaph_status oaph_batch_variant_info(int variant, int* threads, int* rows);
+/* The variant's shape with the depth of its row ring (register buffers of
+ * rows: depth - 1 steps of loads in flight while one is computed). */
+oaph_status oaph_batch_variant_shape(int variant, int* threads, int* rows, int* depth);
 /* Capture the solve launch in a HIP graph and replay it while its
  * arguments are unchanged (default on). */
 oaph_status oaph_batch_set_graph(oaph_batch* b, int enable);

@@ -257,6 +257,8 @@ _SIGS = {
     "oaph_batch_synth_c3": (C.c_int, [C.c_void_p, C.c_uint64]),
     "oaph_batch_set_variant": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_batch_variant_info": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "oaph_batch_variant_shape": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_int)]),
     "oaph_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "oaph_batch_solve": (C.c_int, [C.c_void_p, U32, U32, C.c_int, C.c_double, U32, C.c_uint64]),
     "oaph_batch_autotune": (C.c_int, [C.c_void_p, U32, U32, C.c_int, C.c_double, U32, C.c_uint64,

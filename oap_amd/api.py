@@ -885,14 +885,16 @@ class CovarianceBatch:
         check(self._lib.oaph_batch_set_variant(self._h, v), "batch_set_variant")
 
     @staticmethod
-    def variants() -> list[tuple[int, int]]:
+    def variants() -> list[tuple[int, int, int]]:
+        """(threads per workgroup, rows per wave step, row-ring depth) of each
+        launch variant"""
         out, v = [], 0
         lib = load()
         while True:
-            t, r = C.c_int(), C.c_int()
-            if lib.oaph_batch_variant_info(v, C.byref(t), C.byref(r)) != 0:
+            t, r, d = C.c_int(), C.c_int(), C.c_int()
+            if lib.oaph_batch_variant_shape(v, C.byref(t), C.byref(r), C.byref(d)) != 0:
                 return out
-            out.append((t.value, r.value))
+            out.append((t.value, r.value, d.value))
             v += 1
 
     def set_graph(self, enable: bool) -> None:

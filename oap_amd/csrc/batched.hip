@@ -104,8 +104,9 @@ __device__ void block_sums(const double (&v)[NV], int nv, double* red, double* o
 // u_r = x_r . v is a segmented cross-lane sum and u_r x_r accumulates in the
 // lanes' registers.  Lanes holding the same columns are summed across the
 // segments, then the waves' partial sums are added into wl in wave order
-// (deterministic, no LDS partial buffer).
-template <int NT, int ROWS, int SEG>
+// (deterministic, no LDS partial buffer).  The rows stream through a ring of
+// D register buffers: D - 1 steps of loads in flight while one is computed.
+template <int NT, int ROWS, int SEG, int D>
 __device__ void covariance_matvec_seg(const float* __restrict__ X, uint32_t n, uint32_t ldx,
                                       uint32_t s, const double* vl, double* wl) {
   constexpr int NW = NT / 64, RPL = 64 / SEG;
@@ -166,17 +167,21 @@ __device__ void covariance_matvec_seg(const float* __restrict__ X, uint32_t n, u
     }
   };
   const uint32_t stride = NW * RPL * ROWS;
-  f4 xa[ROWS][Q4], xb[ROWS][Q4];
-  uint32_t r0 = wave * RPL * ROWS;
-  if (r0 < s) load(xa, r0);
-  while (r0 < s) {
-    if (r0 + stride < s) load(xb, r0 + stride);
-    step(xa);
-    r0 += stride;
-    if (r0 >= s) break;
-    if (r0 + stride < s) load(xa, r0 + stride);
-    step(xb);
-    r0 += stride;
+  f4 xr[D][ROWS][Q4];
+  const uint32_t r0 = wave * RPL * ROWS;
+#pragma unroll
+  for (int b = 0; b + 1 < D; ++b)
+    if (r0 + b * stride < s) load(xr[b], r0 + b * stride);
+  for (uint32_t base = r0; base < s; base += D * stride) {
+#pragma unroll
+    for (int b = 0; b < D; ++b) {
+      const uint32_t rr = base + b * stride;
+      if (rr < s) {
+        const uint32_t rn = rr + (D - 1) * stride;
+        if (rn < s) load(xr[(b + D - 1) % D], rn);
+        step(xr[b]);
+      }
+    }
   }
   // lanes with the same columns: sum across the row segments
   if (SEG < 64) {
@@ -207,13 +212,13 @@ __device__ void covariance_matvec_seg(const float* __restrict__ X, uint32_t n, u
   }
 }
 
-template <int NT, int ROWS>
+template <int NT, int ROWS, int D>
 __device__ void covariance_matvec(const float* __restrict__ X, uint32_t n, uint32_t ldx,
                                   uint32_t s, const double* vl, double* wl) {
   const uint32_t nq = ldx / 4;
-  if (nq <= 16) covariance_matvec_seg<NT, ROWS, 16>(X, n, ldx, s, vl, wl);
-  else if (nq <= 32) covariance_matvec_seg<NT, ROWS, 32>(X, n, ldx, s, vl, wl);
-  else covariance_matvec_seg<NT, ROWS, 64>(X, n, ldx, s, vl, wl);
+  if (nq <= 16) covariance_matvec_seg<NT, ROWS, 16, D>(X, n, ldx, s, vl, wl);
+  else if (nq <= 32) covariance_matvec_seg<NT, ROWS, 32, D>(X, n, ldx, s, vl, wl);
+  else covariance_matvec_seg<NT, ROWS, 64, D>(X, n, ldx, s, vl, wl);
 }
 
 struct SolveArgs {
@@ -435,7 +440,7 @@ __device__ void sym_restart(SymLds& L, int m, int k, int which, double tol, doub
   wave_fence();
 }
 
-template <int NT, int ROWS>
+template <int NT, int ROWS, int D = 2>
 __global__ __launch_bounds__(NT, 2) void k_batched_solve(SolveArgs a) {
   constexpr int CPT = Ctx<NT>::CPT;
   __shared__ double vl[kMaxN], wl[kMaxN];
@@ -473,7 +478,7 @@ __global__ __launch_bounds__(NT, 2) void k_batched_solve(SolveArgs a) {
       const long long t0 = clock64();
       for (uint32_t c = tid; c < n; c += NT) vl[c] = V[(size_t)j * n + c];
       __syncthreads();
-      covariance_matvec<NT, ROWS>(X, n, P.ldx, a.s, vl, wl);
+      covariance_matvec<NT, ROWS, D>(X, n, P.ldx, a.s, vl, wl);
       ++matvecs;
       const long long t1 = clock64();
       cyc[0] += t1 - t0;
@@ -651,7 +656,7 @@ __global__ __launch_bounds__(NT, 2) void k_batched_solve(SolveArgs a) {
     const double inv = 1.0 / sqrt(coef[0]);
     for (uint32_t c = tid; c < n; c += NT) vl[c] *= inv;
     __syncthreads();
-    covariance_matvec<NT, ROWS>(X, n, P.ldx, a.s, vl, wl);
+    covariance_matvec<NT, ROWS, D>(X, n, P.ldx, a.s, vl, wl);
     ++matvecs;
     double pr[1] = {0};
     for (uint32_t c = tid; c < n; c += NT) pr[0] += vl[c] * wl[c];
@@ -740,20 +745,25 @@ using namespace oaph::batched;
 
 namespace {
 
-// Launch shapes the autotuner chooses between: threads per workgroup and
-// rows of X each wave streams per step (bytes in flight vs registers).
+// Launch shapes the autotuner chooses between: threads per workgroup, rows
+// of X each wave streams per step and the depth of the row ring (bytes in
+// flight vs registers).
 struct Variant {
-  int threads, rows;
+  int threads, rows, depth;
   void (*kernel)(SolveArgs);
 };
 const Variant kVariants[] = {
-    {256, 1, k_batched_solve<256, 1>},
-    {128, 1, k_batched_solve<128, 1>},
-    {512, 1, k_batched_solve<512, 1>},
-    {256, 2, k_batched_solve<256, 2>},
-    {512, 2, k_batched_solve<512, 2>},
-    {1024, 1, k_batched_solve<1024, 1>},
-    {128, 2, k_batched_solve<128, 2>},
+    {256, 1, 2, k_batched_solve<256, 1>},
+    {128, 1, 2, k_batched_solve<128, 1>},
+    {512, 1, 2, k_batched_solve<512, 1>},
+    {256, 2, 2, k_batched_solve<256, 2>},
+    {512, 2, 2, k_batched_solve<512, 2>},
+    {1024, 1, 2, k_batched_solve<1024, 1>},
+    {128, 2, 2, k_batched_solve<128, 2>},
+    {256, 1, 3, k_batched_solve<256, 1, 3>},
+    {256, 1, 4, k_batched_solve<256, 1, 4>},
+    {256, 2, 3, k_batched_solve<256, 2, 3>},
+    {512, 1, 4, k_batched_solve<512, 1, 4>},
 };
 constexpr int kNumVariants = sizeof(kVariants) / sizeof(kVariants[0]);
 
@@ -983,9 +993,14 @@ oaph_status oaph_batch_set_variant(oaph_batch* b, int variant) {
 }
 
 oaph_status oaph_batch_variant_info(int variant, int* threads, int* rows) {
+  return oaph_batch_variant_shape(variant, threads, rows, nullptr);
+}
+
+oaph_status oaph_batch_variant_shape(int variant, int* threads, int* rows, int* depth) {
   OAPH_CHECK(variant >= 0 && variant < kNumVariants, OAPH_ERR_ARG);
   if (threads) *threads = kVariants[variant].threads;
   if (rows) *rows = kVariants[variant].rows;
+  if (depth) *depth = kVariants[variant].depth;
   return OAPH_OK;
 }
 
