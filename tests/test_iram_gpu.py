@@ -277,6 +277,47 @@ def test_wanted_set_over_start_vectors(oracle):
     assert wanted_set_misses(A, dA, Precision.MIXED_FP32, None, seeds) == []
 
 
+def test_restart_keep(oracle):
+    # kept size of the exact-shift restarts: with m / 2 kept both solvers
+    # return the dense top 8 of the n = 1024 C1 construction, as with the
+    # reference's rule; the mixed flow's default (KEEP_AUTO) is m / 2, the
+    # FP64 solver's the reference's k + min(nconv, (m - k) / 2) (explicit 0)
+    n, m = 1024, 40
+    ar, ai = oracle.gen_c1(n, 0x5EED0001, 1e-2, 500.0)
+    A = ar + 1j * ai
+    dA = DeviceMatrix.from_numpy(A.real, A.imag)
+    want = top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues)
+    seeds = range(10176, 10184)
+    counts = {}
+    auto = CuHArnoldiCallback.KEEP_AUTO
+    for prec, keep in ((Precision.FP64, 0), (Precision.FP64, auto), (Precision.FP64, m // 2),
+                       (Precision.MIXED_FP32, auto), (Precision.MIXED_FP32, m // 2)):
+        s = solver()
+        s.setMatrix(dA)
+        s.setPrecision(prec)
+        s.setRestartKeep(keep)
+        re, im = np.zeros(8), np.zeros(8)
+        s.setOutputsEigenvalues(re, im)
+        total, stats = 0, []
+        for seed in seeds:
+            s.setSeed(seed)
+            s.execute(m, 8, MatrixInfo(True, True, n, n))
+            match(re + 1j * im, want, 1e-10)
+            for t in range(8):
+                assert s.testOutcome(t) <= 1e-10 * np.linalg.norm(A)
+            st = s.stats()
+            stats.append((st["matvecs"], st["restarts"]))
+        counts[(prec, keep)] = stats
+    # KEEP_AUTO is the reference's rule in FP64 and m / 2 in the mixed flow:
+    # the same counts, solve for solve, and a kept size that changes them
+    assert counts[(Precision.FP64, auto)] == counts[(Precision.FP64, 0)]
+    assert counts[(Precision.MIXED_FP32, auto)] == counts[(Precision.MIXED_FP32, m // 2)]
+    assert counts[(Precision.FP64, m // 2)] != counts[(Precision.FP64, 0)]
+    s = solver()
+    with pytest.raises(OapError):
+        s.setRestartKeep(257)  # above the largest Krylov dimension
+
+
 def test_mixed_fallback_to_fp32(oracle):
     # start vector 262 on the n = 1024 C1 construction: the FP32 phase ends
     # without one wanted eigenvalue, the FP64 cycles meet it and go back to
