@@ -568,9 +568,20 @@ oaph_status ritz_pairs(oaph_arnoldi* s, uint32_t mact) {
   double* Xim = Xre + (size_t)s->ky * n;
   double* AXre = s->AX.as<double>();
   double* AXim = AXre + (size_t)k * n;
-  for (uint32_t t = 0; t < k; ++t)
-    OAPH_TRY(apply_operator(s, Xre + (size_t)t * n, Xim + (size_t)t * n, AXre + (size_t)t * n,
-                            AXim + (size_t)t * n, OAPH_TYPE_EIGENVECTOR, false));
+  const bool multi =
+      s->A && k <= kMultiMaxVectors && n % 2 == 0 && s->A->columns % 2 == 0 &&
+      ((((uintptr_t)s->A->reValues) | ((uintptr_t)s->A->imValues) | ((uintptr_t)Xre) |
+        ((uintptr_t)Xim)) & 15) == 0;
+  if (multi) {
+    // the dense operator: the k products in one pass over A
+    OAPH_TRY(launch_zgemv_multi(st, n, n, s->A->columns, s->A->reValues, s->A->imValues, Xre,
+                                Xim, n, k, AXre, AXim, n));
+    s->stats.matvecs += k;
+  } else {
+    for (uint32_t t = 0; t < k; ++t)
+      OAPH_TRY(apply_operator(s, Xre + (size_t)t * n, Xim + (size_t)t * n, AXre + (size_t)t * n,
+                              AXim + (size_t)t * n, OAPH_TYPE_EIGENVECTOR, false));
+  }
   double2* lam = s->coef.as<double2>() + 2 * (s->m + 1);
   double2* out = lam + (s->m + 1);
   double* part = s->partials.as<double>();

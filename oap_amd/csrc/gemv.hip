@@ -200,6 +200,82 @@ __global__ __launch_bounds__(256) void k_zgemv(uint32_t rows, uint32_t cols, uin
   }
 }
 
+// Y[:, t] = A X[:, t] for t < K in one pass over A (the mixed solver's FP64
+// check of its K wanted Ritz vectors: one read of the operator instead of
+// K).  A wave takes R rows, a lane one 16-byte pair of each row per step and
+// the same pair of every x vector (x from L2: the K vectors are a few MiB);
+// the R K sums are reduced across the lanes at the end.  MODE 0: complex A;
+// 2: real A (complex x and y).  Requires cols even and 16-byte aligned rows
+// and vectors.
+template <int R, int K, int MODE>
+__global__ __launch_bounds__(256) void k_zgemv_multi(uint32_t rows, uint32_t cols, uint32_t lda,
+                                                     const double* __restrict__ are,
+                                                     const double* __restrict__ aim,
+                                                     const double* __restrict__ xre,
+                                                     const double* __restrict__ xim,
+                                                     uint32_t ldx, uint32_t nvec,
+                                                     double* __restrict__ yre,
+                                                     double* __restrict__ yim, uint32_t ldy) {
+  constexpr bool CPLX = MODE == 0;
+  const int lane = threadIdx.x & 63;
+  const uint32_t r0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * R;
+  if (r0 >= rows) return;  // no barrier below
+  double sr[R][K], si[R][K];
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int t = 0; t < K; ++t) { sr[q][t] = 0.0; si[q][t] = 0.0; }
+  const v2d* ar2[R];
+  const v2d* ai2[R];
+#pragma unroll
+  for (int q = 0; q < R; ++q) {
+    const uint32_t r = r0 + q < rows ? r0 + q : rows - 1;  // clamped rows are dropped
+    ar2[q] = reinterpret_cast<const v2d*>(are + (size_t)r * lda);
+    ai2[q] = CPLX ? reinterpret_cast<const v2d*>(aim + (size_t)r * lda) : nullptr;
+  }
+  const uint32_t npair = cols >> 1;
+  for (uint32_t p = lane; p < npair; p += 64) {
+    v2d a[R], b[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+      a[q] = __builtin_nontemporal_load(ar2[q] + p);
+      if (CPLX) b[q] = __builtin_nontemporal_load(ai2[q] + p);
+    }
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      if ((uint32_t)t < nvec) {  // uniform
+        const v2d xa = reinterpret_cast<const v2d*>(xre + (size_t)t * ldx)[p];
+        const v2d xb = reinterpret_cast<const v2d*>(xim + (size_t)t * ldx)[p];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+          sr[q][t] = fma(a[q].x, xa.x, sr[q][t]);
+          sr[q][t] = fma(a[q].y, xa.y, sr[q][t]);
+          si[q][t] = fma(a[q].x, xb.x, si[q][t]);
+          si[q][t] = fma(a[q].y, xb.y, si[q][t]);
+          if (CPLX) {
+            sr[q][t] = fma(-b[q].x, xb.x, sr[q][t]);
+            sr[q][t] = fma(-b[q].y, xb.y, sr[q][t]);
+            si[q][t] = fma(b[q].x, xa.x, si[q][t]);
+            si[q][t] = fma(b[q].y, xa.y, si[q][t]);
+          }
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < R; ++q)
+#pragma unroll
+    for (int t = 0; t < K; ++t) {
+      if ((uint32_t)t < nvec) {
+        const double vr = wave_sum(sr[q][t]), vi = wave_sum(si[q][t]);
+        if (lane == 0 && r0 + q < rows) {
+          yre[(size_t)t * ldy + r0 + q] = vr;
+          yim[(size_t)t * ldy + r0 + q] = vi;
+        }
+      }
+    }
+}
+
 // any alignment: one wave per row, scalar loads
 template <int MODE>
 __global__ __launch_bounds__(256) void k_zgemv_scalar(uint32_t rows, uint32_t cols, uint32_t lda,
@@ -788,6 +864,28 @@ oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t l
     else
       hipLaunchKernelGGL(k_zgemv_scalar<2>, grid, dim3(256), 0, s, rows, cols, lda, are, aim, xre, xim, yre, yim);
   }
+  OAPH_HIP(hipGetLastError());
+  return OAPH_OK;
+}
+
+oaph_status launch_zgemv_multi(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
+                               const double* are, const double* aim, const double* xre,
+                               const double* xim, uint32_t ldx, uint32_t nvec, double* yre,
+                               double* yim, uint32_t ldy) {
+  OAPH_CHECK(nvec >= 1 && nvec <= kMultiMaxVectors, OAPH_ERR_UNSUPPORTED);
+  OAPH_CHECK(xim != nullptr && yim != nullptr, OAPH_ERR_PLANE);
+  const bool ok = (cols % 2 == 0) && (lda % 2 == 0) && (ldx % 2 == 0) &&
+                  ((((uintptr_t)are) | ((uintptr_t)aim) | ((uintptr_t)xre) | ((uintptr_t)xim)) &
+                   15) == 0;
+  OAPH_CHECK(ok, OAPH_ERR_UNSUPPORTED);
+  constexpr int R = 2;
+  const dim3 grid((rows + 4 * R - 1) / (4 * R));
+  if (aim)
+    hipLaunchKernelGGL((k_zgemv_multi<R, kMultiMaxVectors, 0>), grid, dim3(256), 0, s, rows, cols,
+                       lda, are, aim, xre, xim, ldx, nvec, yre, yim, ldy);
+  else
+    hipLaunchKernelGGL((k_zgemv_multi<R, kMultiMaxVectors, 2>), grid, dim3(256), 0, s, rows, cols,
+                       lda, are, nullptr, xre, xim, ldx, nvec, yre, yim, ldy);
   OAPH_HIP(hipGetLastError());
   return OAPH_OK;
 }

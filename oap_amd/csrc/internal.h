@@ -122,6 +122,13 @@ struct MvProj {
 oaph_status launch_zgemv(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
                          const double* are, const double* aim, const double* xre,
                          const double* xim, double* yre, double* yim, MvProj* pj = nullptr);
+// Y[:, t] = A X[:, t], t < nvec <= kMultiMaxVectors, in one pass over A (FP64;
+// A complex or real, x and y complex; vector t at x + t ldx, y + t ldy)
+constexpr uint32_t kMultiMaxVectors = 8;
+oaph_status launch_zgemv_multi(hipStream_t s, uint32_t rows, uint32_t cols, uint32_t lda,
+                               const double* are, const double* aim, const double* xre,
+                               const double* xim, uint32_t ldx, uint32_t nvec, double* yre,
+                               double* yim, uint32_t ldy);
 // out(x, y) = sum_k A[y][k] B[x][k] (A * B^T, B read in place), W x H, K terms
 // the matrix-core complex product (gemm.hip): 128 x 128 tiles, for W, H >= 256
 bool gemm_mfma_eligible(uint32_t W, uint32_t H, uint32_t K);

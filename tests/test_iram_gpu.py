@@ -168,6 +168,37 @@ def test_mixed_check_on_c1_n1024(oracle, f16):
     ev = re + 1j * im
     match(ev, top_k(np.linalg.eigvals(A), 8, ArnUtils.SortLargestValues), 1e-10)
     check_residuals(s, A, ev, vecs, 8)
+    if st["fp64_polish"] == 0:
+        check_reported_residuals(s, A, ev, vecs, 8)
+
+
+def check_reported_residuals(s, A, ev, vecs, k):
+    # the check's residuals (the k FP64 products in one pass over A) are the
+    # returned pairs' residuals, recomputed here from the vectors
+    fro = np.linalg.norm(A)
+    for t in range(k):
+        x = vecs[t].to_complex()[:, 0]
+        r = np.linalg.norm(A @ x - ev[t] * x) / np.linalg.norm(x)
+        assert abs(s.testOutcome(t) - r) <= 1e-6 * r + 1e-14 * fro, (t, s.testOutcome(t), r)
+
+
+@pytest.mark.parametrize("cplx", [True, False])
+def test_mixed_check_products_of_the_operator(cplx):
+    # the FP64 check's products in one pass over a complex or a real operator
+    # (entries not FP32 values, so the check runs on FP32-phase vectors)
+    n = 768
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((n, n)) * 1e-2 / np.sqrt(n)
+    if cplx:
+        A = A + 1j * rng.standard_normal((n, n)) * 1e-2 / np.sqrt(n)
+    np.fill_diagonal(A, np.exp(-np.arange(n) / 40.0) + 0.1 * np.pi)
+    s, ev, vecs = run_dense(A, 6, 32, precision=Precision.MIXED_FP32)
+    st = s.stats()
+    assert st["matvecs_low"] > 0
+    match(ev, top_k(np.linalg.eigvals(A), 6, ArnUtils.SortLargestValues), 1e-10)
+    check_residuals(s, A, ev, vecs, 6)
+    if st["fp64_polish"] == 0:
+        check_reported_residuals(s, A, ev, vecs, 6)
 
 
 def test_mixed_check_polishes_when_fp32_vectors_fail():
