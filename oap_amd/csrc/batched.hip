@@ -440,6 +440,9 @@ __device__ void sym_restart(SymLds& L, int m, int k, int which, double tol, doub
   wave_fence();
 }
 
+// One workgroup per problem.  The second launch bound (two waves per SIMD)
+// caps the registers at 256, so two 256-thread workgroups share a CU: the
+// two-row variant needed 280 without it and ran one workgroup per CU.
 template <int NT, int ROWS, int D = 2>
 __global__ __launch_bounds__(NT, 2) void k_batched_solve(SolveArgs a) {
   constexpr int CPT = Ctx<NT>::CPT;
