@@ -61,3 +61,20 @@ def test_cpp_facade_example_is_built():
     # the facade header compiles as C++17 against the C ABI only
     exe = ROOT / "tests" / "cpp" / "facade_example"
     assert exe.exists()
+
+
+def test_batch_variant_shapes():
+    # the batched solver's launch variants (no GPU needed to list them):
+    # threads per workgroup, rows of X per wave step, depth of the row ring
+    from oap_amd.api import CovarianceBatch
+    shapes = CovarianceBatch.variants()
+    assert len(shapes) >= 7 and len(set(shapes)) == len(shapes)
+    for threads, rows, depth in shapes:
+        assert threads in (128, 256, 512, 1024) and rows in (1, 2) and depth in (2, 3, 4)
+    import ctypes as C
+    from oap_amd._lib import load
+    lib = load()
+    t, r = C.c_int(), C.c_int()
+    assert lib.oaph_batch_variant_info(0, C.byref(t), C.byref(r)) == 0
+    assert (t.value, r.value) == shapes[0][:2]
+    assert lib.oaph_batch_variant_shape(len(shapes), None, None, None) != 0
