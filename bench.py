@@ -381,7 +381,8 @@ def profiled_traffic(kernel: str):
 
 def traffic_probe() -> int:
     """Launches for the rocprofv3 counter runs: 5 FP32-operand matvecs, 5 FP64
-    matvecs, 5 FP16-operand matvecs, 5 multiply_re_constant copies of a
+    matvecs, 5 FP16-operand matvecs, one mixed solve (its FP64 check is the
+    multi-vector product), 5 multiply_re_constant copies of a
     2^27-double plane (1 GiB
     read + 1 GiB write) as the calibration kernel of known size, then 3
     complex 4096^3 products on the FP64 matrix cores and 3 magnitude and 3
@@ -397,6 +398,11 @@ def traffic_probe() -> int:
     check(lib.oaph_arnoldi_time_matvec(s._h, 1, 5, C.byref(ms)), "time_matvec")
     check(lib.oaph_arnoldi_time_matvec(s._h, 0, 5, C.byref(ms)), "time_matvec")
     check(lib.oaph_arnoldi_time_matvec(s._h, 2, 5, C.byref(ms)), "time_matvec")
+    # one mixed solve: its FP64 check launches k_zgemv_multi (the 8 wanted
+    # vectors' products in one pass over A)
+    from oap_amd.api import MatrixInfo
+    s.setSeed(1000)
+    s.execute(M, K, MatrixInfo(True, True, N, N), raise_on_not_converged=False)
     api = CuProceduresApi()
     a = DeviceMatrix(1 << 14, 1 << 13, True, False)
     o = DeviceMatrix(1 << 14, 1 << 13, True, False)
