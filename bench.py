@@ -477,11 +477,11 @@ def main() -> int:
     t16 = ms.value
 
     # the mixed flow's restart schedule (Ritz-vector-sum restarts at 2e-3 and
-    # 1e-6 of the largest Ritz value, m / 2 Ritz vectors kept by the exact-shift
+    # 1e-7 of the largest Ritz value, m / 2 Ritz vectors kept by the exact-shift
     # restarts) with every cycle in FP64: what the schedule gains apart from
     # precision
     ssc, resc, imsc = make_solver(dA, Precision.FP64)
-    ssc.setRestartStages([2e-3, 1e-6])
+    ssc.setRestartStages([2e-3, 1e-7])
     ssc.setRestartKeep(M // 2)
     log("FP64 solver, the mixed flow's restart schedule:")
     goodsc, wallsc, stsc, medsc, misssc, errsc = solve_series(ssc, resc, imsc, args.steps,
@@ -570,7 +570,7 @@ def main() -> int:
             "matvecs_median": int(np.median([x["matvecs"] for x in stsc])),
             "ratio_mixed_over_this": round(epsmx / (goodsc / wallsc), 3) if goodsc > 0 else None,
             "note": "every cycle FP64, with the mixed flow's Ritz-vector-sum restarts at 2e-3 and "
-                    "1e-6 and its m/2 kept Ritz vectors per exact-shift restart: the mixed flow's "
+                    "1e-7 and its m/2 kept Ritz vectors per exact-shift restart: the mixed flow's "
                     "gain from precision alone is ratio_mixed_over_this"},
         "mixed_bf16x3_projection": {
             "value": round(goodbf / wallbf, 4), "ms_per_step": round(wallbf / args.steps * 1e3, 3),

@@ -473,7 +473,7 @@ oaph_status oaph_arnoldi_set_restart_keep(oaph_arnoldi* a, uint32_t keep);
  * to rounding either way. */
 oaph_status oaph_arnoldi_set_schur_method(oaph_arnoldi* a, int method);
 /* OAPH_PREC_MIXED_FP32: residual tolerance of the FP32 phase, relative to
- * the largest Ritz value (default 1e-6) */
+ * the largest Ritz value (default 1e-7) */
 oaph_status oaph_arnoldi_set_mixed_tolerance(oaph_arnoldi* a, double tol_low);
 /* OAPH_PREC_MIXED_FP32, what follows the FP32 phase (no reference
  * counterpart).  1 (default): FP32-operand cycles go on from the Ritz-vector

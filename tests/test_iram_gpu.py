@@ -362,6 +362,7 @@ def test_mixed_fallback_to_fp32(oracle):
     s.setPrecision(Precision.MIXED_FP32)
     s.setF16Phase(False)  # from this start vector the FP16 phase finds the set itself
     s.setRestartKeep(0)  # and so do FP32 cycles that keep m / 2 Ritz vectors
+    s.setMixedTolerance(1e-6)  # and an FP32 phase that goes on to 1e-7
     re, im = np.zeros(8), np.zeros(8)
     s.setOutputsEigenvalues(re, im)
     vecs = [DeviceMatrix(1, n) for _ in range(8)]
