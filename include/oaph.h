@@ -454,8 +454,9 @@ oaph_status oaph_arnoldi_set_restart_guard_weight(oaph_arnoldi* a, double weight
  * min(nconv, (m - k) / 2) as ARPACK adjusts it; a larger value keeps more of
  * the basis per restart (fewer matvecs on clustered spectra, more restarts);
  * capped at m - 1.  OAPH_KEEP_AUTO (default): m / 2 in the mixed flow (the
- * Krylov-Schur practice), 0 in FP64 -- the FP64 solver restarts as the
- * reference does.  No reference counterpart. */
+ * Krylov-Schur practice), 0 in FP64 and for an operator whose FP32 copy is
+ * exact (all cycles FP32) -- those restart as the reference does.  No
+ * reference counterpart. */
 #define OAPH_KEEP_AUTO 0xFFFFFFFFu
 oaph_status oaph_arnoldi_set_restart_keep(oaph_arnoldi* a, uint32_t keep);
 /* How the restart finds the eigenpairs of H (extension, no reference

@@ -618,7 +618,8 @@ class CuHArnoldiCallback:
     def setRestartKeep(self, keep):
         """smallest number of Ritz vectors an exact-shift restart keeps: 0 is
         the reference's k plus min(nconv, (m - k) / 2) (ARPACK's adjustment);
-        KEEP_AUTO (the default) is m / 2 in the mixed flow and 0 in FP64"""
+        KEEP_AUTO (the default) is m / 2 in the mixed flow and 0 in FP64 and
+        for an operator whose FP32 copy is exact"""
         check(self._lib.oaph_arnoldi_set_restart_keep(self._h, keep), "setRestartKeep")
 
     def setSchurMethod(self, method):

@@ -71,7 +71,7 @@ struct oaph_arnoldi {
   double guard_weight = 1e-3;
   // smallest kept size of the exact-shift restarts (0: the reference's k plus
   // ARPACK's min(nconv, (m - k) / 2) adjustment; OAPH_KEEP_AUTO: m / 2 in the
-  // mixed flow, 0 otherwise).  The C1 operator (k = 8, m = 48, 24 start
+  // mixed flow's phases, 0 otherwise).  The C1 operator (k = 8, m = 48, 24 start
   // vectors), mixed flow: kept 0 -> 72 ms and 668 matvecs per solve, 16 ->
   // 60 ms, 24 -> 46 ms and 425, 32 -> 47 ms, 40 -> 56 ms (more restarts);
   // the eigenvalues near the k-th stay in the basis instead of being
@@ -880,7 +880,6 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
     mixed = false;
     s->stats.mixed_declined = 1;
   }
-  s->keep_active = s->keep != OAPH_KEEP_AUTO ? s->keep : mixed ? s->m / 2 : 0;
   OAPH_TRY(start_vector(s));
   OAPH_TRY(normalize_v0(s));
   uint32_t cycles = 0;
@@ -889,6 +888,12 @@ oaph_status oaph_arnoldi_execute(oaph_arnoldi* s, uint32_t hdim, uint32_t wanted
   // the FP32-operand matvec (FP64 accumulation) is then as accurate as the
   // FP64 one and every cycle runs on it, with no FP64 restart
   const bool all_low = mixed && s->a32_exact;
+  // m / 2 kept where the cycles run in phases ended by Ritz-sum restarts
+  // (the mixed flow); continuing restarts to the final tolerance (FP64, or
+  // an operator whose FP32 copy is exact) keep the reference's rule: there
+  // a large kept size was measured to stall (FP64 at n = 1024: 1.3-1.9x the
+  // matvecs; the FP32-valued CDR operators, m = 24: 22.8 vs 19.0 ms)
+  s->keep_active = s->keep != OAPH_KEEP_AUTO ? s->keep : (mixed && !all_low) ? s->m / 2 : 0;
   // restart from the sum of the wanted Ritz vectors (a fresh Arnoldi start)
   auto restart_from_ritz_sum = [&](uint32_t count) -> oaph_status {
     // after a lucky breakdown the restart kernel worked on the invariant
